@@ -1,0 +1,623 @@
+/*
+ * bader.c -- CPU restatement of critic2's near-grid Bader integration.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Loaded by tests/, smoke() and the
+ * cpu_baseline leg of bench.py as the checker; never shipped as the product.
+ *
+ * Follows /root/reference/src/bader@proc.f90 statement by statement:
+ *   bader_integrate       bader@proc.f90:80-234
+ *   refine_edge           bader@proc.f90:300-422
+ *   max_neargrid          bader@proc.f90:427-450
+ *   step_neargrid         bader@proc.f90:455-494   (dr is SAVE state)
+ *   step_ongrid           bader@proc.f90:500-527
+ *   rho_grad_dir          bader@proc.f90:532-567
+ *   is_max                bader@proc.f90:571-597
+ *   pbc/rho_val/volnum_val bader@proc.f90:601-660
+ *   assign_surrounding_pts/known_volnum_ongrid bader@proc.f90:665-726
+ *   is_vol_edge           bader@proc.f90:730-752
+ *   reassign_volnum_ongrid2 bader@proc.f90:756-771
+ *   matinv (dgetrf2/dgetri/dtrti2) tools_math@proc.f90:1203-1235,
+ *     src/oldlibs/dgetrf2.f, dgetri.f, dtrti2.f
+ *   identify_atom/nearest_atom crystalmod@env.f90:516-614
+ *   are_lclose/shortest   crystalmod@proc.f90:1054-1133
+ *
+ * Indices are 0-based; the iteration order (k innermost, then j, then i) and
+ * every comparison, rounding and accumulation order is kept.  Compile without
+ * FP contraction (-ffp-contract=off): the reference build emits no FMA.
+ */
+#include "c2oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+typedef struct {
+  const double *f;
+  int n[3];
+  int64_t N;
+  double car2lat[9];
+  double lid[27];
+  int32_t *volnum;
+  int32_t *known;
+  int *path; /* 3*cap */
+  int64_t pcap;
+  int64_t pnum;
+  double dr[3];
+  int nbasin;
+  int64_t nsteps;
+} bstate;
+
+static inline int wrap(int p, int n) {
+  /* pbc: repeated +n / -n until 1..n (0..n-1 here) */
+  while (p < 0) p += n;
+  while (p >= n) p -= n;
+  return p;
+}
+static inline int64_t lin(const bstate *s, int i, int j, int k) {
+  return (int64_t)i + (int64_t)s->n[0] * ((int64_t)j + (int64_t)s->n[1] * k);
+}
+static inline double rho_val(const bstate *s, int i, int j, int k) {
+  return s->f[lin(s, wrap(i, s->n[0]), wrap(j, s->n[1]), wrap(k, s->n[2]))];
+}
+static inline int32_t volnum_val(const bstate *s, int i, int j, int k) {
+  return s->volnum[lin(s, wrap(i, s->n[0]), wrap(j, s->n[1]), wrap(k, s->n[2]))];
+}
+static inline long fnint(double x) { return lround(x); } /* half away from 0 */
+
+/* rho_grad_dir, bader@proc.f90:532-567 */
+static void rho_grad_dir(const bstate *s, const int p[3], double res[3]) {
+  int p1 = p[0], p2 = p[1], p3 = p[2];
+  double rho000 = rho_val(s, p1, p2, p3);
+  double rho001 = rho_val(s, p1, p2, p3 + 1);
+  double rho010 = rho_val(s, p1, p2 + 1, p3);
+  double rho100 = rho_val(s, p1 + 1, p2, p3);
+  double rho00_1 = rho_val(s, p1, p2, p3 - 1);
+  double rho_100 = rho_val(s, p1 - 1, p2, p3);
+  double rho0_10 = rho_val(s, p1, p2 - 1, p3);
+  double gl[3], gc[3];
+  gl[0] = (rho100 - rho_100) / 2.0;
+  gl[1] = (rho010 - rho0_10) / 2.0;
+  gl[2] = (rho001 - rho00_1) / 2.0;
+  if (rho100 < rho000 && rho_100 < rho000) gl[0] = 0.0;
+  if (rho010 < rho000 && rho0_10 < rho000) gl[1] = 0.0;
+  if (rho001 < rho000 && rho00_1 < rho000) gl[2] = 0.0;
+  /* rho_grad_car = matmul(rho_grad_lat, car2lat): gc(j) = sum_i gl(i)*c(i,j) */
+  const double *c = s->car2lat;
+  for (int j = 0; j < 3; j++) {
+    double acc = 0.0;
+    for (int i = 0; i < 3; i++) acc = acc + gl[i] * c[i + 3 * j];
+    gc[j] = acc;
+  }
+  /* res = matmul(car2lat, rho_grad_car): res(i) = sum_j c(i,j)*gc(j) */
+  for (int i = 0; i < 3; i++) {
+    double acc = 0.0;
+    for (int j = 0; j < 3; j++) acc = acc + c[i + 3 * j] * gc[j];
+    res[i] = acc;
+  }
+}
+
+/* is_max, bader@proc.f90:571-597 */
+static int is_max(const bstate *s, const int p[3]) {
+  double rho = rho_val(s, p[0], p[1], p[2]);
+  for (int d1 = -1; d1 <= 1; d1++)
+    for (int d2 = -1; d2 <= 1; d2++)
+      for (int d3 = -1; d3 <= 1; d3++)
+        if (rho_val(s, p[0] + d1, p[1] + d2, p[2] + d3) > rho) return 0;
+  return 1;
+}
+
+/* step_ongrid, bader@proc.f90:500-527 */
+static void step_ongrid(const bstate *s, int p[3]) {
+  int pm[3] = {p[0], p[1], p[2]};
+  double rho_ctr = rho_val(s, p[0], p[1], p[2]);
+  double rho_max = rho_ctr;
+  for (int d1 = -1; d1 <= 1; d1++)
+    for (int d2 = -1; d2 <= 1; d2++)
+      for (int d3 = -1; d3 <= 1; d3++) {
+        int pt0 = p[0] + d1, pt1 = p[1] + d2, pt2 = p[2] + d3;
+        double rho_tmp = rho_val(s, pt0, pt1, pt2);
+        rho_tmp = rho_ctr + (rho_tmp - rho_ctr) * s->lid[(d1 + 1) + 3 * (d2 + 1) + 9 * (d3 + 1)];
+        if (rho_tmp > rho_max) {
+          rho_max = rho_tmp;
+          pm[0] = pt0; pm[1] = pt1; pm[2] = pt2;
+        }
+      }
+  p[0] = wrap(pm[0], s->n[0]);
+  p[1] = wrap(pm[1], s->n[1]);
+  p[2] = wrap(pm[2], s->n[2]);
+}
+
+/* step_neargrid, bader@proc.f90:455-494 */
+static void step_neargrid(bstate *s, int p[3]) {
+  double g[3];
+  int pm[3];
+  s->nsteps++;
+  if (s->pnum == 1) { s->dr[0] = s->dr[1] = s->dr[2] = 0.0; }
+  rho_grad_dir(s, p, g);
+  double gmax = fabs(g[0]);
+  if (fabs(g[1]) > gmax) gmax = fabs(g[1]);
+  if (fabs(g[2]) > gmax) gmax = fabs(g[2]);
+  if (gmax < 1e-30) {
+    s->dr[0] = s->dr[1] = s->dr[2] = 0.0;
+    if (is_max(s, p)) return;
+    pm[0] = p[0]; pm[1] = p[1]; pm[2] = p[2];
+    step_ongrid(s, pm);
+  } else {
+    double coeff = 1.0 / gmax;
+    for (int c = 0; c < 3; c++) {
+      g[c] = coeff * g[c];
+      long ng = fnint(g[c]);
+      pm[c] = p[c] + (int)ng;
+      s->dr[c] = (s->dr[c] + g[c]) - (double)ng;
+      long nd = fnint(s->dr[c]);
+      pm[c] = pm[c] + (int)nd;
+      s->dr[c] = s->dr[c] - (double)nd;
+    }
+  }
+  s->known[lin(s, p[0], p[1], p[2])] = 1;
+  pm[0] = wrap(pm[0], s->n[0]);
+  pm[1] = wrap(pm[1], s->n[1]);
+  pm[2] = wrap(pm[2], s->n[2]);
+  if (s->known[lin(s, pm[0], pm[1], pm[2])] == 1) {
+    pm[0] = p[0]; pm[1] = p[1]; pm[2] = p[2];
+    step_ongrid(s, pm);
+    s->dr[0] = s->dr[1] = s->dr[2] = 0.0;
+  }
+  p[0] = pm[0]; p[1] = pm[1]; p[2] = pm[2];
+}
+
+static void path_push(bstate *s, const int p[3]) {
+  if (s->pnum >= s->pcap) {
+    s->pcap = 2 * (s->pnum + 1);
+    s->path = (int *)realloc(s->path, sizeof(int) * 3 * s->pcap);
+  }
+  s->path[3 * s->pnum + 0] = p[0];
+  s->path[3 * s->pnum + 1] = p[1];
+  s->path[3 * s->pnum + 2] = p[2];
+  s->pnum++;
+}
+
+/* max_neargrid, bader@proc.f90:427-450 */
+static void max_neargrid(bstate *s, int p[3]) {
+  s->pnum = 0;
+  path_push(s, p); /* pnum = 1 */
+  for (;;) {
+    step_neargrid(s, p);
+    const int *last = &s->path[3 * (s->pnum - 1)];
+    if (p[0] == last[0] && p[1] == last[1] && p[2] == last[2]) break;
+    path_push(s, p);
+    if (s->known[lin(s, p[0], p[1], p[2])] == 2) break;
+  }
+}
+
+/* known_volnum_ongrid, bader@proc.f90:705-726 */
+static void known_volnum_ongrid(bstate *s, const int p[3]) {
+  int p1 = p[0], p2 = p[1], p3 = p[2];
+  int32_t v = volnum_val(s, p1, p2, p3);
+  if (v <= 0) return;
+  if (volnum_val(s, p1, p2, p3 + 1) != v) return;
+  if (volnum_val(s, p1, p2, p3 - 1) != v) return;
+  if (volnum_val(s, p1, p2 + 1, p3) != v) return;
+  if (volnum_val(s, p1, p2 - 1, p3) != v) return;
+  if (volnum_val(s, p1 + 1, p2, p3) != v) return;
+  if (volnum_val(s, p1 - 1, p2, p3) != v) return;
+  s->known[lin(s, p1, p2, p3)] = 2;
+}
+
+/* assign_surrounding_pts, bader@proc.f90:665-700 */
+static void assign_surrounding_pts(bstate *s, const int p[3]) {
+  static const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0},
+                                {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int t = 0; t < 6; t++) {
+    int pt[3] = {wrap(p[0] + off[t][0], s->n[0]), wrap(p[1] + off[t][1], s->n[1]),
+                 wrap(p[2] + off[t][2], s->n[2])};
+    if (s->known[lin(s, pt[0], pt[1], pt[2])] != 2) known_volnum_ongrid(s, pt);
+  }
+}
+
+/* is_vol_edge, bader@proc.f90:730-752 */
+static int is_vol_edge(const bstate *s, const int p[3]) {
+  int32_t v = s->volnum[lin(s, p[0], p[1], p[2])];
+  for (int d1 = -1; d1 <= 1; d1++)
+    for (int d2 = -1; d2 <= 1; d2++)
+      for (int d3 = -1; d3 <= 1; d3++) {
+        int32_t vn = volnum_val(s, p[0] + d1, p[1] + d2, p[2] + d3);
+        if (abs(vn) != abs(v)) return 1;
+      }
+  return 0;
+}
+
+/* reassign_volnum_ongrid2, bader@proc.f90:756-771 */
+static void reassign_volnum_ongrid2(bstate *s, const int p[3]) {
+  for (int d1 = -1; d1 <= 1; d1++)
+    for (int d2 = -1; d2 <= 1; d2++)
+      for (int d3 = -1; d3 <= 1; d3++)
+        s->known[lin(s, wrap(p[0] + d1, s->n[0]), wrap(p[1] + d2, s->n[1]),
+                     wrap(p[2] + d3, s->n[2]))] = 0;
+}
+
+/* refine_edge, bader@proc.f90:300-422.  Returns num_reassign, or -1 on the
+ * "should be no new maxima in edge refinement" fatal error. */
+static int64_t refine_edge(bstate *s, int ref_itrs) {
+  const int n1 = s->n[0], n2 = s->n[1], n3 = s->n[2];
+  int p[3];
+  if (ref_itrs == 1) {
+    for (int i = 0; i < n1; i++)
+      for (int j = 0; j < n2; j++)
+        for (int k = 0; k < n3; k++) {
+          int64_t id = lin(s, i, j, k);
+          p[0] = i; p[1] = j; p[2] = k;
+          if (s->volnum[id] == s->nbasin + 1) continue;
+          if (is_vol_edge(s, p) && !is_max(s, p)) {
+            s->volnum[id] = -s->volnum[id];
+            s->known[id] = 0;
+            reassign_volnum_ongrid2(s, p);
+          }
+        }
+  } else {
+    for (int i = 0; i < n1; i++)
+      for (int j = 0; j < n2; j++)
+        for (int k = 0; k < n3; k++) {
+          int64_t id = lin(s, i, j, k);
+          p[0] = i; p[1] = j; p[2] = k;
+          if (s->volnum[id] == s->nbasin + 1) continue;
+          if (s->volnum[id] < 0 && s->known[id] != -1) {
+            int pt[3] = {0, 0, 0};
+            for (int d1 = -1; d1 <= 1; d1++)
+              for (int d2 = -1; d2 <= 1; d2++)
+                for (int d3 = -1; d3 <= 1; d3++) {
+                  pt[0] = wrap(i + d1, n1); pt[1] = wrap(j + d2, n2); pt[2] = wrap(k + d3, n3);
+                  int64_t it = lin(s, pt[0], pt[1], pt[2]);
+                  if (s->volnum[it] == s->nbasin + 1) continue;
+                  if (!is_max(s, pt)) {
+                    if (s->volnum[it] > 0) {
+                      s->volnum[it] = -s->volnum[it];
+                      s->known[it] = -1;
+                    } else if (s->volnum[it] < 0 && s->known[it] == 0) {
+                      s->known[it] = -2;
+                    }
+                  }
+                }
+            /* pt keeps its last value, p + (1,1,1) wrapped (bader@proc.f90:359) */
+            if (s->known[lin(s, pt[0], pt[1], pt[2])] != -2)
+              s->volnum[id] = abs(s->volnum[id]);
+          }
+        }
+    for (int i = 0; i < n1; i++)
+      for (int j = 0; j < n2; j++)
+        for (int k = 0; k < n3; k++) {
+          if (s->volnum[lin(s, i, j, k)] < 0) {
+            for (int d1 = -1; d1 <= 1; d1++)
+              for (int d2 = -1; d2 <= 1; d2++)
+                for (int d3 = -1; d3 <= 1; d3++) {
+                  int64_t it = lin(s, wrap(i + d1, n1), wrap(j + d2, n2), wrap(k + d3, n3));
+                  if (s->known[it] == 2) s->known[it] = 0;
+                }
+          }
+        }
+  }
+
+  int64_t num_reassign = 0;
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++)
+      for (int k = 0; k < n3; k++) {
+        int64_t id = lin(s, i, j, k);
+        int32_t bvolnum = s->volnum[id];
+        if (bvolnum < 0) {
+          p[0] = i; p[1] = j; p[2] = k;
+          max_neargrid(s, p);
+          int32_t pv = s->volnum[lin(s, p[0], p[1], p[2])];
+          if (pv < 0 || pv > s->nbasin) return -1;
+          s->volnum[id] = pv;
+          if (abs(bvolnum) != pv) {
+            num_reassign++;
+            s->volnum[id] = -pv;
+          }
+          for (int64_t l = 0; l < s->pnum; l++) {
+            int64_t it = lin(s, s->path[3 * l], s->path[3 * l + 1], s->path[3 * l + 2]);
+            if (s->known[it] != 2) s->known[it] = 0;
+          }
+        }
+      }
+  return num_reassign;
+}
+
+/* ---- crystal geometry used by attractor assignment ---- */
+
+static void x2c_apply(const double m[9], const double x[3], double out[3]) {
+  /* matmul(m_x2c, x): out(i) = sum_j m(i,j)*x(j) */
+  for (int i = 0; i < 3; i++) {
+    double acc = 0.0;
+    for (int j = 0; j < 3; j++) acc = acc + m[i + 3 * j] * x[j];
+    out[i] = acc;
+  }
+}
+
+/* Shortest lattice-translated distance of a fractional difference vector.
+ * Brute force over translations -2..2 after folding by nint (exact minimum
+ * for the reduced cells used here). */
+static double lattice_dist(const double x2c[9], const double xd[3]) {
+  double x0[3];
+  for (int c = 0; c < 3; c++) x0[c] = xd[c] - (double)lround(xd[c]);
+  double best = DBL_MAX;
+  for (int a = -2; a <= 2; a++)
+    for (int b = -2; b <= 2; b++)
+      for (int c = -2; c <= 2; c++) {
+        double xt[3] = {x0[0] + a, x0[1] + b, x0[2] + c}, xc[3];
+        x2c_apply(x2c, xt, xc);
+        double d = sqrt(xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
+        if (d < best) best = d;
+      }
+  return best;
+}
+
+/* identify_atom(x, icrd_crys, distmax): nearest atom id (1-based) within
+ * distmax, first one in list order on exact ties; 0 if none. */
+static int identify_atom(const orc_geom *g, const double x[3], double distmax) {
+  int best = 0;
+  double bd = DBL_MAX;
+  for (int a = 0; a < g->nat; a++) {
+    double xd[3] = {x[0] - g->xat[3 * a], x[1] - g->xat[3 * a + 1], x[2] - g->xat[3 * a + 2]};
+    double d = lattice_dist(g->x2c, xd);
+    if (d < bd) { bd = d; best = a + 1; }
+  }
+  if (best == 0) return 0;
+  if (distmax < 1.0) {
+    return (bd <= distmax) ? best : 0; /* list_near_atoms up2d */
+  }
+  return (bd > distmax) ? 0 : best;
+}
+
+static int are_lclose(const orc_geom *g, const double x0[3], const double x1[3], double eps) {
+  double xd[3] = {x0[0] - x1[0], x0[1] - x1[1], x0[2] - x1[2]};
+  return lattice_dist(g->x2c, xd) < eps;
+}
+
+/* ---- 3x3 matinv: dgetrf2 + dgetri (dtrti2 + unblocked loop) ---- */
+#define A_(r, c) a[(r) + 3 * (c)]
+static void matinv3(double *a) {
+  int ipiv[3];
+  /* dgetrf2 recursion for 3x3: n1=1 panel, then 2x2 (n1=1), then 1x1 */
+  /* panel column 0, rows 0..2 */
+  {
+    int ip = 0;
+    double mx = fabs(A_(0, 0));
+    for (int r = 1; r < 3; r++)
+      if (fabs(A_(r, 0)) > mx) { mx = fabs(A_(r, 0)); ip = r; }
+    ipiv[0] = ip;
+    if (A_(ip, 0) != 0.0) {
+      if (ip != 0) { double t = A_(0, 0); A_(0, 0) = A_(ip, 0); A_(ip, 0) = t; }
+      if (fabs(A_(0, 0)) >= DBL_MIN) {
+        double rc = 1.0 / A_(0, 0);
+        for (int r = 1; r < 3; r++) A_(r, 0) = rc * A_(r, 0);
+      } else {
+        for (int r = 1; r < 3; r++) A_(r, 0) = A_(r, 0) / A_(0, 0);
+      }
+    }
+    /* dlaswp on columns 1..2 */
+    if (ip != 0)
+      for (int c = 1; c < 3; c++) { double t = A_(0, c); A_(0, c) = A_(ip, c); A_(ip, c) = t; }
+    /* dgemm: A22 -= A21*A12 (k=1): C(i,j) = C(i,j) + (-B(0,j))*A(i,0) */
+    for (int c = 1; c < 3; c++) {
+      double temp = -A_(0, c);
+      for (int r = 1; r < 3; r++) A_(r, c) = A_(r, c) + temp * A_(r, 0);
+    }
+  }
+  /* 2x2 sub-block rows/cols 1..2 */
+  {
+    int ip = 1;
+    double mx = fabs(A_(1, 1));
+    if (fabs(A_(2, 1)) > mx) { ip = 2; }
+    ipiv[1] = ip;
+    if (A_(ip, 1) != 0.0) {
+      if (ip != 1) { double t = A_(1, 1); A_(1, 1) = A_(ip, 1); A_(ip, 1) = t; }
+      if (fabs(A_(1, 1)) >= DBL_MIN) {
+        double rc = 1.0 / A_(1, 1);
+        A_(2, 1) = rc * A_(2, 1);
+      } else {
+        A_(2, 1) = A_(2, 1) / A_(1, 1);
+      }
+    }
+    if (ip != 1) { double t = A_(1, 2); A_(1, 2) = A_(ip, 2); A_(ip, 2) = t; }
+    double temp = -A_(1, 2);
+    A_(2, 2) = A_(2, 2) + temp * A_(2, 1);
+    ipiv[2] = 2;
+    /* dlaswp of the 2x2 pivots applied to column 0 (left of the sub-block) */
+    if (ip != 1) { double t = A_(1, 0); A_(1, 0) = A_(ip, 0); A_(ip, 0) = t; }
+  }
+  /* dtrti2 upper non-unit */
+  for (int j = 0; j < 3; j++) {
+    A_(j, j) = 1.0 / A_(j, j);
+    double ajj = -A_(j, j);
+    /* dtrmv upper, no-trans, non-unit, order j, x = A(0:j-1, j) */
+    for (int jj = 0; jj < j; jj++) {
+      if (A_(jj, j) != 0.0) {
+        double temp = A_(jj, j);
+        for (int i = 0; i < jj; i++) A_(i, j) = A_(i, j) + temp * A_(i, jj);
+        A_(jj, j) = A_(jj, j) * A_(jj, jj);
+      }
+    }
+    for (int i = 0; i < j; i++) A_(i, j) = ajj * A_(i, j);
+  }
+  /* dgetri unblocked: solve inv(A)*L = inv(U) */
+  double work[3];
+  for (int j = 2; j >= 0; j--) {
+    for (int i = j + 1; i < 3; i++) { work[i] = A_(i, j); A_(i, j) = 0.0; }
+    if (j < 2) {
+      /* dgemv N: y = y + (-1)*A(:,j+1:n)*work(j+1:n), column by column */
+      for (int jj = j + 1; jj < 3; jj++) {
+        double temp = -work[jj];
+        for (int i = 0; i < 3; i++) A_(i, j) = A_(i, j) + temp * A_(i, jj);
+      }
+    }
+  }
+  for (int j = 1; j >= 0; j--) {
+    int jp = ipiv[j];
+    if (jp != j)
+      for (int r = 0; r < 3; r++) { double t = A_(r, j); A_(r, j) = A_(r, jp); A_(r, jp) = t; }
+  }
+}
+#undef A_
+
+void oracle_bader_metrics(const double x2c[9], const int n[3], double lat2car[9],
+                          double car2lat[9], double lat_i_dist[27]) {
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) lat2car[r + 3 * c] = x2c[r + 3 * c] / (double)n[c];
+  memcpy(car2lat, lat2car, 9 * sizeof(double));
+  matinv3(car2lat);
+  for (int i = -1; i <= 1; i++)
+    for (int j = -1; j <= 1; j++)
+      for (int k = -1; k <= 1; k++) {
+        double dlat[3] = {(double)i, (double)j, (double)k}, dcar[3];
+        x2c_apply(lat2car, dlat, dcar);
+        double d = sqrt(dcar[0] * dcar[0] + dcar[1] * dcar[1] + dcar[2] * dcar[2]);
+        int id = (i + 1) + 3 * (j + 1) + 9 * (k + 1);
+        lat_i_dist[id] = (i == 0 && j == 0 && k == 0) ? 0.0 : 1.0 / d;
+      }
+}
+
+/* First sweep of bader_integrate, bader@proc.f90:109-215. */
+static int first_sweep(bstate *s, const orc_geom *g, double **xattr, int *xcap) {
+  const int n1 = s->n[0], n2 = s->n[1], n3 = s->n[2];
+  int atexist = (g->mode != ORC_ATTR_NOATOMS);
+  double ratom = (g->mode == ORC_ATTR_NEAREST_ATOM) ? 1e40 : g->ratom;
+  s->nbasin = 0;
+  if (atexist) {
+    s->nbasin = g->nat;
+    for (int a = 0; a < g->nat; a++)
+      for (int c = 0; c < 3; c++) (*xattr)[3 * a + c] = g->xat[3 * a + c];
+  }
+  int p[3];
+  int64_t nstarts = 0;
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++)
+      for (int k = 0; k < n3; k++) {
+        if (s->volnum[lin(s, i, j, k)] != 0) continue;
+        nstarts++;
+        p[0] = i; p[1] = j; p[2] = k;
+        max_neargrid(s, p);
+        int32_t pv = s->volnum[lin(s, p[0], p[1], p[2])];
+        if (pv == 0) {
+          double dv[3] = {(double)p[0] / (double)n1, (double)p[1] / (double)n2,
+                          (double)p[2] / (double)n3};
+          int assigned = 0;
+          if (atexist) {
+            int nid = identify_atom(g, dv, ratom);
+            if (nid > 0) { pv = nid; assigned = 1; }
+          }
+          if (!assigned && ratom > 1e-40) { /* vsmall = 1d-40 (param.F90) */
+            for (int l = 0; l < s->nbasin; l++)
+              if (are_lclose(g, dv, &(*xattr)[3 * l], ratom)) { pv = l + 1; assigned = 1; break; }
+          }
+          if (!assigned) {
+            s->nbasin++;
+            if (s->nbasin > *xcap) {
+              *xcap = 2 * s->nbasin;
+              *xattr = (double *)realloc(*xattr, sizeof(double) * 3 * (*xcap));
+            }
+            pv = s->nbasin;
+            for (int c = 0; c < 3; c++) (*xattr)[3 * (s->nbasin - 1) + c] = dv[c];
+          }
+        }
+        for (int64_t l = 0; l < s->pnum; l++) {
+          int pt[3] = {s->path[3 * l], s->path[3 * l + 1], s->path[3 * l + 2]};
+          int64_t it = lin(s, pt[0], pt[1], pt[2]);
+          if (s->volnum[it] != -1) s->volnum[it] = pv;
+          if (s->known[it] != 2) s->known[it] = 0;
+          assign_surrounding_pts(s, pt);
+        }
+      }
+  return (int)(nstarts > 0x7fffffff ? 0x7fffffff : nstarts);
+}
+
+static int run_bader(const double *f, const int n[3], const double car2lat[9],
+                     const double lat_i_dist[27], const orc_geom *geom,
+                     int32_t *volnum_out, int32_t *known_out, double *xattr_out,
+                     int xattr_cap, orc_bader_info *info, int do_refine) {
+  bstate s;
+  memset(&s, 0, sizeof(s));
+  s.f = f;
+  s.n[0] = n[0]; s.n[1] = n[1]; s.n[2] = n[2];
+  s.N = (int64_t)n[0] * n[1] * n[2];
+  memcpy(s.car2lat, car2lat, sizeof(s.car2lat));
+  memcpy(s.lid, lat_i_dist, sizeof(s.lid));
+  s.volnum = volnum_out;
+  memset(s.volnum, 0, sizeof(int32_t) * s.N);
+  s.known = (int32_t *)calloc((size_t)s.N, sizeof(int32_t));
+  s.pcap = 16;
+  s.path = (int *)malloc(sizeof(int) * 3 * s.pcap);
+  int xcap = geom->nat > 4 ? geom->nat : 4;
+  double *xattr = (double *)malloc(sizeof(double) * 3 * xcap);
+
+  int64_t nstarts = 0;
+  {
+    /* count starts inside first_sweep through nsteps bookkeeping */
+    nstarts = first_sweep(&s, geom, &xattr, &xcap);
+  }
+  int64_t sweep_steps = s.nsteps;
+  int status = 0, iters = 0;
+  if (do_refine) {
+    int ref_itrs = 1;
+    for (;;) {
+      int64_t nr = refine_edge(&s, ref_itrs);
+      iters++;
+      if (nr < 0) { status = 1; break; }
+      if (nr == 0) break;
+      ref_itrs++;
+    }
+  }
+  if (known_out) memcpy(known_out, s.known, sizeof(int32_t) * s.N);
+  if (info) {
+    info->nattr = s.nbasin;
+    info->refine_iters = iters;
+    info->nstarts = nstarts;
+    info->nsteps = sweep_steps;
+    info->status = status;
+    info->nmax_found = 0;
+  }
+  int ncopy = s.nbasin < xattr_cap ? s.nbasin : xattr_cap;
+  if (xattr_out) memcpy(xattr_out, xattr, sizeof(double) * 3 * ncopy);
+  free(xattr);
+  free(s.known);
+  free(s.path);
+  return status;
+}
+
+int oracle_bader(const double *f, const int n[3], const double car2lat[9],
+                 const double lat_i_dist[27], const orc_geom *geom,
+                 int32_t *volnum_out, double *xattr_out, int xattr_cap,
+                 orc_bader_info *info) {
+  return run_bader(f, n, car2lat, lat_i_dist, geom, volnum_out, NULL, xattr_out,
+                   xattr_cap, info, 1);
+}
+
+int oracle_bader_sweep_only(const double *f, const int n[3],
+                            const double car2lat[9], const double lat_i_dist[27],
+                            const orc_geom *geom, int32_t *volnum_out,
+                            int32_t *known_out, double *xattr_out, int xattr_cap,
+                            orc_bader_info *info) {
+  return run_bader(f, n, car2lat, lat_i_dist, geom, volnum_out, known_out,
+                   xattr_out, xattr_cap, info, 0);
+}
+
+void oracle_basin_sums(const int32_t *idg, int64_t npts, int nattr, int nfields,
+                       const double *const *fields, int64_t *counts, double *sums) {
+  for (int a = 0; a < nattr; a++) counts[a] = 0;
+  for (int64_t q = 0; q < (int64_t)nattr * nfields; q++) sums[q] = 0.0;
+  /* count(idg == i) and sum(fint, idg == i), one sequential pass per
+   * attractor in memory order (integration@proc.f90:1215, :1296); the pass
+   * order over attractors does not change any individual sum. */
+  for (int64_t p = 0; p < npts; p++) {
+    int32_t a = idg[p];
+    if (a < 1 || a > nattr) continue;
+    counts[a - 1]++;
+    for (int fi = 0; fi < nfields; fi++) sums[(int64_t)fi * nattr + (a - 1)] += fields[fi][p];
+  }
+}
+
+void oracle_relabel(int32_t *idg, int64_t npts, const int32_t *map, int nmap) {
+  for (int64_t p = 0; p < npts; p++) {
+    int32_t a = idg[p];
+    if (a >= 1 && a <= nmap) idg[p] = map[a - 1];
+  }
+}

@@ -1,0 +1,119 @@
+/*
+ * c2oracle.h -- CPU restatement of critic2's on-grid Bader and NCI path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in critic2_amd/.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  It is never the thing measured
+ * as the GPU result and never a fallback for the product path.
+ *
+ * Conventions (same as critic2):
+ *   - grids are Fortran column-major f(n1,n2,n3): index = i + n1*(j + n2*k),
+ *     i (x) fastest, all indices 0-based here (critic2 uses 1-based).
+ *   - 3x3 matrices are column-major: m[r + 3*c] == m(r+1,c+1) in Fortran.
+ *   - lat_i_dist is indexed [(d1+1) + 3*(d2+1) + 9*(d3+1)], d in {-1,0,1}.
+ */
+#ifndef C2ORACLE_H
+#define C2ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Attractor-assignment modes of BADER (integration@proc.f90:236-248). */
+enum {
+  ORC_ATTR_NEAREST_ATOM = 0, /* default (nonnm): atexist, ratom = 1d40     */
+  ORC_ATTR_RATOM = 1,        /* NNM / RATOM r: atexist, ratom = r           */
+  ORC_ATTR_NOATOMS = 2       /* NOATOMS: no atoms, maxima merged by ratom   */
+};
+
+typedef struct {
+  double x2c[9];      /* lattice vectors as columns, bohr                   */
+  int nat;            /* atoms in the cell (complete list, critic2 order)   */
+  const double *xat;  /* fractional atom coordinates, 3*nat                 */
+  int mode;           /* ORC_ATTR_*                                         */
+  double ratom;       /* bohr                                               */
+} orc_geom;
+
+/* Outputs of oracle_bader. */
+typedef struct {
+  int nattr;          /* number of attractors (bas%nattr after bader)       */
+  int nmax_found;     /* number of distinct grid maxima reached             */
+  int refine_iters;   /* number of refine_edge calls                        */
+  int64_t nstarts;    /* first-sweep paths started                          */
+  int64_t nsteps;     /* first-sweep step_neargrid calls                    */
+  int status;         /* 0 ok; 1 new maximum during refinement             */
+} orc_bader_info;
+
+/* bader_integrate (bader@proc.f90:80-234).  volnum_out: N int32 labels,
+ * 1-based attractor ids.  xattr_out: capacity 3*xattr_cap fractional
+ * attractor positions (discovery order, atoms first when atexist).
+ * lat2car/car2lat/lat_i_dist are passed in so the caller fixes their bits. */
+int oracle_bader(const double *f, const int n[3], const double car2lat[9],
+                 const double lat_i_dist[27], const orc_geom *geom,
+                 int32_t *volnum_out, double *xattr_out, int xattr_cap,
+                 orc_bader_info *info);
+
+/* Same, but stop after the first sweep (no refine_edge). Test hook. */
+int oracle_bader_sweep_only(const double *f, const int n[3],
+                            const double car2lat[9], const double lat_i_dist[27],
+                            const orc_geom *geom, int32_t *volnum_out,
+                            int32_t *known_out, double *xattr_out, int xattr_cap,
+                            orc_bader_info *info);
+
+/* lat2car(:,i) = x2c(:,i)/n(i); car2lat = inverse (LU with partial pivoting
+ * and the dgetri column order); lat_i_dist as bader@proc.f90:124-145. */
+void oracle_bader_metrics(const double x2c[9], const int n[3], double lat2car[9],
+                          double car2lat[9], double lat_i_dist[27]);
+
+/* Per-basin sums (integration@proc.f90:1208-1218, :1289-1300):
+ * counts[a] = count(idg == a+1); sums[fi*nattr + a] = sum(field_fi, idg==a+1)
+ * summed in column (memory) order, sequentially. */
+void oracle_basin_sums(const int32_t *idg, int64_t npts, int nattr, int nfields,
+                       const double *const *fields, int64_t *counts, double *sums);
+
+/* Relabel: idg = map[idg-1] for idg >= 1 (int_reorder_gridout remap). */
+void oracle_relabel(int32_t *idg, int64_t npts, const int32_t *map, int nmap);
+
+/* Promolecular radial tables (grid1mod@proc.f90:206-331 semantics):
+ * r[i] = a*exp(b*i), tables f, fp, fpp already built. */
+typedef struct {
+  int ngrid;
+  double a, b, rmax;
+  const double *r, *f, *fp, *fpp;
+} orc_rgrid;
+
+/* grid1%interp (grid1mod@proc.f90:86-137). */
+void oracle_grid1_interp(const orc_rgrid *g, double r0, double *f, double *fp,
+                         double *fpp);
+
+/* Build f, fp, fpp from rr0 = sum(occ*psi^2) on the log grid
+ * (grid1mod@proc.f90:281-313); ngrid is the already-truncated count. */
+void oracle_grid1_build(int ngrid, const double *r, const double *rr0,
+                        double *f, double *fp, double *fpp, double *a,
+                        double *b);
+
+/* Promolecular density, gradient, Hessian at Cartesian point xc
+ * (crystalmod@env.f90:622-753), atoms listed with Cartesian positions of all
+ * images already expanded (the caller chooses the image set). Each atom uses
+ * table tab[zidx[i]] and cutoff cut[zidx[i]]. */
+void oracle_promol(const double xc[3], int nimg, const double *ximg,
+                   const int *zidx, const orc_rgrid *tabs, const double *cut,
+                   int nder, double *f, double g[3], double h[9]);
+
+/* NCI per-point quantities from node values (nci@proc.f90:543-605):
+ * s = gmod / (const * max(rho,1d-80)^(4/3)); crho = sign(rho, l2)*100. */
+void oracle_nci_point(double rho, double gmod, const double ehess_or_h[9],
+                      int from_full_hessian, double *crho, double *s);
+
+/* Synthetic Slater-sum density (FP64) on an n1*n2*n3 grid, x fastest.
+ * rho = sum_{atoms,images} Z*(zeta^3/(8 pi))*exp(-zeta*|r-R|),
+ * zeta = 2*Z^(1/3) bohr^-1, images within rimg bohr. */
+void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
+                        const double *xat, const int *zat, double rimg,
+                        double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
