@@ -1,0 +1,178 @@
+"""Host-side mirror of critic2's BADER keyword on a loaded grid.
+
+``bader_integrate`` follows bader_integrate(s, bas, iref)
+(bader.f90:48-53, bader@proc.f90:80-234) and ``intgrid_fields`` the
+per-basin Volume / Pop / Lap integration of intgrid_fields
+(integration@proc.f90:1170-1391).  ``bader`` mirrors the BADER keyword as
+run by intgrid_driver (integration@proc.f90:60-379): options, labelling,
+attractor reordering (identity in the default mode, :1027-1166) and the
+integrated properties.  The arithmetic runs in libc2gpu on the GPU.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import Context, C2GpuError
+
+
+@dataclass
+class BasinData:
+    """The parts of critic2's basindat (types.f90:361-383) this path fills."""
+
+    n: tuple
+    atexist: bool = True
+    ratom: float = 1e40
+    nattr: int = 0
+    xattr: np.ndarray = None       # (nattr, 3) fractional
+    idg: np.ndarray = None         # labels, Fortran order (n1,n2,n3) flattened x-fastest
+    icp: np.ndarray = None
+    stats: dict = field(default_factory=dict)
+
+
+def _mode(nnm: bool, ratom, noatoms: bool):
+    """Option resolution of intgrid_driver (integration@proc.f90:236-248)."""
+    if noatoms:
+        return _lib.ATTR_NOATOMS, (1.0 if ratom is None else ratom)
+    if nnm or ratom is not None:
+        return _lib.ATTR_RATOM, (1.0 if ratom is None else ratom)
+    return _lib.ATTR_NEAREST_ATOM, 1e40
+
+
+def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noatoms=False,
+                    car2lat=None, lat_i_dist=None, labels_out=None) -> BasinData:
+    """Near-grid Bader labelling of the FP64 grid rho (x fastest, n1*n2*n3).
+
+    rho may be a numpy array (host) or a DeviceArray resident on the GPU;
+    labels_out (optional) may likewise be a device int32 DeviceArray that
+    receives the labels without a host copy.  car2lat / lat_i_dist default to the reference's
+    metrics (bader@proc.f90:123-145)."""
+    n = tuple(int(v) for v in n)
+    npts = n[0] * n[1] * n[2]
+    mode, ra = _mode(nnm, ratom, noatoms)
+    if car2lat is None or lat_i_dist is None:
+        _, car2lat, lat_i_dist = _lib.bader_metrics(x2c, n)
+    xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1, 3)
+    opts = _lib.AttractorOpts()
+    opts.mode = mode
+    opts.ratom = ra
+    for q, v in enumerate(_lib.mat_f(x2c)):
+        opts.x2c[q] = v
+    opts.nat = xat.shape[0]
+    xat_flat = np.ascontiguousarray(xat.reshape(-1))
+    opts.xat = xat_flat.ctypes.data
+    nn = np.asarray(n, dtype=np.int32)
+    c2l = np.ascontiguousarray(car2lat, dtype=np.float64)
+    lid = np.ascontiguousarray(lat_i_dist, dtype=np.float64)
+    on_device = hasattr(rho, "data_ptr")
+    flags = _lib.PTR_DEVICE if on_device else 0
+    if on_device:
+        if labels_out is None:
+            labels_out = ctx.empty((npts,), np.int32)
+        lab = labels_out
+    else:
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        lab = np.zeros(npts, dtype=np.int32)
+    cap = max(opts.nat, 1) * 4 + 64
+    xattr = np.zeros(3 * cap)
+    nattr = _lib.C.c_int(0)
+    st = _lib.BaderStats()
+    rc = ctx.lib.c2gpu_bader(ctx.h, _lib.ptr(rho), nn.ctypes.data, c2l.ctypes.data, lid.ctypes.data,
+                             _lib.C.byref(opts), flags, _lib.ptr(lab), _lib.C.byref(nattr),
+                             xattr.ctypes.data, cap, _lib.C.byref(st))
+    ctx.check(rc, "c2gpu_bader")
+    bas = BasinData(n=n, atexist=(mode != _lib.ATTR_NOATOMS), ratom=ra, nattr=nattr.value,
+                    xattr=xattr[:3 * nattr.value].reshape(-1, 3).copy(), idg=lab,
+                    stats=st.as_dict())
+    bas.icp = np.arange(1, bas.nattr + 1, dtype=np.int32)  # default mode: atoms in cell order
+    return bas
+
+
+def basin_sums(ctx: Context, labels, nattr: int, fields=()):
+    """count(idg == i) and sum(field, idg == i) for i = 1..nattr."""
+    on_device = hasattr(labels, "data_ptr")
+    npts = int(labels.numel() if on_device else labels.size)
+    flags = _lib.PTR_DEVICE if on_device else 0
+    if on_device:
+        # one memory space per call: host fields go to the device first
+        fields = [f if hasattr(f, "data_ptr") else ctx.to_device(np.ascontiguousarray(f, dtype=np.float64))
+                  for f in fields]
+    else:
+        labels = np.ascontiguousarray(labels, dtype=np.int32).reshape(-1)
+        fields = [f.numpy() if hasattr(f, "data_ptr") else f for f in fields]
+        fields = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1) for f in fields]
+    nf = len(fields)
+    fp = (_lib.C.c_void_p * max(nf, 1))(*[_lib.ptr(f) for f in fields])
+    sums = np.zeros(max(nf, 1) * nattr)
+    counts = np.zeros(nattr, dtype=np.int64)
+    rc = ctx.lib.c2gpu_basin_sums(ctx.h, _lib.ptr(labels), npts, nattr, nf, fp, flags,
+                                  sums.ctypes.data, counts.ctypes.data)
+    ctx.check(rc, "c2gpu_basin_sums")
+    return counts, sums[: nf * nattr].reshape(nf, nattr)
+
+
+def fft_deriv(ctx: Context, rho, n, x2c, kind):
+    """grid3%fft (grid3mod@proc.f90:1757-1872) on a host grid."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+    out = np.zeros_like(rho)
+    nn = np.asarray(n, dtype=np.int32)
+    x = _lib.mat_f(x2c)
+    rc = ctx.lib.c2gpu_fft_deriv(ctx.h, rho.ctypes.data, nn.ctypes.data, x.ctypes.data, kind, 0,
+                                 out.ctypes.data)
+    ctx.check(rc, "c2gpu_fft_deriv")
+    return out
+
+
+def intgrid_fields(ctx: Context, bas: BasinData, rho, x2c, lap=True):
+    """Volume, Pop and (optionally) Lap per basin, scaled by Omega/N as in
+    integration@proc.f90:1215 and :1296.  Lap integrates the FFT Laplacian
+    of the grid (:1249)."""
+    n = bas.n
+    npts = n[0] * n[1] * n[2]
+    omega = abs(float(np.linalg.det(np.asarray(x2c))))
+    fields = [rho]
+    if lap:
+        host_rho = rho.numpy() if hasattr(rho, "data_ptr") else rho
+        lapg = fft_deriv(ctx, host_rho, n, x2c, _lib.FT_LAP)
+        if hasattr(rho, "data_ptr"):
+            lapg = ctx.to_device(lapg)
+        fields.append(lapg)
+    counts, sums = basin_sums(ctx, bas.idg, bas.nattr, fields)
+    w = omega / npts
+    out = {"volume": counts * w, "pop": sums[0] * w}
+    if lap:
+        out["lap"] = sums[1] * w
+    return out
+
+
+def bader(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noatoms=False, lap=True):
+    """BADER keyword: labels + attractors + integrated Volume/Pop/Lap."""
+    bas = bader_integrate(ctx, rho, n, x2c, xat, nnm=nnm, ratom=ratom, noatoms=noatoms)
+    props = intgrid_fields(ctx, bas, rho, x2c, lap=lap)
+    return bas, props
+
+
+def synth_slater(ctx: Context, cell, rimg_ang=8.0, device_out=None):
+    """FP64 Slater-sum grid of a synth.Cell, generated on the GPU."""
+    n = np.asarray(cell.n, dtype=np.int32)
+    npts = int(n.prod())
+    x = _lib.mat_f(cell.x2c)
+    xat = np.ascontiguousarray(cell.xat.reshape(-1), dtype=np.float64)
+    zat = np.ascontiguousarray(cell.zat, dtype=np.int32)
+    rimg = rimg_ang / 0.52917720859
+    if device_out is not None:
+        rc = ctx.lib.c2gpu_synth_slater(ctx.h, x.ctypes.data, n.ctypes.data, len(zat), xat.ctypes.data,
+                                        zat.ctypes.data, rimg, _lib.PTR_DEVICE, _lib.ptr(device_out))
+        ctx.check(rc, "c2gpu_synth_slater")
+        return device_out
+    out = np.zeros(npts)
+    rc = ctx.lib.c2gpu_synth_slater(ctx.h, x.ctypes.data, n.ctypes.data, len(zat), xat.ctypes.data,
+                                    zat.ctypes.data, rimg, 0, out.ctypes.data)
+    ctx.check(rc, "c2gpu_synth_slater")
+    return out
+
+
+__all__ = ["BasinData", "bader_integrate", "basin_sums", "intgrid_fields", "bader", "fft_deriv",
+           "synth_slater", "Context", "C2GpuError"]

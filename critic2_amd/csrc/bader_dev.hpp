@@ -1,0 +1,200 @@
+// Device-side near-grid ascent primitives (gfx950).  Every function follows
+// the FP64 operation order of critic2's bader@proc.f90 so that paths are
+// bit-identical to the serial reference; this file must be compiled with
+// -ffp-contract=off (no FMA contraction: the reference build emits none).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace c2 {
+
+struct BParams {
+  const double *__restrict__ f;
+  int n1, n2, n3;
+  double c[9];    // car2lat, column-major
+  double lid[27]; // lat_i_dist
+};
+
+__device__ __forceinline__ int wrap1(int p, int n) {
+  return p < 0 ? p + n : (p >= n ? p - n : p);
+}
+__device__ __forceinline__ uint32_t lin3(const BParams &P, int i, int j, int k) {
+  return (uint32_t)i + (uint32_t)P.n1 * ((uint32_t)j + (uint32_t)P.n2 * (uint32_t)k);
+}
+__device__ __forceinline__ void coords(const BParams &P, uint32_t l, int &i, int &j, int &k) {
+  i = (int)(l % (uint32_t)P.n1);
+  uint32_t r = l / (uint32_t)P.n1;
+  j = (int)(r % (uint32_t)P.n2);
+  k = (int)(r / (uint32_t)P.n2);
+}
+// sweep-order index: do i; do j; do k (k innermost), bader@proc.f90:151-153
+__device__ __forceinline__ uint64_t sweep_index(const BParams &P, uint32_t l) {
+  int i, j, k;
+  coords(P, l, i, j, k);
+  return ((uint64_t)i * (uint64_t)P.n2 + (uint64_t)j) * (uint64_t)P.n3 + (uint64_t)k;
+}
+// rho_val with periodic wrap (offsets of at most one cell), :619-639
+__device__ __forceinline__ double rv(const BParams &P, int i, int j, int k) {
+  return P.f[lin3(P, wrap1(i, P.n1), wrap1(j, P.n2), wrap1(k, P.n3))];
+}
+
+// is_max, bader@proc.f90:571-597 (strict >, plateaus count as maxima)
+__device__ inline bool is_max(const BParams &P, int i, int j, int k) {
+  const double r = rv(P, i, j, k);
+  bool m = true;
+#pragma unroll
+  for (int d1 = -1; d1 <= 1; d1++)
+#pragma unroll
+    for (int d2 = -1; d2 <= 1; d2++)
+#pragma unroll
+      for (int d3 = -1; d3 <= 1; d3++)
+        if (rv(P, i + d1, j + d2, k + d3) > r) m = false;
+  return m;
+}
+
+// step_ongrid, bader@proc.f90:500-527: first strict maximum in (d1,d2,d3)
+// order of rho_ctr + (rho - rho_ctr)*lat_i_dist
+__device__ inline void step_ongrid(const BParams &P, int p[3]) {
+  int pm0 = p[0], pm1 = p[1], pm2 = p[2];
+  const double rc = rv(P, p[0], p[1], p[2]);
+  double rmax = rc;
+#pragma unroll
+  for (int d1 = -1; d1 <= 1; d1++)
+#pragma unroll
+    for (int d2 = -1; d2 <= 1; d2++)
+#pragma unroll
+      for (int d3 = -1; d3 <= 1; d3++) {
+        double t = rv(P, p[0] + d1, p[1] + d2, p[2] + d3);
+        t = rc + (t - rc) * P.lid[(d1 + 1) + 3 * (d2 + 1) + 9 * (d3 + 1)];
+        if (t > rmax) {
+          rmax = t;
+          pm0 = p[0] + d1;
+          pm1 = p[1] + d2;
+          pm2 = p[2] + d3;
+        }
+      }
+  p[0] = wrap1(pm0, P.n1);
+  p[1] = wrap1(pm1, P.n2);
+  p[2] = wrap1(pm2, P.n3);
+}
+
+// rho_grad_dir, bader@proc.f90:532-567
+__device__ inline void grad_dir(const BParams &P, const int p[3], double res[3]) {
+  const int a = p[0], b = p[1], c = p[2];
+  const double r000 = rv(P, a, b, c);
+  const double r001 = rv(P, a, b, c + 1);
+  const double r010 = rv(P, a, b + 1, c);
+  const double r100 = rv(P, a + 1, b, c);
+  const double r00m = rv(P, a, b, c - 1);
+  const double rm00 = rv(P, a - 1, b, c);
+  const double r0m0 = rv(P, a, b - 1, c);
+  double gl[3];
+  gl[0] = (r100 - rm00) / 2.0;
+  gl[1] = (r010 - r0m0) / 2.0;
+  gl[2] = (r001 - r00m) / 2.0;
+  if (r100 < r000 && rm00 < r000) gl[0] = 0.0;
+  if (r010 < r000 && r0m0 < r000) gl[1] = 0.0;
+  if (r001 < r000 && r00m < r000) gl[2] = 0.0;
+  double gc[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {  // matmul(rho_grad_lat, car2lat)
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) acc = acc + gl[i] * P.c[i + 3 * j];
+    gc[j] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {  // matmul(car2lat, rho_grad_car)
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) acc = acc + P.c[i + 3 * j] * gc[j];
+    res[i] = acc;
+  }
+}
+
+// Fortran nint for the small values met here: round half away from zero.
+__device__ __forceinline__ int fnint(double x) { return (int)round(x); }
+
+// One step_neargrid call (bader@proc.f90:455-494) up to, but excluding, the
+// known(pm)==1 test, which needs the caller's path.  Returns false when the
+// is_max early return fires (no move, known(p) untouched); otherwise pm
+// holds the wrapped candidate target.
+__device__ inline bool step_near_candidate(const BParams &P, const int p[3], double dr[3],
+                                           int pm[3]) {
+  double g[3];
+  grad_dir(P, p, g);
+  double gm = fabs(g[0]);
+  if (fabs(g[1]) > gm) gm = fabs(g[1]);
+  if (fabs(g[2]) > gm) gm = fabs(g[2]);
+  if (gm < 1e-30) {
+    dr[0] = dr[1] = dr[2] = 0.0;
+    if (is_max(P, p[0], p[1], p[2])) return false;
+    pm[0] = p[0];
+    pm[1] = p[1];
+    pm[2] = p[2];
+    step_ongrid(P, pm);
+    return true;
+  }
+  const double coeff = 1.0 / gm;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    g[c] = coeff * g[c];
+    const int ng = fnint(g[c]);
+    int q = p[c] + ng;
+    dr[c] = (dr[c] + g[c]) - (double)ng;
+    const int nd = fnint(dr[c]);
+    q = q + nd;
+    dr[c] = dr[c] - (double)nd;
+    pm[c] = q;
+  }
+  pm[0] = wrap1(pm[0], P.n1);
+  pm[1] = wrap1(pm[1], P.n2);
+  pm[2] = wrap1(pm[2], P.n3);
+  return true;
+}
+
+// max_neargrid (bader@proc.f90:427-450) from `start` with dr = 0.
+// path[] (capacity maxlen) receives the visited points; known(pm)==1 is
+// membership in path[].  stop(x) is the known(x)==2 test (false for free
+// trajectories).  Returns the path length, or -1 if it exceeds maxlen.
+template <class Stop, class Visit>
+__device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path, int maxlen,
+                                Stop stop, Visit visit) {
+  int p[3];
+  coords(P, start, p[0], p[1], p[2]);
+  double dr[3] = {0.0, 0.0, 0.0};
+  int len = 1;
+  path[0] = start;
+  visit(start);
+  for (;;) {
+    int pm[3];
+    if (!step_near_candidate(P, p, dr, pm)) break;  // maximum: no move
+    uint32_t lm = lin3(P, pm[0], pm[1], pm[2]);
+    bool seen = false;
+    for (int q = len - 1; q >= 0; q--)
+      if (path[q] == lm) {
+        seen = true;
+        break;
+      }
+    if (seen) {  // known(pm) == 1: on-grid step, dr = 0
+      pm[0] = p[0];
+      pm[1] = p[1];
+      pm[2] = p[2];
+      step_ongrid(P, pm);
+      dr[0] = dr[1] = dr[2] = 0.0;
+      lm = lin3(P, pm[0], pm[1], pm[2]);
+    }
+    if (pm[0] == p[0] && pm[1] == p[1] && pm[2] == p[2]) break;  // no move
+    if (len >= maxlen) return -1;
+    path[len++] = lm;
+    visit(lm);
+    p[0] = pm[0];
+    p[1] = pm[1];
+    p[2] = pm[2];
+    if (stop(lm)) break;
+  }
+  return len;
+}
+
+}  // namespace c2

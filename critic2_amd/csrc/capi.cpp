@@ -1,0 +1,364 @@
+// extern "C" entry points of libc2gpu (declared in include/c2gpu.h) and the
+// host-side pieces of the path: grid metrics (bader@proc.f90:123-145 with a
+// 3x3 restatement of LAPACK dgetrf2/dgetri) and the attractor assignment of
+// new maxima (bader@proc.f90:159-199).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace c2 {
+void bader_run(c2gpu_ctx *, const double *, const int *, const double *, const double *,
+               const c2gpu_attractor_opts *, int, int32_t *, int *, double *, int, c2gpu_bader_stats *);
+void step_pointers_run(c2gpu_ctx *, const double *, const int *, const double *, const double *, int,
+                       int64_t *, int64_t *, double *);
+void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const double *const *, int,
+                    double *, int64_t *);
+void relabel_run(c2gpu_ctx *, int64_t, const int32_t *, int, int, int32_t *);
+void fft_deriv_run(c2gpu_ctx *, const double *, const int *, const double *, int, int, double *);
+void nci_run(c2gpu_ctx *, const double *, const int *, const double *, int, int, const c2gpu_promol *,
+             int, int, void *, void *, int);
+void synth_slater_run(c2gpu_ctx *, const double *, const int *, int, const double *, const int *,
+                      double, int, double *);
+
+// matmul(m, x) for a column-major 3x3 m
+static void mat3_apply(const double m[9], const double x[3], double out[3]) {
+  for (int i = 0; i < 3; i++) {
+    double acc = 0.0;
+    for (int j = 0; j < 3; j++) acc = acc + m[i + 3 * j] * x[j];
+    out[i] = acc;
+  }
+}
+
+// Inverse of a 3x3 column-major matrix with the operation order of the
+// reference LAPACK: dgetrf2 (recursive, n1 = 1 split, reciprocal-scaled
+// pivot column), dtrti2 (upper inverse), dgetri's unblocked solve and the
+// final column interchanges (src/oldlibs/dgetrf2.f, dgetri.f, dtrti2.f).
+static bool inv3_lapack(double *a) {
+#define M(r, c) a[(r) + 3 * (c)]
+  int ipiv[3];
+  for (int col = 0; col < 2; col++) {
+    int ip = col;
+    double mx = std::fabs(M(col, col));
+    for (int r = col + 1; r < 3; r++)
+      if (std::fabs(M(r, col)) > mx) {
+        mx = std::fabs(M(r, col));
+        ip = r;
+      }
+    ipiv[col] = ip;
+    if (M(ip, col) == 0.0) return false;
+    if (ip != col) std::swap(M(col, col), M(ip, col));
+    if (std::fabs(M(col, col)) >= DBL_MIN) {
+      const double rc = 1.0 / M(col, col);
+      for (int r = col + 1; r < 3; r++) M(r, col) = rc * M(r, col);
+    } else {
+      for (int r = col + 1; r < 3; r++) M(r, col) = M(r, col) / M(col, col);
+    }
+    if (ip != col)
+      for (int c = col + 1; c < 3; c++) std::swap(M(col, c), M(ip, c));
+    for (int c = col + 1; c < 3; c++) {
+      const double temp = -M(col, c);
+      for (int r = col + 1; r < 3; r++) M(r, c) = M(r, c) + temp * M(r, col);
+    }
+    if (col == 1 && ip != 1) std::swap(M(1, 0), M(ip, 0));
+  }
+  ipiv[2] = 2;
+  if (M(2, 2) == 0.0) return false;
+  for (int j = 0; j < 3; j++) {  // dtrti2
+    M(j, j) = 1.0 / M(j, j);
+    const double ajj = -M(j, j);
+    for (int jj = 0; jj < j; jj++) {
+      if (M(jj, j) != 0.0) {
+        const double temp = M(jj, j);
+        for (int i = 0; i < jj; i++) M(i, j) = M(i, j) + temp * M(i, jj);
+        M(jj, j) = M(jj, j) * M(jj, jj);
+      }
+    }
+    for (int i = 0; i < j; i++) M(i, j) = ajj * M(i, j);
+  }
+  double work[3];
+  for (int j = 2; j >= 0; j--) {  // dgetri, NB >= N branch
+    for (int i = j + 1; i < 3; i++) {
+      work[i] = M(i, j);
+      M(i, j) = 0.0;
+    }
+    for (int jj = j + 1; jj < 3; jj++) {
+      const double temp = -work[jj];
+      for (int i = 0; i < 3; i++) M(i, j) = M(i, j) + temp * M(i, jj);
+    }
+  }
+  for (int j = 1; j >= 0; j--)
+    if (ipiv[j] != j)
+      for (int r = 0; r < 3; r++) std::swap(M(r, j), M(r, ipiv[j]));
+#undef M
+  return true;
+}
+
+// Shortest lattice-translated length of a fractional difference (Cartesian
+// norm), by search over translations -2..2 after folding.
+static double lattice_dist(const double x2c[9], const double xd[3]) {
+  double x0[3];
+  for (int c = 0; c < 3; c++) x0[c] = xd[c] - std::round(xd[c]);
+  double best = DBL_MAX;
+  for (int a = -2; a <= 2; a++)
+    for (int b = -2; b <= 2; b++)
+      for (int c = -2; c <= 2; c++) {
+        const double xt[3] = {x0[0] + a, x0[1] + b, x0[2] + c};
+        double xc[3];
+        mat3_apply(x2c, xt, xc);
+        const double d = std::sqrt(xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
+        if (d < best) best = d;
+      }
+  return best;
+}
+
+int nearest_atom_label(const c2gpu_attractor_opts *o, const double x[3]) {
+  int best = 0;
+  double bd = DBL_MAX;
+  for (int a = 0; a < o->nat; a++) {
+    const double xd[3] = {x[0] - o->xat[3 * a], x[1] - o->xat[3 * a + 1], x[2] - o->xat[3 * a + 2]};
+    const double d = lattice_dist(o->x2c, xd);
+    if (d < bd) {
+      bd = d;
+      best = a + 1;
+    }
+  }
+  return best;
+}
+
+}  // namespace c2
+
+using namespace c2;
+
+extern "C" {
+
+int c2gpu_ctx_create(c2gpu_ctx **ctx, int device) {
+  if (!ctx) return C2GPU_ERR_ARG;
+  *ctx = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return C2GPU_ERR_HIP;
+  c2gpu_ctx *c = new c2gpu_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return C2GPU_ERR_HIP;
+  }
+  c->own_stream = true;
+  *ctx = c;
+  return C2GPU_OK;
+}
+
+int c2gpu_ctx_destroy(c2gpu_ctx *ctx) {
+  if (!ctx) return C2GPU_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  release_all(ctx);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return C2GPU_OK;
+}
+
+const char *c2gpu_ctx_error(const c2gpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int c2gpu_ctx_set_stream(c2gpu_ctx *ctx, void *hip_stream) {
+  return guarded(ctx, [&] {
+    if (ctx->own_stream && ctx->stream) {
+      C2_HIP(hipStreamSynchronize(ctx->stream));
+      C2_HIP(hipStreamDestroy(ctx->stream));
+    }
+    if (hip_stream) {
+      ctx->stream = (hipStream_t)hip_stream;
+      ctx->own_stream = false;
+    } else {
+      C2_HIP(hipStreamCreate(&ctx->stream));
+      ctx->own_stream = true;
+    }
+  });
+}
+
+int c2gpu_ctx_sync(c2gpu_ctx *ctx) {
+  return guarded(ctx, [&] { C2_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int c2gpu_dev_alloc(c2gpu_ctx *ctx, uint64_t bytes, void **ptr) {
+  return guarded(ctx, [&] {
+    if (!ptr) fail(C2GPU_ERR_ARG, "null argument");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) fail(C2GPU_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  });
+}
+
+int c2gpu_dev_free(c2gpu_ctx *ctx, void *ptr) {
+  return guarded(ctx, [&] {
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    if (ptr) C2_HIP(hipFree(ptr));
+  });
+}
+
+int c2gpu_memcpy(c2gpu_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind) {
+  return guarded(ctx, [&] {
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                      : kind == 2 ? hipMemcpyDeviceToHost
+                      : kind == 3 ? hipMemcpyDeviceToDevice
+                                  : hipMemcpyDefault;
+    if (kind < 1 || kind > 3) fail(C2GPU_ERR_ARG, "c2gpu_memcpy: bad kind");
+    C2_HIP(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int c2gpu_ctx_timing(c2gpu_ctx *ctx, int enable) {
+  return guarded(ctx, [&] {
+    flush_timing(ctx);
+    ctx->ktime.clear();
+    ctx->timing = enable != 0;
+  });
+}
+
+int c2gpu_ctx_kernel_time(c2gpu_ctx *ctx, const char *kernel, double *ms, int64_t *launches) {
+  return guarded(ctx, [&] {
+    if (!kernel) fail(C2GPU_ERR_ARG, "null kernel name");
+    auto it = ctx->ktime.find(kernel);
+    if (ms) *ms = it == ctx->ktime.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == ctx->ktime.end() ? 0 : it->second.launches;
+  });
+}
+
+int c2gpu_ctx_kernel_names(c2gpu_ctx *ctx, char *names_out, uint64_t len) {
+  return guarded(ctx, [&] {
+    if (!names_out || len == 0) fail(C2GPU_ERR_ARG, "null buffer");
+    std::string s;
+    for (auto &kv : ctx->ktime) {
+      if (!s.empty()) s += ",";
+      s += kv.first;
+    }
+    const size_t m = s.size() < len - 1 ? s.size() : (size_t)len - 1;
+    std::memcpy(names_out, s.data(), m);
+    names_out[m] = '\0';
+  });
+}
+
+int c2gpu_bader_metrics(const double x2c[9], const int n[3], double lat2car[9], double car2lat[9],
+                        double lat_i_dist[27]) {
+  if (!x2c || !n || !lat2car || !car2lat || !lat_i_dist) return C2GPU_ERR_ARG;
+  for (int c = 0; c < 3; c++)
+    if (n[c] < 1) return C2GPU_ERR_ARG;
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) lat2car[r + 3 * c] = x2c[r + 3 * c] / (double)n[c];
+  std::memcpy(car2lat, lat2car, 9 * sizeof(double));
+  if (!inv3_lapack(car2lat)) return C2GPU_ERR_ARG;
+  for (int i = -1; i <= 1; i++)
+    for (int j = -1; j <= 1; j++)
+      for (int k = -1; k <= 1; k++) {
+        const double dlat[3] = {(double)i, (double)j, (double)k};
+        double dcar[3];
+        mat3_apply(lat2car, dlat, dcar);
+        const double d = std::sqrt(dcar[0] * dcar[0] + dcar[1] * dcar[1] + dcar[2] * dcar[2]);
+        lat_i_dist[(i + 1) + 3 * (j + 1) + 9 * (k + 1)] = (i == 0 && j == 0 && k == 0) ? 0.0 : 1.0 / d;
+      }
+  return C2GPU_OK;
+}
+
+int c2gpu_bader_step_pointers(c2gpu_ctx *ctx, const double *rho, const int n[3], const double car2lat[9],
+                              const double lat_i_dist[27], int flags, int64_t *ongrid_next,
+                              int64_t *neargrid_next0, double *dr1_out) {
+  return guarded(ctx, [&] {
+    if (!rho || !n || !car2lat || !lat_i_dist) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) {
+      require_device_ptr(rho, "rho");
+      if (ongrid_next) require_device_ptr(ongrid_next, "ongrid_next");
+      if (neargrid_next0) require_device_ptr(neargrid_next0, "neargrid_next0");
+      if (dr1_out) require_device_ptr(dr1_out, "dr1_out");
+    }
+    step_pointers_run(ctx, rho, n, car2lat, lat_i_dist, flags, ongrid_next, neargrid_next0, dr1_out);
+  });
+}
+
+int c2gpu_bader(c2gpu_ctx *ctx, const double *rho, const int n[3], const double car2lat[9],
+                const double lat_i_dist[27], const c2gpu_attractor_opts *opts, int flags, int32_t *label_out,
+                int *nattr, double *xattr_out, int xattr_cap, c2gpu_bader_stats *stats) {
+  return guarded(ctx, [&] {
+    if (!rho || !n || !car2lat || !lat_i_dist || !label_out) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) {
+      require_device_ptr(rho, "rho");
+      require_device_ptr(label_out, "label_out");
+    }
+    bader_run(ctx, rho, n, car2lat, lat_i_dist, opts, flags, label_out, nattr, xattr_out, xattr_cap, stats);
+  });
+}
+
+int c2gpu_relabel(c2gpu_ctx *ctx, int64_t npts, const int32_t *map, int nmap, int flags, int32_t *label) {
+  return guarded(ctx, [&] {
+    if (!map || !label) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) require_device_ptr(label, "label");
+    relabel_run(ctx, npts, map, nmap, flags, label);
+  });
+}
+
+int c2gpu_basin_sums(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nattr, int nfields,
+                     const double *const *fields, int flags, double *sums_out, int64_t *counts_out) {
+  return guarded(ctx, [&] {
+    if (!label || (nfields > 0 && !fields)) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) {
+      require_device_ptr(label, "label");
+      for (int q = 0; q < nfields; q++) require_device_ptr(fields[q], "fields[k]");
+    }
+    basin_sums_run(ctx, label, npts, nattr, nfields, fields, flags, sums_out, counts_out);
+  });
+}
+
+int c2gpu_fft_deriv(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], int kind,
+                    int flags, double *out) {
+  return guarded(ctx, [&] {
+    if (!rho || !n || !x2c || !out) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) {
+      require_device_ptr(rho, "rho");
+      require_device_ptr(out, "out");
+    }
+    fft_deriv_run(ctx, rho, n, x2c, kind, flags, out);
+  });
+}
+
+int c2gpu_nci(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], int scheme,
+              int precision, const c2gpu_promol *promol, int zfast, int flags, double *crho_out,
+              double *cgrad_out) {
+  return guarded(ctx, [&] {
+    if (!n || !x2c || !crho_out || !cgrad_out) fail(C2GPU_ERR_ARG, "null argument");
+    if (scheme == C2GPU_NCI_FFT && !rho) fail(C2GPU_ERR_ARG, "null grid");
+    if (flags & C2GPU_PTR_DEVICE) {
+      if (scheme == C2GPU_NCI_FFT) require_device_ptr(rho, "rho");
+      require_device_ptr(crho_out, "crho_out");
+      require_device_ptr(cgrad_out, "cgrad_out");
+    }
+    nci_run(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 0);
+  });
+}
+
+int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], int scheme,
+                  int precision, const c2gpu_promol *promol, int zfast, int flags, float *crho_out,
+                  float *cgrad_out) {
+  return guarded(ctx, [&] {
+    if (!n || !x2c || !crho_out || !cgrad_out) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) {
+      if (scheme == C2GPU_NCI_FFT) require_device_ptr(rho, "rho");
+      require_device_ptr(crho_out, "crho_out");
+      require_device_ptr(cgrad_out, "cgrad_out");
+    }
+    nci_run(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 1);
+  });
+}
+
+int c2gpu_synth_slater(c2gpu_ctx *ctx, const double x2c[9], const int n[3], int nat, const double *xat,
+                       const int *zat, double rimg, int flags, double *rho_out) {
+  return guarded(ctx, [&] {
+    if (!x2c || !n || !xat || !zat || !rho_out) fail(C2GPU_ERR_ARG, "null argument");
+    if (flags & C2GPU_PTR_DEVICE) require_device_ptr(rho_out, "rho_out");
+    synth_slater_run(ctx, x2c, n, nat, xat, zat, rimg, flags, rho_out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// Shared host-side plumbing of libc2gpu: context, error reporting and a
+// grow-only device buffer cache.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/c2gpu.h"
+
+struct c2gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  // named grow-only device allocations reused across calls
+  struct Buf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Buf> bufs;
+  // per-kernel device timing (c2gpu_ctx_timing): events recorded around
+  // each launch while enabled, resolved at the end of every API call
+  bool timing = false;
+  struct Pending {
+    const char *name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> evpool;
+  struct KTime {
+    double ms = 0;
+    int64_t launches = 0;
+  };
+  std::map<std::string, KTime> ktime;
+};
+
+namespace c2 {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+inline void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+#define C2_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      ::c2::fail(C2GPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Returns a device buffer of at least `bytes` bytes registered under `name`
+// in the context (grow-only; contents are not preserved on growth).
+inline void *buffer(c2gpu_ctx *ctx, const std::string &name, size_t bytes) {
+  auto &b = ctx->bufs[name];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) {
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    C2_HIP(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    fail(C2GPU_ERR_ALLOC, "hipMalloc(" + std::to_string(want) + " B) for " + name + ": " +
+                              hipGetErrorString(e));
+  }
+  b.bytes = want;
+  return b.ptr;
+}
+
+template <class T>
+inline T *buf(c2gpu_ctx *ctx, const std::string &name, size_t count) {
+  return static_cast<T *>(buffer(ctx, name, count * sizeof(T)));
+}
+
+inline hipEvent_t event_get(c2gpu_ctx *ctx) {
+  if (!ctx->evpool.empty()) {
+    hipEvent_t e = ctx->evpool.back();
+    ctx->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  C2_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// Accumulate the recorded launch times (waits for the recorded events).
+inline void flush_timing(c2gpu_ctx *ctx) {
+  for (auto &p : ctx->pending) {
+    C2_HIP(hipEventSynchronize(p.b));
+    float ms = 0;
+    C2_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto &k = ctx->ktime[p.name];
+    k.ms += ms;
+    k.launches += 1;
+    ctx->evpool.push_back(p.a);
+    ctx->evpool.push_back(p.b);
+  }
+  ctx->pending.clear();
+}
+
+// Kernel launch on the context's stream, timed when timing is enabled.
+#define C2_LAUNCH(ctx, kern, grid, block, lds, ...)                            \
+  do {                                                                         \
+    hipEvent_t a_ = nullptr, b_ = nullptr;                                     \
+    if ((ctx)->timing) {                                                       \
+      a_ = ::c2::event_get(ctx);                                               \
+      b_ = ::c2::event_get(ctx);                                               \
+      C2_HIP(hipEventRecord(a_, (ctx)->stream));                               \
+    }                                                                          \
+    hipLaunchKernelGGL(kern, grid, block, lds, (ctx)->stream, __VA_ARGS__);    \
+    C2_HIP(hipGetLastError());                                                 \
+    if ((ctx)->timing) {                                                       \
+      C2_HIP(hipEventRecord(b_, (ctx)->stream));                               \
+      (ctx)->pending.push_back({#kern, a_, b_});                               \
+    }                                                                          \
+  } while (0)
+
+inline void release_all(c2gpu_ctx *ctx) {
+  for (auto &kv : ctx->bufs)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  ctx->bufs.clear();
+}
+
+// Runs f(), converting exceptions into a status code and ctx->err.
+template <class F>
+inline int guarded(c2gpu_ctx *ctx, F &&f) {
+  if (!ctx) return C2GPU_ERR_ARG;
+  try {
+    ctx->err.clear();
+    C2_HIP(hipSetDevice(ctx->device));
+    f();
+    flush_timing(ctx);
+    return C2GPU_OK;
+  } catch (const Error &e) {
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    return C2GPU_ERR_ARG;
+  }
+}
+
+// With C2GPU_PTR_DEVICE the caller promises device memory: verify it, so a
+// host pointer is rejected with C2GPU_ERR_ARG instead of faulting the GPU.
+inline void require_device_ptr(const void *p, const char *what) {
+  if (!p) fail(C2GPU_ERR_ARG, std::string(what) + " is NULL");
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(C2GPU_ERR_ARG, std::string(what) + " is not a device pointer (C2GPU_PTR_DEVICE set)");
+  }
+  if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)
+    fail(C2GPU_ERR_ARG, std::string(what) + " is not device memory (C2GPU_PTR_DEVICE set)");
+}
+
+inline unsigned grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffff) g = 0x7fffffff;
+  return (unsigned)g;
+}
+
+// Number of resident threads used by persistent (grid-stride) kernels.
+inline int persistent_threads(int device, int block) {
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  return ncu * 8 * block;
+}
+
+}  // namespace c2

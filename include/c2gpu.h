@@ -1,0 +1,226 @@
+/*
+ * c2gpu.h -- C ABI of the MI355X (gfx950) implementation of critic2's
+ * on-grid density-analysis path: near-grid Bader basin labelling, per-basin
+ * integration, and the NCI reduced density gradient.
+ *
+ * All entry points return an int status (C2GPU_OK == 0).  A thin Fortran
+ * shim turns a non-zero status into ferror(...,faterr), which keeps the
+ * keyword-level behaviour of critic2 (tools_io@proc.F90:1573-1618).
+ *
+ * Array conventions follow critic2:
+ *   - 3-D grids are Fortran column-major f(n1,n2,n3): x (n1) fastest.
+ *   - 3x3 matrices are column-major, m[r + 3*c] == m(r+1,c+1).
+ *   - lat_i_dist is lat_i_dist(d1,d2,d3), d in -1..1, stored
+ *     [(d1+1) + 3*(d2+1) + 9*(d3+1)] (bader@proc.f90:62-63).
+ * Array arguments are host memory unless flags has C2GPU_PTR_DEVICE, in
+ * which case the grid/label/field/output arrays are device pointers on the
+ * context's device (small parameter arrays are always host memory).
+ *
+ * Contexts are independent and reentrant (critic2's bader module keeps
+ * module-global state; this library keeps everything in the context).
+ * Calls on one context are serialised on the context's stream.
+ */
+#ifndef C2GPU_H
+#define C2GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  C2GPU_OK = 0,
+  C2GPU_ERR_ARG = 1,          /* invalid argument                          */
+  C2GPU_ERR_HIP = 2,          /* HIP runtime error                         */
+  C2GPU_ERR_ALLOC = 3,        /* device allocation failed                  */
+  C2GPU_ERR_UNSUPPORTED = 4,  /* option not implemented on the GPU path    */
+  C2GPU_ERR_NOCONVERGE = 5,   /* first-sweep fixpoint did not converge     */
+  C2GPU_ERR_NEWMAX = 6,       /* bader@proc.f90:400-402: new maximum in
+                                 edge refinement (fatal in critic2 too)    */
+  C2GPU_ERR_PATHLEN = 7,      /* an ascent path exceeded the path limit    */
+  C2GPU_ERR_DEGENERATE = 8    /* plateau case whose serial outcome depends
+                                 on visiting order inside one sweep        */
+};
+
+enum { C2GPU_PTR_DEVICE = 1 };
+
+typedef struct c2gpu_ctx c2gpu_ctx;
+
+/* Context management.  device: HIP device ordinal. */
+int c2gpu_ctx_create(c2gpu_ctx **ctx, int device);
+int c2gpu_ctx_destroy(c2gpu_ctx *ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char *c2gpu_ctx_error(const c2gpu_ctx *ctx);
+/* Use an existing hipStream_t (NULL: the context's own stream). */
+int c2gpu_ctx_set_stream(c2gpu_ctx *ctx, void *hip_stream);
+/* Wait for all work queued on the context's stream. */
+int c2gpu_ctx_sync(c2gpu_ctx *ctx);
+
+/* Device memory for callers that keep grids resident on the GPU (no
+ * counterpart in critic2, whose arrays live in host memory).  kind for
+ * c2gpu_memcpy: 1 host->device, 2 device->host, 3 device->device. */
+int c2gpu_dev_alloc(c2gpu_ctx *ctx, uint64_t bytes, void **ptr);
+int c2gpu_dev_free(c2gpu_ctx *ctx, void *ptr);
+int c2gpu_memcpy(c2gpu_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
+
+/* Per-kernel device timing, the counterpart of critic2's tictac wall-clock
+ * stamps (tools_io@proc.F90:1646) at kernel granularity: while enabled, each
+ * kernel launch is bracketed by HIP events on the context's stream and its
+ * time accumulated under the kernel's name.  c2gpu_ctx_timing resets the
+ * table.  c2gpu_ctx_kernel_time returns the accumulated milliseconds and
+ * launch count of one kernel (0 and 0 if it never ran); names_out receives
+ * the comma-separated kernel names (truncated to len bytes). */
+int c2gpu_ctx_timing(c2gpu_ctx *ctx, int enable);
+int c2gpu_ctx_kernel_time(c2gpu_ctx *ctx, const char *kernel, double *ms, int64_t *launches);
+int c2gpu_ctx_kernel_names(c2gpu_ctx *ctx, char *names_out, uint64_t len);
+
+/* ------------------------------------------------------------------ */
+/* Bader (near-grid, Henkelman et al.)                                 */
+/* ------------------------------------------------------------------ */
+
+/* Grid metrics of bader_integrate (bader@proc.f90:123-145):
+ * lat2car(:,i) = x2c(:,i)/n(i); car2lat = matinv(lat2car) with the
+ * dgetrf/dgetri operation order (tools_math@proc.f90:1203-1235);
+ * lat_i_dist(d) = 1/|lat2car*d| (0 at d = 0). */
+int c2gpu_bader_metrics(const double x2c[9], const int n[3], double lat2car[9],
+                        double car2lat[9], double lat_i_dist[27]);
+
+/* Per-point, order-free step table (step_neargrid/step_ongrid/is_max of
+ * bader@proc.f90:455-597 evaluated from every grid point with dr = 0):
+ *   ongrid_next[p]    step_ongrid target of p, or p itself if is_max(p);
+ *   neargrid_next0[p] target of the first step_neargrid call from p on a
+ *                     fresh path (known all 0, dr = 0), p if no move;
+ *   dr1_out[3*p+c]    dr after that step (0 when an on-grid step was taken).
+ * Indices are 0-based linear grid indices.  Any output may be NULL. */
+int c2gpu_bader_step_pointers(c2gpu_ctx *ctx, const double *rho, const int n[3],
+                              const double car2lat[9], const double lat_i_dist[27],
+                              int flags, int64_t *ongrid_next,
+                              int64_t *neargrid_next0, double *dr1_out);
+
+/* Attractor assignment options of BADER (integration@proc.f90:236-248). */
+enum {
+  C2GPU_ATTR_NEAREST_ATOM = 0, /* default: atexist, ratom = 1d40 (nonnm)   */
+  C2GPU_ATTR_RATOM = 1,        /* NNM / RATOM r                            */
+  C2GPU_ATTR_NOATOMS = 2       /* NOATOMS                                  */
+};
+typedef struct {
+  int mode;           /* C2GPU_ATTR_*                                      */
+  double ratom;       /* bohr (ignored for NEAREST_ATOM)                   */
+  double x2c[9];      /* lattice vectors as columns (bohr), crystal m_x2c  */
+  int nat;            /* atoms of the complete cell list (cpcel order)     */
+  const double *xat;  /* fractional atom coordinates, 3*nat, host memory   */
+} c2gpu_attractor_opts;
+
+typedef struct {
+  int nattr;            /* bas%nattr on return                            */
+  int nmaxima;          /* distinct grid maxima reached by the sweep      */
+  int sweep_iters;      /* fixpoint iterations of the first sweep         */
+  int refine_iters;     /* refine_edge passes (bader@proc.f90:218-224)    */
+  int64_t nstarts;      /* paths started by the first sweep               */
+  int64_t traj_points;  /* stored free-trajectory points                  */
+  int64_t edge_points;  /* points re-ascended in the first refine pass    */
+  double ms_traj, ms_sweep, ms_refine, ms_total; /* device time per phase */
+} c2gpu_bader_stats;
+
+/* bader_integrate(s,bas,iref) (bader@proc.f90:80-234; interface
+ * bader.f90:48-53).  label_out receives bas%idg (1-based attractor ids,
+ * before int_reorder_gridout); xattr_out (host, 3*xattr_cap) receives
+ * bas%xattr in fractional coordinates; *nattr receives bas%nattr.
+ * car2lat and lat_i_dist are taken as given so that the caller fixes their
+ * bits (critic2 computes them with LAPACK).  stats may be NULL. */
+int c2gpu_bader(c2gpu_ctx *ctx, const double *rho, const int n[3],
+                const double car2lat[9], const double lat_i_dist[27],
+                const c2gpu_attractor_opts *opts, int flags, int32_t *label_out,
+                int *nattr, double *xattr_out, int xattr_cap,
+                c2gpu_bader_stats *stats);
+
+/* where (idg == i) idg = map(i) for i = 1..nmap (the idg update of
+ * int_reorder_gridout, integration@proc.f90:1113-1122). */
+int c2gpu_relabel(c2gpu_ctx *ctx, int64_t npts, const int32_t *map, int nmap,
+                  int flags, int32_t *label);
+
+/* Per-basin sums (intgrid_fields, integration@proc.f90:1208-1218 and
+ * :1289-1300) without the Omega/N factor, which stays with the caller:
+ *   counts_out[a]          = count(idg == a+1)
+ *   sums_out[k*nattr + a]  = sum(field_k, idg == a+1)
+ * fields: nfields pointers to N-point FP64 grids.  Deterministic: the
+ * summation order is fixed for a given grid size.  sums_out/counts_out are
+ * host memory. */
+int c2gpu_basin_sums(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nattr,
+                     int nfields, const double *const *fields, int flags,
+                     double *sums_out, int64_t *counts_out);
+
+/* ------------------------------------------------------------------ */
+/* FFT derivative grids and NCI                                        */
+/* ------------------------------------------------------------------ */
+
+enum {
+  C2GPU_FT_GRAD = 0, /* |grad f| (ifformat_as_ft_grad)                    */
+  C2GPU_FT_XX = 1,   /* d2f/dx2 (ifformat_as_ft_xx)                       */
+  C2GPU_FT_YY = 2,
+  C2GPU_FT_ZZ = 3,
+  C2GPU_FT_LAP = 4   /* Laplacian (ifformat_as_ft_lap)                    */
+};
+
+/* grid3%fft(fnew, fold, iff) (grid3mod@proc.f90:1757-1872): spectral
+ * derivative of a periodic grid, Nyquist components at +n/2. */
+int c2gpu_fft_deriv(c2gpu_ctx *ctx, const double *rho, const int n[3],
+                    const double x2c[9], int kind, int flags, double *out);
+
+/* Radial density tables of the promolecular density (grid1 type,
+ * grid1mod.f90:26-48): nodes r (a = r(1), b = log(r(2)/r(1)), the index
+ * formula of grid1%interp assumes r_i = a*exp(b*(i-1))), f = rho,
+ * fp = rho', fpp = rho'', rmax = r(ngrid).  cutrad: cutoff radius of the
+ * species (global.f90:53, bohr). */
+typedef struct {
+  int ngrid;
+  double a, b, rmax, cutrad;
+  const double *r, *f, *fp, *fpp; /* host arrays, ngrid each               */
+} c2gpu_radial_table;
+
+typedef struct {
+  int nspecies;
+  const c2gpu_radial_table *tables; /* nspecies tables                    */
+  int nat;                          /* atoms in the cell                   */
+  const double *xat;                /* fractional coordinates, 3*nat       */
+  const int *species;               /* table index of each atom            */
+} c2gpu_promol;
+
+enum { C2GPU_NCI_FFT = 0, C2GPU_NCI_PROMOL = 2 };
+enum { C2GPU_FP64 = 0, C2GPU_FP32 = 1 };
+
+/* The NCIPLOT per-point block (nci@proc.f90:525-605) on the nodes of a
+ * periodic grid:
+ *   cgrad = |grad rho| / (2 (3 pi^2)^(1/3) max(rho, vsmall)^(4/3))
+ *   crho  = sign(rho, lambda2) * 100
+ * scheme C2GPU_NCI_FFT: rho is the loaded FP64 grid; derivatives by FFT and
+ *   the lambda2 sign from count(diag Hessian > 0) >= 2 (nci@proc.f90:546-564).
+ * scheme C2GPU_NCI_PROMOL: rho is ignored; density, gradient and Hessian from
+ *   the promolecular radial tables (crystalmod@env.f90:622-753), lambda2 from
+ *   the Hessian eigenvalues (nci@proc.f90:565-571); precision selects FP64 or
+ *   FP32 arithmetic.
+ * crho_out/cgrad_out are written z-fastest, (0:n3-1,0:n2-1,0:n1-1) as in
+ * nci@proc.f90:502-505, unless zfast == 0 (then x-fastest like the input). */
+int c2gpu_nci(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9],
+              int scheme, int precision, const c2gpu_promol *promol, int zfast,
+              int flags, double *crho_out, double *cgrad_out);
+
+/* FP32-output variant of c2gpu_nci (same arguments, float outputs). */
+int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9],
+                  int scheme, int precision, const c2gpu_promol *promol, int zfast,
+                  int flags, float *crho_out, float *cgrad_out);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic input (no counterpart in critic2: test and benchmark input) */
+/* ------------------------------------------------------------------ */
+
+/* rho = sum_{atoms,images within rimg} Z zeta^3/(8 pi) exp(-zeta |r-R|),
+ * zeta = 2 Z^(1/3), evaluated in FP64 on the grid nodes x = (i/n1,j/n2,k/n3). */
+int c2gpu_synth_slater(c2gpu_ctx *ctx, const double x2c[9], const int n[3], int nat,
+                       const double *xat, const int *zat, double rimg, int flags,
+                       double *rho_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

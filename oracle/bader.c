@@ -621,3 +621,36 @@ void oracle_relabel(int32_t *idg, int64_t npts, const int32_t *map, int nmap) {
     if (a >= 1 && a <= nmap) idg[p] = map[a - 1];
   }
 }
+
+/* Step table of c2gpu_bader_step_pointers, from the restated routines:
+ * step_ongrid target (p if is_max), first step_neargrid target of a fresh
+ * path from p (dr = 0, only p known), and dr after that step. */
+void oracle_step_pointers(const double *f, const int n[3], const double car2lat[9],
+                          const double lat_i_dist[27], int64_t *ongrid, int64_t *near0,
+                          double *dr1) {
+  bstate s;
+  memset(&s, 0, sizeof(s));
+  s.f = f;
+  s.n[0] = n[0]; s.n[1] = n[1]; s.n[2] = n[2];
+  s.N = (int64_t)n[0] * n[1] * n[2];
+  memcpy(s.car2lat, car2lat, sizeof(s.car2lat));
+  memcpy(s.lid, lat_i_dist, sizeof(s.lid));
+  s.known = (int32_t *)calloc((size_t)s.N, sizeof(int32_t));
+  s.volnum = (int32_t *)calloc((size_t)s.N, sizeof(int32_t));
+  for (int k = 0; k < n[2]; k++)
+    for (int j = 0; j < n[1]; j++)
+      for (int i = 0; i < n[0]; i++) {
+        int64_t id = lin(&s, i, j, k);
+        int p[3] = {i, j, k};
+        if (is_max(&s, p)) ongrid[id] = id;
+        else { step_ongrid(&s, p); ongrid[id] = lin(&s, p[0], p[1], p[2]); }
+        p[0] = i; p[1] = j; p[2] = k;
+        s.pnum = 1;
+        step_neargrid(&s, p);
+        near0[id] = lin(&s, p[0], p[1], p[2]);
+        dr1[3 * id] = s.dr[0]; dr1[3 * id + 1] = s.dr[1]; dr1[3 * id + 2] = s.dr[2];
+        s.known[id] = 0;
+      }
+  free(s.known);
+  free(s.volnum);
+}

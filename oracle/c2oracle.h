@@ -72,6 +72,11 @@ void oracle_bader_metrics(const double x2c[9], const int n[3], double lat2car[9]
 void oracle_basin_sums(const int32_t *idg, int64_t npts, int nattr, int nfields,
                        const double *const *fields, int64_t *counts, double *sums);
 
+/* Step table (see c2gpu_bader_step_pointers). */
+void oracle_step_pointers(const double *f, const int n[3], const double car2lat[9],
+                          const double lat_i_dist[27], int64_t *ongrid, int64_t *near0,
+                          double *dr1);
+
 /* Relabel: idg = map[idg-1] for idg >= 1 (int_reorder_gridout remap). */
 void oracle_relabel(int32_t *idg, int64_t npts, const int32_t *map, int nmap);
 

@@ -1,0 +1,265 @@
+"""Python access to the oracle (CPU restatement of critic2's path).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Never used by critic2_amd.
+
+C functions (liboracle.so, built by oracle/Makefile) restate
+bader@proc.f90, intgrid_fields, grid1%interp / read_critic,
+promolecular_atom and the NCI point formulas; see the file headers of
+bader.c and nci.c for the line-by-line references.  The FFT derivative grids
+of grid3%fft (grid3mod@proc.f90:1757-1872) are restated here with numpy's
+FFT: same transform conventions (forward exp(-i), scaled 1/N; backward
+exp(+i), unscaled), same reciprocal vectors and Nyquist handling; only the
+round-off of the FFT itself differs from fftpack5.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+class OrcGeom(C.Structure):
+    _fields_ = [("x2c", C.c_double * 9), ("nat", C.c_int), ("xat", C.c_void_p), ("mode", C.c_int),
+                ("ratom", C.c_double)]
+
+
+class OrcInfo(C.Structure):
+    _fields_ = [("nattr", C.c_int), ("nmax_found", C.c_int), ("refine_iters", C.c_int),
+                ("nstarts", C.c_int64), ("nsteps", C.c_int64), ("status", C.c_int)]
+
+
+class OrcRgrid(C.Structure):
+    _fields_ = [("ngrid", C.c_int), ("a", C.c_double), ("b", C.c_double), ("rmax", C.c_double),
+                ("r", C.c_void_p), ("f", C.c_void_p), ("fp", C.c_void_p), ("fpp", C.c_void_p)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        lib.oracle_bader.argtypes = [vp, vp, vp, vp, C.POINTER(OrcGeom), vp, vp, C.c_int, C.POINTER(OrcInfo)]
+        lib.oracle_bader_sweep_only.argtypes = [vp, vp, vp, vp, C.POINTER(OrcGeom), vp, vp, vp, C.c_int,
+                                                C.POINTER(OrcInfo)]
+        lib.oracle_bader_metrics.argtypes = [vp, vp, vp, vp, vp]
+        lib.oracle_basin_sums.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp]
+        lib.oracle_grid1_build.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        lib.oracle_grid1_interp.argtypes = [C.POINTER(OrcRgrid), C.c_double, vp, vp, vp]
+        lib.oracle_promol.argtypes = [vp, C.c_int, vp, vp, C.POINTER(OrcRgrid), vp, C.c_int, vp, vp, vp]
+        lib.oracle_nci_point.argtypes = [C.c_double, C.c_double, vp, C.c_int, vp, vp]
+        lib.oracle_slater_grid.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
+        _lib = lib
+    return _lib
+
+
+def _matf(m):
+    return np.asfortranarray(np.asarray(m, dtype=np.float64)).reshape(-1, order="F").copy()
+
+
+def metrics(x2c, n):
+    lib = load()
+    x = _matf(x2c)
+    nn = np.asarray(n, dtype=np.int32)
+    l2c, c2l, lid = np.zeros(9), np.zeros(9), np.zeros(27)
+    lib.oracle_bader_metrics(x.ctypes.data, nn.ctypes.data, l2c.ctypes.data, c2l.ctypes.data, lid.ctypes.data)
+    return l2c, c2l, lid
+
+
+def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, lat_i_dist=None):
+    """Serial bader_integrate; returns (labels int32 x-fastest, info dict,
+    xattr (nattr,3), known or None)."""
+    lib = load()
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    nn = np.asarray(n, dtype=np.int32)
+    if car2lat is None:
+        _, car2lat, lat_i_dist = metrics(x2c, n)
+    c2l = np.ascontiguousarray(car2lat, dtype=np.float64)
+    lid = np.ascontiguousarray(lat_i_dist, dtype=np.float64)
+    xa = np.ascontiguousarray(np.asarray(xat, dtype=np.float64).reshape(-1))
+    g = OrcGeom()
+    for q, v in enumerate(_matf(x2c)):
+        g.x2c[q] = v
+    g.nat = len(xa) // 3
+    g.xat = xa.ctypes.data
+    g.mode = mode
+    g.ratom = ratom
+    lab = np.zeros(f.size, dtype=np.int32)
+    cap = 4 * g.nat + 256
+    xattr = np.zeros(3 * cap)
+    info = OrcInfo()
+    known = None
+    if sweep_only:
+        known = np.zeros(f.size, dtype=np.int32)
+        lib.oracle_bader_sweep_only(f.ctypes.data, nn.ctypes.data, c2l.ctypes.data, lid.ctypes.data,
+                                    C.byref(g), lab.ctypes.data, known.ctypes.data, xattr.ctypes.data,
+                                    cap, C.byref(info))
+    else:
+        lib.oracle_bader(f.ctypes.data, nn.ctypes.data, c2l.ctypes.data, lid.ctypes.data, C.byref(g),
+                         lab.ctypes.data, xattr.ctypes.data, cap, C.byref(info))
+    d = {k: getattr(info, k) for k, _ in info._fields_}
+    return lab, d, xattr[: 3 * info.nattr].reshape(-1, 3), known
+
+
+def basin_sums(labels, nattr, fields=()):
+    lib = load()
+    labels = np.ascontiguousarray(labels, dtype=np.int32).reshape(-1)
+    fields = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1) for f in fields]
+    fp = (C.c_void_p * max(len(fields), 1))(*[f.ctypes.data for f in fields])
+    counts = np.zeros(nattr, dtype=np.int64)
+    sums = np.zeros(max(len(fields), 1) * nattr)
+    lib.oracle_basin_sums(labels.ctypes.data, labels.size, nattr, len(fields), fp, counts.ctypes.data,
+                          sums.ctypes.data)
+    return counts, sums[: len(fields) * nattr].reshape(len(fields), nattr)
+
+
+def grid1_build(r, rr0):
+    lib = load()
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    rr0 = np.ascontiguousarray(rr0, dtype=np.float64)
+    f, fp, fpp = np.zeros_like(r), np.zeros_like(r), np.zeros_like(r)
+    a, b = C.c_double(), C.c_double()
+    lib.oracle_grid1_build(len(r), r.ctypes.data, rr0.ctypes.data, f.ctypes.data, fp.ctypes.data,
+                           fpp.ctypes.data, C.byref(a), C.byref(b))
+    return f, fp, fpp, a.value, b.value
+
+
+class Table:
+    """Oracle-side radial table (keeps the arrays alive)."""
+
+    def __init__(self, r, f, fp, fpp, cutrad):
+        self.r, self.f, self.fp, self.fpp = (np.ascontiguousarray(v, dtype=np.float64) for v in (r, f, fp, fpp))
+        self.s = OrcRgrid()
+        self.s.ngrid = len(self.r)
+        self.s.a = float(self.r[0])
+        self.s.b = math.log(self.r[1] / self.r[0])
+        self.s.rmax = float(self.r[-1])
+        self.s.r, self.s.f, self.s.fp, self.s.fpp = (v.ctypes.data for v in (self.r, self.f, self.fp, self.fpp))
+        self.cutrad = cutrad
+
+
+def promol_nci(points_cart, images, img_species, tables, full_hessian=True):
+    """rho, |grad|, s, crho at Cartesian points; images: (m,3) Cartesian
+    positions of every atom image to include; img_species: table index."""
+    lib = load()
+    tabs = (OrcRgrid * len(tables))(*[t.s for t in tables])
+    cut = np.array([min(t.cutrad, t.s.rmax) for t in tables], dtype=np.float64)
+    ximg = np.ascontiguousarray(np.asarray(images, dtype=np.float64).reshape(-1))
+    zi = np.ascontiguousarray(np.asarray(img_species, dtype=np.int32))
+    pts = np.asarray(points_cart, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(pts), 4))
+    g = np.zeros(3)
+    h = np.zeros(9)
+    f = C.c_double()
+    cr, s = C.c_double(), C.c_double()
+    for q, x in enumerate(pts):
+        xc = np.ascontiguousarray(x)
+        lib.oracle_promol(xc.ctypes.data, len(zi), ximg.ctypes.data, zi.ctypes.data, tabs, cut.ctypes.data, 2,
+                          C.byref(f), g.ctypes.data, h.ctypes.data)
+        # nucleus: gradient nullified within 1e-5 bohr (fieldmod@proc.f90:1146-1155)
+        d = np.sqrt(((np.asarray(images).reshape(-1, 3) - x) ** 2).sum(1)).min() if len(zi) else 1.0
+        gm = 0.0 if d <= 1e-5 else float(np.sqrt((g * g).sum()))
+        lib.oracle_nci_point(f.value, gm, h.ctypes.data, 1 if full_hessian else 0, C.byref(cr), C.byref(s))
+        out[q] = (f.value, gm, s.value, cr.value)
+    return out
+
+
+def nci_point(rho, gmod, hdiag):
+    lib = load()
+    e = np.zeros(9)
+    e[:3] = hdiag
+    cr, s = C.c_double(), C.c_double()
+    lib.oracle_nci_point(float(rho), float(gmod), e.ctypes.data, 0, C.byref(cr), C.byref(s))
+    return cr.value, s.value
+
+
+def slater_grid(x2c, n, xat, zat, rimg_bohr):
+    lib = load()
+    out = np.zeros(int(np.prod(n)))
+    x = _matf(x2c)
+    nn = np.asarray(n, dtype=np.int32)
+    xa = np.ascontiguousarray(np.asarray(xat, dtype=np.float64).reshape(-1))
+    za = np.ascontiguousarray(np.asarray(zat, dtype=np.int32))
+    lib.oracle_slater_grid(x.ctypes.data, nn.ctypes.data, len(za), xa.ctypes.data, za.ctypes.data,
+                           float(rimg_bohr), out.ctypes.data)
+    return out
+
+
+# ---- grid3%fft restated with numpy (grid3mod@proc.f90:1757-1872)
+
+def _vgc(x2c, n):
+    x2c = np.asarray(x2c, dtype=np.float64)
+    b = np.zeros((3, 3))
+    b[:, 0] = np.cross(x2c[:, 2], x2c[:, 1])
+    b[:, 1] = np.cross(x2c[:, 0], x2c[:, 2])
+    b[:, 2] = np.cross(x2c[:, 1], x2c[:, 0])
+    b = 2.0 * math.pi / abs(np.linalg.det(x2c)) * b
+
+    def idx(m):
+        i = np.arange(m)
+        return np.where(i <= m // 2, i, i - m).astype(np.float64)
+
+    i1, i2, i3 = np.meshgrid(idx(n[0]), idx(n[1]), idx(n[2]), indexing="ij")
+    return [i1 * b[c, 0] + i2 * b[c, 1] + i3 * b[c, 2] for c in range(3)]
+
+
+def fft_deriv(f, n, x2c, kind):
+    """kind: 'grad', 'xx', 'yy', 'zz', 'lap'.  f is x-fastest flat."""
+    g = np.asarray(f, dtype=np.float64).reshape(n[2], n[1], n[0]).transpose(2, 1, 0)  # (n1,n2,n3)
+    N = g.size
+    z = np.fft.fftn(g) / N
+    v = _vgc(x2c, n)
+    if kind == "grad":
+        acc = np.zeros(g.shape)
+        for c in range(3):
+            w = v[c] * (1j * z)
+            acc += np.real(np.fft.ifftn(w) * N) ** 2
+        out = np.sqrt(acc)
+    elif kind in ("xx", "yy", "zz"):
+        c = "xyz".index(kind[0])
+        out = np.real(np.fft.ifftn(-v[c] * v[c] * z) * N)
+    elif kind == "lap":
+        out = np.real(np.fft.ifftn(-(v[0] ** 2 + v[1] ** 2 + v[2] ** 2) * z) * N)
+    else:
+        raise ValueError(kind)
+    return out.transpose(2, 1, 0).reshape(-1)
+
+
+def nci_grid_fft(f, n, x2c):
+    """NCI node quantities of the fourierint branch (nci@proc.f90:546-564),
+    x-fastest flat arrays (crho, s)."""
+    f = np.asarray(f, dtype=np.float64).reshape(-1)
+    gm = fft_deriv(f, n, x2c, "grad")
+    hs = [fft_deriv(f, n, x2c, k) for k in ("xx", "yy", "zz")]
+    npos = sum((h > 0).astype(int) for h in hs)
+    cst = 2.0 * (3.0 * math.pi ** 2) ** (1.0 / 3.0)
+    s = gm / (cst * np.maximum(f, 1e-80) ** (4.0 / 3.0))
+    crho = np.where(npos >= 2, np.abs(f), -np.abs(f)) * 100.0
+    return crho, s
+
+
+def step_pointers(f, n, x2c):
+    lib = load()
+    lib.oracle_step_pointers.argtypes = [C.c_void_p] * 7
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    _, c2l, lid = metrics(x2c, n)
+    nn = np.asarray(n, dtype=np.int32)
+    on = np.zeros(f.size, dtype=np.int64)
+    ne = np.zeros(f.size, dtype=np.int64)
+    dr = np.zeros(3 * f.size)
+    lib.oracle_step_pointers(f.ctypes.data, nn.ctypes.data, c2l.ctypes.data, lid.ctypes.data,
+                             on.ctypes.data, ne.ctypes.data, dr.ctypes.data)
+    return on, ne, dr

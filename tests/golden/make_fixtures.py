@@ -1,0 +1,83 @@
+"""Write the golden fixtures this repository's tests read.
+
+Run in a checkout that has the reference mounted at /root/reference (the
+GPU box does not; the fixtures are committed).  Everything written is data
+from the reference's own test and data files:
+
+  urea_nci.json      urea cell and atoms (dat/lib/crystal.dat 'structure urea')
+                     and the expected node values of the reference test
+                     tests/015_grdplot/005_nciplot_basic (its ref/*-dens.cube,
+                     ref/*-grad.cube and ref/*.dat), plus the test's options.
+  urea_tables.npz    radial grids r and rr0 = sum(occ*psi^2) of the H, C, N, O
+                     atomic wfc files (dat/wfc/*_pbe.wfc), truncated where
+                     critic2's read_critic stops reading (rho < 1e-8).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", ".."))
+from critic2_amd.radial import read_wfc  # noqa: E402
+
+
+def urea_structure():
+    lines = open(os.path.join(REF, "dat/lib/crystal.dat")).read().splitlines()
+    i = lines.index("structure urea")
+    cell, atoms = None, []
+    for ln in lines[i + 1:]:
+        t = ln.split()
+        if not t:
+            continue
+        if t[0] == "cell":
+            cell = [float(v) for v in t[1:7]]
+        elif t[0] == "neq":
+            atoms.append(([float(v) for v in t[1:4]], t[4]))
+        elif t[0] == "endcrystal":
+            break
+    return cell, atoms
+
+
+def cube_values(path):
+    toks = open(path).read().splitlines()
+    nat = int(toks[2].split()[0])
+    vals = " ".join(toks[6 + nat:]).split()
+    return [float(v.replace("D", "E")) for v in vals]
+
+
+def main():
+    cell, atoms = urea_structure()
+    base = os.path.join(REF, "tests/015_grdplot/ref/005_nciplot_basic-gen")
+    dens = cube_values(base + "-dens.cube")
+    grad = cube_values(base + "-grad.cube")
+    dat = [[float(v) for v in ln.split()] for ln in open(base + ".dat") if ln.strip()]
+    out = {
+        "source": "tests/015_grdplot/005_nciplot_basic.cri and dat/lib/crystal.dat (urea)",
+        "cell_bohr_deg": cell,
+        "atoms": [{"x": x, "sym": s} for x, s in atoms],
+        "nstep": [2, 2, 2],
+        "options": {"rhocut": 0.2, "dimcut": 2.0, "rhoplot": 0.5, "dimplot": 0.05,
+                    "srhorange": [-5.0, 5.0]},
+        "cube_order": "z fastest: value index = k + 2*(j + 2*i)",
+        "dens_cube": dens,
+        "grad_cube": grad,
+        "dat": dat,
+        "cube_atol": 1e-5,
+    }
+    with open(os.path.join(HERE, "urea_nci.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    zmap = {"h": 1, "c": 6, "n": 7, "o": 8}
+    arrs = {}
+    for sym, z in zmap.items():
+        fn = os.path.join(REF, "dat/wfc", f"{sym}__pbe.wfc")
+        r, rr0 = read_wfc(fn, z)
+        arrs[f"r_{sym}"] = r
+        arrs[f"rr0_{sym}"] = rr0
+    np.savez_compressed(os.path.join(HERE, "urea_tables.npz"), **arrs)
+
+
+if __name__ == "__main__":
+    main()

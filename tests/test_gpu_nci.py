@@ -1,0 +1,104 @@
+"""GPU parity of the NCI path: promolecular FP64 against the reference's
+golden urea values and the oracle; FP32 against FP64 (error reported);
+FFT scheme against the oracle's grid3%fft restatement."""
+import math
+
+import numpy as np
+import pytest
+
+from critic2_amd import _lib
+from critic2_amd import bader as gb
+from critic2_amd import nci as gn
+from critic2_amd.radial import slater_grid1
+from critic2_amd.synth import config, random_cell
+from oracle import oracle
+from tests.helpers import urea, fmt6, images, node_points
+
+pytestmark = pytest.mark.gpu
+
+
+def test_urea_golden_on_gpu_fp64(ctx):
+    """tests/015_grdplot/005_nciplot_basic through the GPU promolecular path."""
+    d, x2c, tables, xat, species = urea()
+    crho, s = gn.nciplot_promol(ctx, d["nstep"], x2c, tables, xat, species, precision="fp64")
+    o = d["options"]
+    dat, cg = gn.cutoffs(crho, s, rhocut=o["rhocut"], dimcut=o["dimcut"], rhoplot=o["rhoplot"],
+                         srhorange=o["srhorange"])
+    assert [fmt6(v) for v in crho] == d["dens_cube"]
+    for v, ref in zip(cg, d["grad_cube"]):
+        if abs(ref) > 1e-10:
+            assert fmt6(v) == ref
+        else:
+            assert abs(v - ref) < d["cube_atol"]
+    np.testing.assert_allclose(dat[:, 0], np.array(d["dat"])[:, 0], rtol=1e-7)
+
+
+def _slater_setup(cell):
+    zs = sorted(set(int(z) for z in cell.zat))
+    tables = [slater_grid1(z) for z in zs]
+    species = np.array([zs.index(int(z)) for z in cell.zat], dtype=np.int32)
+    return tables, species
+
+
+def test_promol_fp64_matches_oracle(ctx):
+    cell = random_cell(31, (6.0, 6.5, 7.0), (90, 90, 90), 5, (1, 6, 7, 8), (10, 11, 12))
+    tables, species = _slater_setup(cell)
+    crho, s = gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp64")
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    pos, idx = images(cell.x2c, cell.xat, lim=3)
+    ref = oracle.promol_nci(node_points(cell.x2c, cell.n), pos, species[idx], otabs)
+    np.testing.assert_allclose(np.abs(crho), np.abs(ref[:, 3]), rtol=1e-12, atol=1e-300)
+    ok = ref[:, 0] > 1e-12
+    np.testing.assert_allclose(s[ok], ref[ok, 2], rtol=1e-9)
+    # lambda2 sign: identical except where lambda2 is round-off close to zero
+    agree = np.mean(np.sign(crho) == np.sign(ref[:, 3]))
+    assert agree > 0.999
+
+
+def test_promol_fp32_error_against_fp64(ctx):
+    """Mixed precision: rho, |grad rho| and s in FP32 against the FP64 path,
+    error reported in the NCI region (s < 2)."""
+    cell = random_cell(32, (8.0, 8.0, 8.0), (90, 90, 90), 8, (1, 6, 7, 8), (48, 48, 48))
+    tables, species = _slater_setup(cell)
+    c64, s64 = gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp64")
+    c32, s32 = gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp32")
+    rho64, rho32 = np.abs(c64) / 100, np.abs(c32.astype(np.float64)) / 100
+    reg = (s64 < 2.0) & (rho64 > 1e-6)
+    e_rho = np.abs(rho32 - rho64)[reg] / rho64[reg]
+    e_s = np.abs(s32 - s64)[reg] / s64[reg]
+    print(f"  FP32 vs FP64 in s<2 ({reg.sum()} pts): rho max {e_rho.max():.2e} rms "
+          f"{np.sqrt((e_rho**2).mean()):.2e}; s max {e_s.max():.2e} rms {np.sqrt((e_s**2).mean()):.2e}")
+    assert e_rho.max() < 1e-5
+    assert np.sqrt((e_s ** 2).mean()) < 1e-5
+
+
+@pytest.mark.parametrize("kind,name", [(_lib.FT_GRAD, "grad"), (_lib.FT_XX, "xx"), (_lib.FT_YY, "yy"),
+                                       (_lib.FT_ZZ, "zz"), (_lib.FT_LAP, "lap")])
+def test_fft_deriv_matches_oracle(ctx, kind, name):
+    cell = random_cell(33, (7.0, 8.0, 9.0), (80, 100, 110), 4, (1, 6, 7, 8), (24, 30, 36))
+    f = gb.synth_slater(ctx, cell)
+    g = gb.fft_deriv(ctx, f, cell.n, cell.x2c, kind)
+    o = oracle.fft_deriv(f, cell.n, cell.x2c, name)
+    np.testing.assert_allclose(g, o, rtol=1e-9, atol=1e-9 * np.abs(o).max())
+
+
+def test_nci_fft_scheme_matches_oracle(ctx):
+    cell = config(1)
+    f = gb.synth_slater(ctx, cell)
+    crho, s = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, zfast=False)
+    cr_ref, s_ref = oracle.nci_grid_fft(f, cell.n, cell.x2c)
+    np.testing.assert_allclose(np.abs(crho), np.abs(cr_ref), rtol=1e-15)
+    # rocFFT and the oracle's FFT differ by round-off, about 1e-15 of the
+    # largest gradient; s amplifies it where rho is small.  NCI region
+    # (s < 2): 1e-6 relative; elsewhere the implied gradient error is bounded.
+    ok = f > 1e-8
+    nci = ok & (s_ref < 2.0)
+    np.testing.assert_allclose(s[nci], s_ref[nci], rtol=1e-6)
+    cst = 2 * (3 * math.pi ** 2) ** (1 / 3)
+    g, g_ref = s * cst * f ** (4 / 3), s_ref * cst * f ** (4 / 3)
+    assert np.abs(g - g_ref)[ok].max() <= 1e-12 * g_ref.max()
+    assert np.mean(np.sign(crho) == np.sign(cr_ref)) > 0.9999
+    # z-fastest layout of the reference's crho/cgrad arrays
+    cz, sz = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, zfast=True)
+    n = cell.n
+    assert np.array_equal(cz.reshape(n[0], n[1], n[2]), crho.reshape(n[2], n[1], n[0]).transpose(2, 1, 0))
