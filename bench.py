@@ -1,0 +1,210 @@
+"""Throughput of the on-grid Bader + NCI path on one MI355X.
+
+Workload (the 512^3 precision form): cubic cell a = 30 A, 512^3 FP64 grid of
+the Slater-sum density of 216 seeded atoms (Z in {1,6,7,8}), resident in HBM
+before the timed region.  One step is the whole path over that grid:
+  1. near-grid Bader labels, exact (FP64 ascent),
+  2. per-basin Volume and Pop (FP64 sums),
+  3. NCI reduced density gradient and signed density from the promolecular
+     radial tables in FP32 (the mixed-precision path).
+The plain FP64 GPU path (same step with the FP64 promolecular evaluator) is
+timed too and reported beside it.  value = grid points / step time.
+
+--gpus above 1 exits: the path runs on one GPU (multi-GPU sharding is a
+design, not part of this build).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from critic2_amd import _lib  # noqa: E402
+from critic2_amd import bader as gb  # noqa: E402
+from critic2_amd import nci as gn  # noqa: E402
+from critic2_amd.radial import slater_grid1  # noqa: E402
+from critic2_amd.synth import config  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name, N, ev_per_launch, traj_points, nfields=1):
+    """Algorithmic bytes per launch of the main kernels: each array element
+    the kernel must read or write, counted once (atomics count read+write)."""
+    E = ev_per_launch
+    table = {
+        "k_kentry": 17 * N + 8 * E,         # eoff, uni, Kt; each event once
+        "k_ev_count": 12 * N + 12 * E,      # ell, toff; path points, count RMW
+        "k_ev_fill": 24 * N + 20 * E,       # ell, lam, toff, eoff; point, cursor RMW, event
+        "k_ev_sort": 8 * N + 16 * E,        # eoff; events read and written
+        "k_ev_uniform": 9 * N + 8 * E,
+        "k_decide": 40 * N + 12 * E,        # story in/out, toff, Kt and points of the paths
+        "k_traj_len": 12 * N,               # grid once, lengths
+        "k_traj_fill": 16 * N + 4 * traj_points,
+        "k_promol_f": 8 * N,                # FP32 crho + cgrad
+        "k_promol_d": 16 * N,
+        "k_basin_partials": (4 + 8 * nfields) * N,
+        "k_reassign": 8 * N,
+    }
+    return table.get(name)
+
+
+def species_of(cell):
+    zs = sorted(set(int(z) for z in cell.zat))
+    return [slater_grid1(z) for z in zs], np.array([zs.index(int(z)) for z in cell.zat], dtype=np.int32)
+
+
+def run_step(ctx, cell, rho, lab, outs, tables, species, precision):
+    bas = gb.bader_integrate(ctx, rho, cell.n, cell.x2c, cell.xat, labels_out=lab)
+    counts, sums = gb.basin_sums(ctx, lab, bas.nattr, [rho])
+    gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision=precision,
+                      outs=outs, out_f32=(precision == "fp32"))
+    ctx.sync()
+    return bas
+
+
+def cpu_baseline(ctx, cell, tables, species):
+    """The oracle (scalar C restatement of critic2's algorithm, 1 thread) on a
+    bounded sample of the same workload: the Bader sweep+refine on the same
+    cell at 128^3, and the promolecular NCI at 4096 nodes of the 512^3 grid.
+    Returns Gpoints/s for Bader+NCI per point."""
+    from oracle import oracle
+
+    small = config(3, n_override=(128, 128, 128))
+    f = gb.synth_slater(ctx, small)
+    t0 = time.perf_counter()
+    oracle.bader(f, small.n, small.x2c, small.xat)
+    t_bader = (time.perf_counter() - t0) / f.size
+    # NCI: a 16x16x16 block of nodes at the cell centre
+    n = cell.n
+    idx = np.arange(248, 264)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    frac = np.stack([I / n[0], J / n[1], K / n[2]], -1).reshape(-1, 3)
+    pts = frac @ cell.x2c.T
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    rc = max(min(g.cutrad, g.rmax) for g in tables)
+    centre = pts.mean(0)
+    rad = np.sqrt(((pts - centre) ** 2).sum(1)).max() + rc
+    pos, sp = [], []
+    for a, x in enumerate(cell.xat):
+        for t in np.ndindex(3, 3, 3):
+            p = cell.x2c @ (x + np.array(t) - 1)
+            if np.linalg.norm(p - centre) <= rad:
+                pos.append(p)
+                sp.append(species[a])
+    t0 = time.perf_counter()
+    oracle.promol_nci(pts, np.array(pos), np.array(sp, dtype=np.int32), otabs)
+    t_nci = (time.perf_counter() - t0) / len(pts)
+    return {"value": 1e-9 / (t_bader + t_nci), "unit": "Gpoints/s", "cores": 1, "kind": "port",
+            "sample": "oracle Bader (serial sweep + refinement) on the same 30 A / 216-atom cell at "
+                      f"128^3 ({t_bader * 1e6:.2f} us/pt) + promolecular NCI at 4096 nodes of the "
+                      f"512^3 grid ({t_nci * 1e6:.2f} us/pt), 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--n", type=int, default=512, help="grid edge (default: the 512^3 workload)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    ctx = _lib.Context(0)
+    cell = config(3, n_override=(args.n,) * 3)
+    N = cell.npts
+    tables, species = species_of(cell)
+    rho = ctx.empty((N,), np.float64)
+    gb.synth_slater(ctx, cell, device_out=rho)
+    lab = ctx.empty((N,), np.int32)
+    outs32 = (ctx.empty((N,), np.float32), ctx.empty((N,), np.float32))
+    print(f"[bench] grid {cell.n} resident, {N} points", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        run_step(ctx, cell, rho, lab, outs32, tables, species, "fp32")
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        bas = run_step(ctx, cell, rho, lab, outs32, tables, species, "fp32")
+        print(f"[bench] step {s + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    ktimes = ctx.kernel_times()
+    ctx.timing(False)
+    st = bas.stats
+
+    # dominant kernel and its roofline
+    dom = max(ktimes.items(), key=lambda kv: kv[1][0])
+    name, (ms_tot, launches) = dom
+    ev_per_launch = st["events_total"] / max(st["sweep_iters"], 1)
+    nbytes = kernel_bytes(name, N, ev_per_launch, st["traj_points"])
+    avg_ms = ms_tot / launches
+    roof = None
+    if nbytes is not None:
+        ach = nbytes / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps,
+                "algorithmic_bytes_per_launch": int(nbytes)}
+
+    fp64 = None
+    if not args.no_fp64:
+        outs64 = (ctx.empty((N,), np.float64), ctx.empty((N,), np.float64))
+        ctx.sync()
+        t1 = time.perf_counter()
+        run_step(ctx, cell, rho, lab, outs64, tables, species, "fp64")
+        t64 = time.perf_counter() - t1
+        fp64 = {"value": N / t64 / 1e9, "ms_per_step": t64 * 1e3}
+        print("[bench] fp64 step done", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ctx, cell, tables, species)
+        print("[bench] cpu baseline done", file=sys.stderr, flush=True)
+
+    value = N / dt / 1e9
+    top = sorted(ktimes.items(), key=lambda kv: -kv[1][0])[:8]
+    out = {
+        "metric": "Gpoints/s Bader+NCI @512^3, mixed vs FP64 GPU (labels exact); %HBM peak",
+        "value": round(value, 6),
+        "unit": "Gpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "cubic cell a=30 A, 216 seeded atoms (Z in 1,6,7,8), Slater-sum FP64 grid "
+                               f"{cell.n[0]}x{cell.n[1]}x{cell.n[2]}; Bader labels + Volume/Pop + FP32 "
+                               "promolecular NCI",
+                   "grid": list(cell.n), "atoms": int(len(cell.zat)), "parallelism": "single GPU"},
+        "hbm_frac_compulsory": round(20.0 * N / dt / 1e9 / HBM_PEAK_GBS, 6),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64_gpu": fp64,
+        "speedup_vs_fp64_gpu": round(fp64["ms_per_step"] / (dt * 1e3), 3) if fp64 else None,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "bader": {k: st[k] for k in ("sweep_iters", "refine_iters", "traj_points", "events_total",
+                                     "ms_traj", "ms_sweep", "ms_refine", "ms_total")},
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in top},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

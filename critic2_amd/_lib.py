@@ -49,7 +49,8 @@ class AttractorOpts(C.Structure):
 class BaderStats(C.Structure):
     _fields_ = [("nattr", C.c_int), ("nmaxima", C.c_int), ("sweep_iters", C.c_int),
                 ("refine_iters", C.c_int), ("nstarts", C.c_int64), ("traj_points", C.c_int64),
-                ("edge_points", C.c_int64), ("ms_traj", C.c_double), ("ms_sweep", C.c_double),
+                ("edge_points", C.c_int64), ("events_total", C.c_int64),
+                ("ms_traj", C.c_double), ("ms_sweep", C.c_double),
                 ("ms_refine", C.c_double), ("ms_total", C.c_double)]
 
     def as_dict(self):

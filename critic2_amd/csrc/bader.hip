@@ -800,6 +800,7 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     C2_LAUNCH(ctx, k_ev_count, dim3(gN), dim3(B), 0, N, d_toff, d_tp, ell, cnt, d_cnt);
     exclusive_scan(ctx, "ev", cnt, eoff, (int64_t)N + 1);
     const uint64_t nev = read_scalar(ctx, eoff + N);
+    S.events_total += (int64_t)nev;
     ev = buf<uint64_t>(ctx, "bader_ev", nev);
     C2_HIP(hipMemsetAsync(cur, 0, sizeof(uint32_t) * N, st));
     C2_LAUNCH(ctx, k_ev_fill, dim3(gN), dim3(B), 0, P, N, d_toff, d_tp, ell, lam, eoff, cur, ev);

@@ -119,6 +119,7 @@ typedef struct {
   int64_t nstarts;      /* paths started by the first sweep               */
   int64_t traj_points;  /* stored free-trajectory points                  */
   int64_t edge_points;  /* points re-ascended in the first refine pass    */
+  int64_t events_total; /* sweep events processed, summed over iterations */
   double ms_traj, ms_sweep, ms_refine, ms_total; /* device time per phase */
 } c2gpu_bader_stats;
 
