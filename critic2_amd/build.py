@@ -18,6 +18,8 @@ ARCH = os.environ.get("C2GPU_ARCH", "gfx950")
 
 SOURCES = [
     ("bader.hip", ["-ffp-contract=off"]),
+    ("sweep.hip", ["-ffp-contract=off"]),
+    ("refine.hip", ["-ffp-contract=off"]),
     ("sums.hip", ["-ffp-contract=off"]),
     ("nci.hip", ["-ffp-contract=fast"]),
     ("capi.cpp", ["-ffp-contract=off"]),
@@ -71,5 +73,34 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_HOST = os.path.join(FORTRAN_DIR, "c2gpu_host")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the iso_c_binding module and the Fortran stand-in host with
+    amdflang (skipped, returning None, when amdflang is absent)."""
+    fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    steps = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "c2gpu_mod.f90"), "-o",
+         os.path.join(objdir, "c2gpu_mod.o"), "-module-dir", objdir],
+        [fc, "-O2", "-I", objdir, os.path.join(FORTRAN_DIR, "c2gpu_host.f90"),
+         os.path.join(objdir, "c2gpu_mod.o"), "-o", FORTRAN_HOST, "-L", HERE, "-lc2gpu",
+         "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stdout.decode(errors="replace"))
+    return FORTRAN_HOST
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

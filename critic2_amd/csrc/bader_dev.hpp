@@ -158,6 +158,21 @@ __device__ inline bool step_near_candidate(const BParams &P, const int p[3], dou
 // path[] (capacity maxlen) receives the visited points; known(pm)==1 is
 // membership in path[].  stop(x) is the known(x)==2 test (false for free
 // trajectories).  Returns the path length, or -1 if it exceeds maxlen.
+// 256-bit membership filter of the points on the current path: a clear bit
+// proves absence, so the exact scan of the path runs only on filter hits.
+struct PathFilter {
+  uint64_t w[4] = {0, 0, 0, 0};
+  __device__ __forceinline__ static uint32_t bit(uint32_t l) { return (l * 0x9E3779B1u) >> 24; }
+  __device__ __forceinline__ void add(uint32_t l) {
+    const uint32_t b = bit(l);
+    w[b >> 6] |= 1ull << (b & 63);
+  }
+  __device__ __forceinline__ bool maybe(uint32_t l) const {
+    const uint32_t b = bit(l);
+    return (w[b >> 6] >> (b & 63)) & 1ull;
+  }
+};
+
 template <class Stop, class Visit>
 __device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path, int maxlen,
                                 Stop stop, Visit visit) {
@@ -166,17 +181,20 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path
   double dr[3] = {0.0, 0.0, 0.0};
   int len = 1;
   path[0] = start;
+  PathFilter filt;
+  filt.add(start);
   visit(start);
   for (;;) {
     int pm[3];
     if (!step_near_candidate(P, p, dr, pm)) break;  // maximum: no move
     uint32_t lm = lin3(P, pm[0], pm[1], pm[2]);
     bool seen = false;
-    for (int q = len - 1; q >= 0; q--)
-      if (path[q] == lm) {
-        seen = true;
-        break;
-      }
+    if (filt.maybe(lm))
+      for (int q = len - 1; q >= 0; q--)
+        if (path[q] == lm) {
+          seen = true;
+          break;
+        }
     if (seen) {  // known(pm) == 1: on-grid step, dr = 0
       pm[0] = p[0];
       pm[1] = p[1];
@@ -188,6 +206,7 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path
     if (pm[0] == p[0] && pm[1] == p[1] && pm[2] == p[2]) break;  // no move
     if (len >= maxlen) return -1;
     path[len++] = lm;
+    filt.add(lm);
     visit(lm);
     p[0] = pm[0];
     p[1] = pm[1];

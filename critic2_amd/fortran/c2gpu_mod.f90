@@ -1,0 +1,112 @@
+! iso_c_binding interfaces of libc2gpu (include/c2gpu.h) for critic2's
+! Fortran host.  Conventions follow critic2: grids are f(n1,n2,n3) with x
+! fastest, 3x3 matrices are Fortran column-major, lat_i_dist is
+! lat_i_dist(-1:1,-1:1,-1:1).  Every function returns 0 on success; the
+! caller turns a non-zero status into ferror(...,faterr).
+module c2gpu
+  use iso_c_binding, only: c_int, c_int32_t, c_int64_t, c_double, c_ptr, c_char
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: C2GPU_OK = 0
+  integer(c_int), parameter, public :: C2GPU_PTR_DEVICE = 1
+  integer(c_int), parameter, public :: C2GPU_ATTR_NEAREST_ATOM = 0
+  integer(c_int), parameter, public :: C2GPU_ATTR_RATOM = 1
+  integer(c_int), parameter, public :: C2GPU_ATTR_NOATOMS = 2
+  integer(c_int), parameter, public :: C2GPU_FT_GRAD = 0, C2GPU_FT_XX = 1, C2GPU_FT_YY = 2, &
+     C2GPU_FT_ZZ = 3, C2GPU_FT_LAP = 4
+
+  type, bind(c), public :: c2gpu_attractor_opts
+     integer(c_int) :: mode
+     real(c_double) :: ratom
+     real(c_double) :: x2c(9)
+     integer(c_int) :: nat
+     type(c_ptr) :: xat
+  end type c2gpu_attractor_opts
+
+  type, bind(c), public :: c2gpu_bader_stats
+     integer(c_int) :: nattr, nmaxima, sweep_iters, refine_iters
+     integer(c_int64_t) :: nstarts, traj_points, edge_points, events_total
+     real(c_double) :: ms_traj, ms_sweep, ms_refine, ms_total
+  end type c2gpu_bader_stats
+
+  public :: c2gpu_ctx_create, c2gpu_ctx_destroy, c2gpu_bader_metrics, c2gpu_bader
+  public :: c2gpu_basin_sums, c2gpu_fft_deriv, c2gpu_relabel
+
+  interface
+     function c2gpu_ctx_create(ctx, device) bind(c, name="c2gpu_ctx_create")
+       import :: c_ptr, c_int
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int), value :: device
+       integer(c_int) :: c2gpu_ctx_create
+     end function c2gpu_ctx_create
+
+     function c2gpu_ctx_destroy(ctx) bind(c, name="c2gpu_ctx_destroy")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+       integer(c_int) :: c2gpu_ctx_destroy
+     end function c2gpu_ctx_destroy
+
+     function c2gpu_bader_metrics(x2c, n, lat2car, car2lat, lat_i_dist) &
+        bind(c, name="c2gpu_bader_metrics")
+       import :: c_int, c_double
+       real(c_double), intent(in) :: x2c(9)
+       integer(c_int), intent(in) :: n(3)
+       real(c_double), intent(out) :: lat2car(9), car2lat(9), lat_i_dist(27)
+       integer(c_int) :: c2gpu_bader_metrics
+     end function c2gpu_bader_metrics
+
+     function c2gpu_bader(ctx, rho, n, car2lat, lat_i_dist, opts, flags, label_out, nattr, &
+        xattr_out, xattr_cap, stats) bind(c, name="c2gpu_bader")
+       import :: c_ptr, c_int, c_int32_t, c_double, c2gpu_attractor_opts, c2gpu_bader_stats
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: rho(*)
+       integer(c_int), intent(in) :: n(3)
+       real(c_double), intent(in) :: car2lat(9), lat_i_dist(27)
+       type(c2gpu_attractor_opts), intent(in) :: opts
+       integer(c_int), value :: flags
+       integer(c_int32_t), intent(out) :: label_out(*)
+       integer(c_int), intent(out) :: nattr
+       real(c_double), intent(out) :: xattr_out(*)
+       integer(c_int), value :: xattr_cap
+       type(c2gpu_bader_stats), intent(out) :: stats
+       integer(c_int) :: c2gpu_bader
+     end function c2gpu_bader
+
+     function c2gpu_relabel(ctx, npts, map, nmap, flags, label) bind(c, name="c2gpu_relabel")
+       import :: c_ptr, c_int, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: npts
+       integer(c_int32_t), intent(in) :: map(*)
+       integer(c_int), value :: nmap, flags
+       integer(c_int32_t), intent(inout) :: label(*)
+       integer(c_int) :: c2gpu_relabel
+     end function c2gpu_relabel
+
+     function c2gpu_basin_sums(ctx, label, npts, nattr, nfields, fields, flags, sums_out, &
+        counts_out) bind(c, name="c2gpu_basin_sums")
+       import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: label(*)
+       integer(c_int64_t), value :: npts
+       integer(c_int), value :: nattr, nfields
+       type(c_ptr), intent(in) :: fields(*)
+       integer(c_int), value :: flags
+       real(c_double), intent(out) :: sums_out(*)
+       integer(c_int64_t), intent(out) :: counts_out(*)
+       integer(c_int) :: c2gpu_basin_sums
+     end function c2gpu_basin_sums
+
+     function c2gpu_fft_deriv(ctx, rho, n, x2c, kind, flags, out) bind(c, name="c2gpu_fft_deriv")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: rho(*)
+       integer(c_int), intent(in) :: n(3)
+       real(c_double), intent(in) :: x2c(9)
+       integer(c_int), value :: kind, flags
+       real(c_double), intent(out) :: out(*)
+       integer(c_int) :: c2gpu_fft_deriv
+     end function c2gpu_fft_deriv
+  end interface
+
+end module c2gpu
