@@ -23,7 +23,7 @@ ERR_NAMES = {
 PTR_DEVICE = 1
 ATTR_NEAREST_ATOM, ATTR_RATOM, ATTR_NOATOMS = 0, 1, 2
 FT_GRAD, FT_XX, FT_YY, FT_ZZ, FT_LAP = 0, 1, 2, 3, 4
-NCI_FFT, NCI_PROMOL = 0, 2
+NCI_FFT, NCI_GRID, NCI_PROMOL = 0, 1, 2
 FP64, FP32 = 0, 1
 
 EXPORTS = (

@@ -180,7 +180,8 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   unsigned long long *d_top = buf<unsigned long long>(ctx, "bader_pooltop", 1);
   C2_HIP(hipMemsetAsync(d_len + N, 0, sizeof(uint32_t), st));
   auto &poolbuf = ctx->bufs["bader_pool"];
-  uint64_t cap = poolbuf.ptr ? poolbuf.bytes / 4 : (uint64_t)N * 48;
+  uint64_t cap = (uint64_t)N * 48;
+  if (poolbuf.ptr && poolbuf.bytes / 4 > cap) cap = poolbuf.bytes / 4;
   uint64_t ntraj = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     uint32_t *pool = buf<uint32_t>(ctx, "bader_pool", cap);

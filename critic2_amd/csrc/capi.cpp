@@ -19,8 +19,8 @@ void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const doubl
                     double *, int64_t *);
 void relabel_run(c2gpu_ctx *, int64_t, const int32_t *, int, int, int32_t *);
 void fft_deriv_run(c2gpu_ctx *, const double *, const int *, const double *, int, int, double *);
-void nci_run(c2gpu_ctx *, const double *, const int *, const double *, int, int, const c2gpu_promol *,
-             int, int, void *, void *, int);
+void nci_run(c2gpu_ctx *, const double *, const int *, const double *, const double *, int, int,
+             const c2gpu_promol *, int, int, void *, void *, int);
 void synth_slater_run(c2gpu_ctx *, const double *, const int *, int, const double *, const int *,
                       double, int, double *);
 
@@ -323,18 +323,30 @@ int c2gpu_fft_deriv(c2gpu_ctx *ctx, const double *rho, const int n[3], const dou
   });
 }
 
+static void nci_checked(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], int scheme,
+                        int precision, const c2gpu_promol *promol, int zfast, int flags, void *crho_out,
+                        void *cgrad_out, int out_f32) {
+  if (!n || !x2c || !crho_out || !cgrad_out) fail(C2GPU_ERR_ARG, "null argument");
+  const bool grid_in = (scheme == C2GPU_NCI_FFT || scheme == C2GPU_NCI_GRID);
+  if (grid_in && !rho) fail(C2GPU_ERR_ARG, "null grid");
+  if (scheme == C2GPU_NCI_PROMOL && !promol) fail(C2GPU_ERR_ARG, "null promolecular tables");
+  if (flags & C2GPU_PTR_DEVICE) {
+    if (grid_in) require_device_ptr(rho, "rho");
+    require_device_ptr(crho_out, "crho_out");
+    require_device_ptr(cgrad_out, "cgrad_out");
+  }
+  // crystal%m_c2x and the grid's c2xl: critic2's matinv of the cell matrix
+  double c2x[9];
+  std::memcpy(c2x, x2c, sizeof(c2x));
+  if (!inv3_lapack(c2x)) fail(C2GPU_ERR_ARG, "c2gpu_nci: singular cell matrix");
+  nci_run(ctx, rho, n, x2c, c2x, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, out_f32);
+}
+
 int c2gpu_nci(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], int scheme,
               int precision, const c2gpu_promol *promol, int zfast, int flags, double *crho_out,
               double *cgrad_out) {
   return guarded(ctx, [&] {
-    if (!n || !x2c || !crho_out || !cgrad_out) fail(C2GPU_ERR_ARG, "null argument");
-    if (scheme == C2GPU_NCI_FFT && !rho) fail(C2GPU_ERR_ARG, "null grid");
-    if (flags & C2GPU_PTR_DEVICE) {
-      if (scheme == C2GPU_NCI_FFT) require_device_ptr(rho, "rho");
-      require_device_ptr(crho_out, "crho_out");
-      require_device_ptr(cgrad_out, "cgrad_out");
-    }
-    nci_run(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 0);
+    nci_checked(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 0);
   });
 }
 
@@ -342,13 +354,7 @@ int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const doubl
                   int precision, const c2gpu_promol *promol, int zfast, int flags, float *crho_out,
                   float *cgrad_out) {
   return guarded(ctx, [&] {
-    if (!n || !x2c || !crho_out || !cgrad_out) fail(C2GPU_ERR_ARG, "null argument");
-    if (flags & C2GPU_PTR_DEVICE) {
-      if (scheme == C2GPU_NCI_FFT) require_device_ptr(rho, "rho");
-      require_device_ptr(crho_out, "crho_out");
-      require_device_ptr(cgrad_out, "cgrad_out");
-    }
-    nci_run(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 1);
+    nci_checked(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 1);
   });
 }
 

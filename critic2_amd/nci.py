@@ -1,7 +1,8 @@
 """Host-side mirror of critic2's NCIPLOT per-point block on periodic grids.
 
 ``nciplot_grid`` is the loaded-grid branch with Fourier derivatives
-(nci@proc.f90:525-564, the default ``fourierint`` for grid fields);
+(nci@proc.f90:525-564, the default ``fourierint`` for grid fields) or, with
+interpolation="grid", the tricubic grid interpolant (:565-571);
 ``nciplot_promol`` is the promolecular branch (:565-571 with
 crystalmod@env.f90:622-753).  Both return crho = sign(rho, lambda2)*100 and
 cgrad = s, z-fastest as nci@proc.f90:502-505 stores them, unless
@@ -60,9 +61,15 @@ def _run(ctx, rho, n, x2c, scheme, precision, pm, zfast, out_f32, outs=None):
     return crho, cgrad
 
 
-def nciplot_grid(ctx: Context, rho, n, x2c, zfast=True, outs=None):
-    """FFT-derivative NCI on a loaded FP64 grid."""
-    return _run(ctx, rho, n, x2c, _lib.NCI_FFT, _lib.FP64, None, zfast, False, outs)
+def nciplot_grid(ctx: Context, rho, n, x2c, zfast=True, outs=None, interpolation="fourier"):
+    """NCI on a loaded FP64 grid.  interpolation="fourier" is the default
+    fourierint branch (FFT derivatives, nci@proc.f90:546-564);
+    interpolation="grid" is NCIPLOT's INTERPOLATION GRID keyword
+    (nci@proc.f90:358-360): tricubic interpolant of the grid, :565-571."""
+    if interpolation not in ("fourier", "grid"):
+        raise ValueError("interpolation must be 'fourier' or 'grid'")
+    scheme = _lib.NCI_FFT if interpolation == "fourier" else _lib.NCI_GRID
+    return _run(ctx, rho, n, x2c, scheme, _lib.FP64, None, zfast, False, outs)
 
 
 def nciplot_promol(ctx: Context, n, x2c, tables: list[Grid1], xat, species, precision="fp32",

@@ -187,7 +187,7 @@ typedef struct {
   const int *species;               /* table index of each atom            */
 } c2gpu_promol;
 
-enum { C2GPU_NCI_FFT = 0, C2GPU_NCI_PROMOL = 2 };
+enum { C2GPU_NCI_FFT = 0, C2GPU_NCI_GRID = 1, C2GPU_NCI_PROMOL = 2 };
 enum { C2GPU_FP64 = 0, C2GPU_FP32 = 1 };
 
 /* The NCIPLOT per-point block (nci@proc.f90:525-605) on the nodes of a
@@ -196,6 +196,11 @@ enum { C2GPU_FP64 = 0, C2GPU_FP32 = 1 };
  *   crho  = sign(rho, lambda2) * 100
  * scheme C2GPU_NCI_FFT: rho is the loaded FP64 grid; derivatives by FFT and
  *   the lambda2 sign from count(diag Hessian > 0) >= 2 (nci@proc.f90:546-564).
+ * scheme C2GPU_NCI_GRID: rho is the loaded FP64 grid (INTERPOLATION GRID, or
+ *   a field with a core contribution); density, gradient and Hessian from the
+ *   grid's tricubic interpolant (grid3mod@proc.f90:2649-2799) at the node as
+ *   field%grd locates it, lambda2 from the Hessian eigenvalues
+ *   (nci@proc.f90:565-571). FP64 only; n >= 2 per axis.
  * scheme C2GPU_NCI_PROMOL: rho is ignored; density, gradient and Hessian from
  *   the promolecular radial tables (crystalmod@env.f90:622-753), lambda2 from
  *   the Hessian eigenvalues (nci@proc.f90:565-571); precision selects FP64 or

@@ -459,6 +459,8 @@ static void matinv3(double *a) {
 }
 #undef A_
 
+void oracle_matinv3(double a[9]) { matinv3(a); }
+
 void oracle_bader_metrics(const double x2c[9], const int n[3], double lat2car[9],
                           double car2lat[9], double lat_i_dist[27]) {
   for (int c = 0; c < 3; c++)

@@ -111,6 +111,23 @@ void oracle_promol(const double xc[3], int nimg, const double *ximg,
 void oracle_nci_point(double rho, double gmod, const double ehess_or_h[9],
                       int from_full_hessian, double *crho, double *s);
 
+/* In-place inverse of a column-major 3x3 matrix in the operation order of
+ * critic2's matinv (LAPACK dgetrf2 + dgetri). */
+void oracle_matinv3(double a[9]);
+
+/* Tricubic interpolation matrix c(64,64) of grid3mod@proc.f90:76,
+ * column-major. */
+void oracle_tricubic_matrix(double c[64 * 64]);
+
+/* grinterp_tricubic (grid3mod@proc.f90:2649-2799) at fractional xi. */
+void oracle_tricubic(const double *f, const int n[3], const double xi[3], double *y,
+                     double yp[3], double ypp[9]);
+
+/* NCIPLOT grid-interpolation scheme on the nodes of a periodic grid
+ * (nci@proc.f90:565-571); outputs z-fastest. */
+void oracle_nci_grid(const double *f, const int n[3], const double x2c[9], const double c2x[9],
+                     double *crho, double *cgrad);
+
 /* Synthetic Slater-sum density (FP64) on an n1*n2*n3 grid, x fastest.
  * rho = sum_{atoms,images} Z*(zeta^3/(8 pi))*exp(-zeta*|r-R|),
  * zeta = 2*Z^(1/3) bohr^-1, images within rimg bohr. */

@@ -263,3 +263,55 @@ def step_pointers(f, n, x2c):
     lib.oracle_step_pointers(f.ctypes.data, nn.ctypes.data, c2l.ctypes.data, lid.ctypes.data,
                              on.ctypes.data, ne.ctypes.data, dr.ctypes.data)
     return on, ne, dr
+
+
+# ---- NCIPLOT grid-interpolation scheme (nci@proc.f90:565-571)
+
+def matinv3(m):
+    """critic2's matinv of a 3x3 matrix (LAPACK operation order)."""
+    lib = load()
+    lib.oracle_matinv3.argtypes = [C.c_void_p]
+    a = _matf(m)
+    lib.oracle_matinv3(a.ctypes.data)
+    return a.reshape(3, 3, order="F")
+
+
+def tricubic_matrix():
+    """The 64x64 tricubic coefficient matrix c(l, m) (a = c b)."""
+    lib = load()
+    lib.oracle_tricubic_matrix.argtypes = [C.c_void_p]
+    c = np.zeros(64 * 64)
+    lib.oracle_tricubic_matrix(c.ctypes.data)
+    return c.reshape(64, 64, order="F")
+
+
+def tricubic(f, n, xi):
+    """grinterp_tricubic at fractional xi: (y, yp[3], ypp[3,3]) with the
+    fractional-coordinate derivatives scaled by n."""
+    lib = load()
+    lib.oracle_tricubic.argtypes = [C.c_void_p] * 6
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    nn = np.asarray(n, dtype=np.int32)
+    x = np.asarray(xi, dtype=np.float64).copy()
+    y = np.zeros(1)
+    yp = np.zeros(3)
+    ypp = np.zeros(9)
+    lib.oracle_tricubic(f.ctypes.data, nn.ctypes.data, x.ctypes.data, y.ctypes.data, yp.ctypes.data,
+                        ypp.ctypes.data)
+    return y[0], yp, ypp.reshape(3, 3, order="F")
+
+
+def nci_grid_interp(f, n, x2c):
+    """NCI node quantities of the grid-interpolation branch, z-fastest flat
+    arrays (crho, s) as nciplot stores them."""
+    lib = load()
+    lib.oracle_nci_grid.argtypes = [C.c_void_p] * 6
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    nn = np.asarray(n, dtype=np.int32)
+    x = _matf(x2c)
+    c2x = _matf(matinv3(x2c))
+    crho = np.zeros(f.size)
+    s = np.zeros(f.size)
+    lib.oracle_nci_grid(f.ctypes.data, nn.ctypes.data, x.ctypes.data, c2x.ctypes.data, crho.ctypes.data,
+                        s.ctypes.data)
+    return crho, s

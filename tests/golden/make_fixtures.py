@@ -77,6 +77,27 @@ def main():
         arrs[f"r_{sym}"] = r
         arrs[f"rr0_{sym}"] = rr0
     np.savez_compressed(os.path.join(HERE, "urea_tables.npz"), **arrs)
+    tricubic_vectors()
+
+
+def tricubic_vectors():
+    """tricubic_ab.json: eight seeded random 64-vectors b (the f, fx, fy, fz,
+    fxy, fxz, fyz, fxyz corner values of grinterp_tricubic) and a = c b, with
+    c the reference's tricubic matrix (src/grid3mod@proc.f90:76, read as
+    data, column-major)."""
+    import re
+
+    src = open(os.path.join(REF, "src/grid3mod@proc.f90")).read()
+    i = src.index("c(64,64) = reshape((/")
+    nums = re.findall(r"-?\d+d0", src[i:src.index("/)", i + 30)])
+    assert len(nums) == 4096
+    c = np.array([float(v[:-2]) for v in nums]).reshape(64, 64, order="F")
+    rng = np.random.default_rng(2649)
+    b = rng.standard_normal((8, 64))
+    a = b @ c.T
+    with open(os.path.join(HERE, "tricubic_ab.json"), "w") as fh:
+        json.dump({"source": "src/grid3mod@proc.f90:76 (c) applied as a = matmul(c,b), :2770",
+                   "b": b.tolist(), "a": a.tolist()}, fh)
 
 
 if __name__ == "__main__":

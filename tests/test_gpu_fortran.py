@@ -25,7 +25,7 @@ def test_fortran_host_bader(ctx, tmp_path):
     case = tmp_path / "case.txt"
     with open(case, "w") as fh:
         fh.write(" ".join(str(v) for v in cell.n) + "\n")
-        fh.write(" ".join(repr(v) for v in np.asfortranarray(cell.x2c).reshape(-1, order="F")) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in np.asarray(cell.x2c).reshape(-1, order="F")) + "\n")
         fh.write(f"{len(cell.zat)}\n")
         for x in cell.xat:
             fh.write(" ".join(repr(float(v)) for v in x) + "\n")

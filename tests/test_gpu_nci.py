@@ -102,3 +102,39 @@ def test_nci_fft_scheme_matches_oracle(ctx):
     cz, sz = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, zfast=True)
     n = cell.n
     assert np.array_equal(cz.reshape(n[0], n[1], n[2]), crho.reshape(n[2], n[1], n[0]).transpose(2, 1, 0))
+
+
+@pytest.mark.parametrize("which", ["cubic64", "triclinic"])
+def test_nci_grid_interpolation_scheme_matches_oracle(ctx, which):
+    """INTERPOLATION GRID (nci@proc.f90:565-571): tricubic value, gradient
+    and Hessian at the node as field%grd locates it.  The node search is
+    reproduced bit for bit (so the same cell's polynomial is evaluated; on
+    the cell's origin planes, rounding below zero folds to 1.0 and both
+    evaluate that cell's cubic at u = n, as the reference does); the
+    polynomial itself is evaluated in a different but equivalent form, so
+    values agree to round-off: 1e-8 relative (the worst case is the u = n
+    extrapolation), and the lambda2 sign everywhere."""
+    if which == "cubic64":
+        cell = config(1)
+    else:
+        cell = random_cell(61, (7.0, 7.5, 8.0), (80, 100, 110), 4, (1, 6, 7, 8), (24, 26, 28))
+    print(f"[case] nci grid scheme {which} n={cell.n}", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    crho, s = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, interpolation="grid")
+    cr_ref, s_ref = oracle.nci_grid_interp(f, cell.n, cell.x2c)
+    np.testing.assert_allclose(np.abs(crho), np.abs(cr_ref), rtol=1e-8)
+    np.testing.assert_allclose(s, s_ref, rtol=1e-8)
+    assert np.array_equal(np.sign(crho), np.sign(cr_ref))
+    # x-fastest layout is the same data
+    cx, sx = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, interpolation="grid", zfast=False)
+    n = cell.n
+    assert np.array_equal(cx.reshape(n[2], n[1], n[0]).transpose(2, 1, 0), crho.reshape(n[0], n[1], n[2]))
+
+
+def test_nci_grid_scheme_rejects_fp32_and_tiny_grids(ctx):
+    cell = random_cell(62, (5.0, 5.0, 5.0), (90, 90, 90), 2, (1, 8), (8, 8, 8))
+    f = gb.synth_slater(ctx, cell)
+    with pytest.raises(_lib.C2GpuError):
+        gn._run(ctx, f, cell.n, cell.x2c, _lib.NCI_GRID, _lib.FP32, None, True, False)
+    with pytest.raises(_lib.C2GpuError):
+        gn.nciplot_grid(ctx, np.ones(8), (8, 1, 1), cell.x2c, interpolation="grid")

@@ -173,3 +173,89 @@ def test_nci_point_formula_constants():
     assert abs(s - 1.0) < 1e-15 and cr == 100.0
     cr, s = oracle.nci_point(0.5, 0.0, [1.0, -1.0, -1.0])
     assert cr == -50.0 and s == 0.0
+
+
+def test_tricubic_matrix_matches_reference_vectors():
+    """The tricubic coefficient matrix (Hermite tensor product) applied to
+    the golden b vectors gives the a = c b the reference's own table gives
+    (src/grid3mod@proc.f90:76)."""
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "tricubic_ab.json")))
+    c = oracle.tricubic_matrix()
+    b = np.array(d["b"])
+    a = np.array(d["a"])
+    np.testing.assert_allclose(b @ c.T, a, rtol=0, atol=1e-12)
+
+
+def _quadratic_grid(n, coef):
+    """f(i,j,k) = quadratic polynomial of the node indices, x-fastest."""
+    I, J, K = np.meshgrid(*[np.arange(q, dtype=float) for q in n], indexing="ij")
+    c0, cx, cy, cz, cxx, cyy, czz, cxy, cxz, cyz = coef
+    f = (c0 + cx * I + cy * J + cz * K + cxx * I * I + cyy * J * J + czz * K * K + cxy * I * J + cxz * I * K
+         + cyz * J * K)
+    return f.transpose(2, 1, 0).reshape(-1)
+
+
+def test_tricubic_reproduces_quadratics():
+    """Cubic Hermite with central-difference derivatives is exact for
+    quadratics: value, gradient and Hessian (fractional derivatives, scaled
+    by n) of a quadratic in the node indices, away from the periodic seam."""
+    rng = np.random.default_rng(5)
+    n = (10, 11, 12)
+    coef = rng.standard_normal(10)
+    f = _quadratic_grid(n, coef)
+    c0, cx, cy, cz, cxx, cyy, czz, cxy, cxz, cyz = coef
+    for _ in range(20):
+        u = np.array([rng.uniform(1.0, q - 3.0) for q in n])
+        y, yp, ypp = oracle.tricubic(f, n, u / np.array(n))
+        val = (c0 + cx * u[0] + cy * u[1] + cz * u[2] + cxx * u[0] ** 2 + cyy * u[1] ** 2 + czz * u[2] ** 2
+               + cxy * u[0] * u[1] + cxz * u[0] * u[2] + cyz * u[1] * u[2])
+        g = np.array([cx + 2 * cxx * u[0] + cxy * u[1] + cxz * u[2],
+                      cy + 2 * cyy * u[1] + cxy * u[0] + cyz * u[2],
+                      cz + 2 * czz * u[2] + cxz * u[0] + cyz * u[1]])
+        h = np.array([[2 * cxx, cxy, cxz], [cxy, 2 * cyy, cyz], [cxz, cyz, 2 * czz]])
+        nn = np.array(n, dtype=float)
+        assert abs(y - val) < 1e-10
+        np.testing.assert_allclose(yp, g * nn, atol=1e-9)
+        np.testing.assert_allclose(ypp, h * np.outer(nn, nn), atol=1e-7)
+
+
+def test_nci_grid_scheme_at_nodes():
+    """Grid-interpolation NCI on a triclinic cell: at a node the tricubic
+    value is the node value, its gradient the central difference, and its
+    second derivative along an axis the one-sided Hermite stencil
+    2 f(-1) - 5 f(0) + 4 f(1) - f(2); the Cartesian gradient and Hessian are
+    c2x^T g and c2x^T H c2x."""
+    from critic2_amd.synth import lattice_from_params
+    x2c = lattice_from_params(7.0, 7.5, 8.0, 95, 100, 80)
+    n = (9, 10, 11)
+    rng = np.random.default_rng(7)
+    f3 = 1.0 + rng.random(n)  # (n1,n2,n3)
+    f = f3.transpose(2, 1, 0).reshape(-1)
+    crho, s = oracle.nci_grid_interp(f, n, x2c)
+    c2x = oracle.matinv3(x2c)
+    nn = np.array(n, dtype=float)
+    cst = 2 * (3 * math.pi ** 2) ** (1 / 3)
+    checked = 0
+    for (i, j, k) in [(3, 4, 5), (1, 1, 1), (7, 2, 9), (4, 8, 3)]:
+        p = np.array([i, j, k])
+        fv = lambda d: f3[tuple((p + d) % nn.astype(int))]
+        e = np.eye(3, dtype=int)
+        g = np.array([0.5 * (fv(e[c]) - fv(-e[c])) for c in range(3)]) * nn
+        H = np.zeros((3, 3))
+        for a in range(3):
+            H[a, a] = (2 * fv(-e[a]) - 5 * fv(0 * e[a]) + 4 * fv(e[a]) - fv(2 * e[a])) * nn[a] ** 2
+            for b in range(a + 1, 3):
+                # mixed: product of the two 1-D first-derivative stencils at 0
+                v = 0.25 * (fv(e[a] + e[b]) - fv(-e[a] + e[b]) - fv(e[a] - e[b]) + fv(-e[a] - e[b]))
+                H[a, b] = H[b, a] = v * nn[a] * nn[b]
+        gc = c2x.T @ g
+        hc = c2x.T @ H @ c2x
+        l2 = np.linalg.eigvalsh(hc)[1]
+        o = k + n[2] * (j + n[1] * i)
+        assert abs(s[o] - np.linalg.norm(gc) / (cst * fv(0 * e[0]) ** (4 / 3))) < 1e-10 * s[o] + 1e-14
+        assert abs(abs(crho[o]) - 100 * fv(0 * e[0])) < 1e-12
+        assert np.sign(crho[o]) == np.sign(l2)
+        checked += 1
+    assert checked == 4
