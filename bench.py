@@ -56,6 +56,20 @@ def kernel_bytes(name, N, ev_per_launch, traj_points, nfields=1):
     return table.get(name)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summaries (FETCH_SIZE and WRITE_SIZE runs of this workload, scaled by a
+    streaming kernel of known traffic): the `traffic_bytes_per_launch` lines
+    of profiles/*_traffic.md.  None when the kernel was not measured."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
+        for line in open(path):
+            t = line.split()
+            if len(t) == 3 and t[0] == "traffic_bytes_per_launch" and t[1] == kernel:
+                return int(t[2])
+    return None
+
+
 def species_of(cell):
     zs = sorted(set(int(z) for z in cell.zat))
     return [slater_grid1(z) for z in zs], np.array([zs.index(int(z)) for z in cell.zat], dtype=np.int32)
@@ -116,11 +130,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--n", type=int, default=512, help="grid edge (default: the 512^3 workload)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a streaming kernel of known traffic (k_sqrt of the FFT |grad| grid: "
+                         "8 B read + 8 B written per point, 256^3) to scale rocprof FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     ctx = _lib.Context(0)
+    if args.copy_probe:
+        probe = config(2)
+        f = gb.synth_slater(ctx, probe)
+        gb.fft_deriv(ctx, f, probe.n, probe.x2c, _lib.FT_GRAD)
+        print(json.dumps({"copy_probe": "k_sqrt", "points": probe.npts,
+                          "bytes_read": 8 * probe.npts, "bytes_written": 8 * probe.npts}), flush=True)
+        return
     cell = config(3, n_override=(args.n,) * 3)
     N = cell.npts
     tables, species = species_of(cell)
@@ -141,21 +165,28 @@ def main():
         print(f"[bench] step {s + 1}/{args.steps} done", file=sys.stderr, flush=True)
     ctx.sync()
     dt = (time.perf_counter() - t0) / args.steps
-    ktimes = ctx.kernel_times()
+    kstats = ctx.kernel_stats()
+    ktimes = {k: v[:2] for k, v in kstats.items()}
     ctx.timing(False)
     st = bas.stats
 
-    # dominant kernel and its roofline
+    # dominant kernel and its roofline: algorithmic bytes reported by the
+    # library for the launches it timed (list-driven kernels vary per launch),
+    # else the per-launch formula of the full-grid kernels
     dom = max(ktimes.items(), key=lambda kv: kv[1][0])
     name, (ms_tot, launches) = dom
-    ev_per_launch = st["events_total"] / max(st["sweep_iters"], 1)
-    nbytes = kernel_bytes(name, N, ev_per_launch, st["traj_points"])
     avg_ms = ms_tot / launches
+    reported = kstats[name][2]
+    if reported > 0:
+        nbytes = reported / launches
+    else:
+        ev_per_launch = st["events_total"] / max(st["sweep_iters"], 1)
+        nbytes = kernel_bytes(name, N, ev_per_launch, st["traj_points"])
     roof = None
     if nbytes is not None:
         ach = nbytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                 "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps,
                 "algorithmic_bytes_per_launch": int(nbytes)}
 

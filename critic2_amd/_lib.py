@@ -29,7 +29,7 @@ FP64, FP32 = 0, 1
 EXPORTS = (
     "c2gpu_ctx_create", "c2gpu_ctx_destroy", "c2gpu_ctx_error", "c2gpu_ctx_set_stream",
     "c2gpu_ctx_sync", "c2gpu_dev_alloc", "c2gpu_dev_free", "c2gpu_memcpy", "c2gpu_ctx_timing",
-    "c2gpu_ctx_kernel_time", "c2gpu_ctx_kernel_names",
+    "c2gpu_ctx_kernel_time", "c2gpu_ctx_kernel_names", "c2gpu_ctx_kernel_bytes",
     "c2gpu_bader_metrics", "c2gpu_bader_step_pointers", "c2gpu_bader", "c2gpu_relabel",
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_synth_slater",
 )
@@ -91,6 +91,7 @@ def load():
         "c2gpu_ctx_timing": [vp, i],
         "c2gpu_ctx_kernel_time": [vp, C.c_char_p, C.POINTER(d), C.POINTER(C.c_int64)],
         "c2gpu_ctx_kernel_names": [vp, C.c_char_p, C.c_uint64],
+        "c2gpu_ctx_kernel_bytes": [vp, C.c_char_p, C.POINTER(d)],
         "c2gpu_bader_metrics": [vp, vp, vp, vp, vp],
         "c2gpu_bader_step_pointers": [vp, vp, vp, vp, vp, i, vp, vp, vp],
         "c2gpu_bader": [vp, vp, vp, vp, vp, C.POINTER(AttractorOpts), i, vp, C.POINTER(i), vp, i,
@@ -168,14 +169,22 @@ class Context:
 
     def kernel_times(self):
         """{kernel name: (ms, launches)} accumulated since timing(True)."""
+        return {k: v[:2] for k, v in self.kernel_stats().items()}
+
+    def kernel_stats(self):
+        """{kernel name: (ms, launches, algorithmic bytes or 0)} accumulated
+        since timing(True); read before timing() is called again, which
+        resets the table."""
         buf = C.create_string_buffer(1 << 16)
         self.check(self.lib.c2gpu_ctx_kernel_names(self.h, buf, len(buf)), "c2gpu_ctx_kernel_names")
         out = {}
         for name in filter(None, buf.value.decode().split(",")):
-            ms, n = C.c_double(), C.c_int64()
+            ms, n, b = C.c_double(), C.c_int64(), C.c_double()
             self.check(self.lib.c2gpu_ctx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)),
                        "c2gpu_ctx_kernel_time")
-            out[name] = (ms.value, n.value)
+            self.check(self.lib.c2gpu_ctx_kernel_bytes(self.h, name.encode(), C.byref(b)),
+                       "c2gpu_ctx_kernel_bytes")
+            out[name] = (ms.value, n.value, b.value)
         return out
 
     def empty(self, shape, dtype):

@@ -22,6 +22,7 @@ SOURCES = [
     ("refine.hip", ["-ffp-contract=off"]),
     ("sums.hip", ["-ffp-contract=off"]),
     ("nci.hip", ["-ffp-contract=fast"]),
+    ("ncigrid.hip", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
 ]
 

@@ -128,6 +128,9 @@ static BParams make_params(const double *d_rho, const int n[3], const double car
   P.n1 = n[0];
   P.n2 = n[1];
   P.n3 = n[2];
+  P.inv1 = 1.0 / n[0];
+  P.inv2 = 1.0 / n[1];
+  P.inv3 = 1.0 / n[2];
   for (int q = 0; q < 9; q++) P.c[q] = car2lat[q];
   for (int q = 0; q < 27; q++) P.lid[q] = lat_i_dist[q];
   return P;

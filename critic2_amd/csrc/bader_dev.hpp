@@ -12,9 +12,27 @@ namespace c2 {
 struct BParams {
   const double *__restrict__ f;
   int n1, n2, n3;
+  double inv1, inv2, inv3;  // 1/n1, 1/n2, 1/n3 (index arithmetic)
   double c[9];    // car2lat, column-major
   double lid[27]; // lat_i_dist
 };
+
+// x / d and x % d for x < 2^32 from the FP64 reciprocal: the product is
+// within 2^-20 of x/d, so truncation is off by at most one, which the
+// remainder test corrects.
+__device__ __forceinline__ uint32_t udiv(uint32_t x, uint32_t d, double inv, uint32_t &rem) {
+  uint32_t q = (uint32_t)((double)x * inv);
+  int64_t r = (int64_t)x - (int64_t)q * (int64_t)d;
+  if (r < 0) {
+    q--;
+    r += d;
+  } else if (r >= (int64_t)d) {
+    q++;
+    r -= d;
+  }
+  rem = (uint32_t)r;
+  return q;
+}
 
 __device__ __forceinline__ int wrap1(int p, int n) {
   return p < 0 ? p + n : (p >= n ? p - n : p);
@@ -23,10 +41,11 @@ __device__ __forceinline__ uint32_t lin3(const BParams &P, int i, int j, int k) 
   return (uint32_t)i + (uint32_t)P.n1 * ((uint32_t)j + (uint32_t)P.n2 * (uint32_t)k);
 }
 __device__ __forceinline__ void coords(const BParams &P, uint32_t l, int &i, int &j, int &k) {
-  i = (int)(l % (uint32_t)P.n1);
-  uint32_t r = l / (uint32_t)P.n1;
-  j = (int)(r % (uint32_t)P.n2);
-  k = (int)(r / (uint32_t)P.n2);
+  uint32_t ri, rj;
+  const uint32_t r = udiv(l, (uint32_t)P.n1, P.inv1, ri);
+  k = (int)udiv(r, (uint32_t)P.n2, P.inv2, rj);
+  i = (int)ri;
+  j = (int)rj;
 }
 // sweep-order index: do i; do j; do k (k innermost), bader@proc.f90:151-153
 __device__ __forceinline__ uint64_t sweep_index(const BParams &P, uint32_t l) {

@@ -228,6 +228,14 @@ int c2gpu_ctx_kernel_time(c2gpu_ctx *ctx, const char *kernel, double *ms, int64_
   });
 }
 
+int c2gpu_ctx_kernel_bytes(c2gpu_ctx *ctx, const char *kernel, double *bytes) {
+  return guarded(ctx, [&] {
+    if (!kernel || !bytes) fail(C2GPU_ERR_ARG, "null argument");
+    auto it = ctx->ktime.find(kernel);
+    *bytes = it == ctx->ktime.end() ? 0.0 : it->second.bytes;
+  });
+}
+
 int c2gpu_ctx_kernel_names(c2gpu_ctx *ctx, char *names_out, uint64_t len) {
   return guarded(ctx, [&] {
     if (!names_out || len == 0) fail(C2GPU_ERR_ARG, "null buffer");

@@ -34,6 +34,7 @@ struct c2gpu_ctx {
   struct KTime {
     double ms = 0;
     int64_t launches = 0;
+    double bytes = 0;  // algorithmic bytes reported by the launching code
   };
   std::map<std::string, KTime> ktime;
 };
@@ -105,6 +106,11 @@ inline void flush_timing(c2gpu_ctx *ctx) {
     ctx->evpool.push_back(p.b);
   }
   ctx->pending.clear();
+}
+
+// Credit algorithmic bytes to a kernel's entry (only while timing is on).
+inline void add_kernel_bytes(c2gpu_ctx *ctx, const char *name, double bytes) {
+  if (ctx->timing) ctx->ktime[name].bytes += bytes;
 }
 
 // Kernel launch on the context's stream, timed when timing is enabled.

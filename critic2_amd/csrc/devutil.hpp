@@ -52,6 +52,17 @@ inline void exclusive_scan(c2gpu_ctx *ctx, const char *tag, const uint32_t *in, 
   C2_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, it, out, (int)n, ctx->stream));
 }
 
+// Sum of 32-bit counts in 64 bits.
+inline uint64_t sum_u32(c2gpu_ctx *ctx, const char *tag, const uint32_t *in, int64_t n) {
+  hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> it(in, ToU64{});
+  uint64_t *d_out = static_cast<uint64_t *>(buffer(ctx, std::string("sum_out_") + tag, 8));
+  size_t tmp = 0;
+  C2_HIP(hipcub::DeviceReduce::Sum(nullptr, tmp, it, d_out, (int)n, ctx->stream));
+  void *t = buffer(ctx, std::string("sum_tmp_") + tag, tmp);
+  C2_HIP(hipcub::DeviceReduce::Sum(t, tmp, it, d_out, (int)n, ctx->stream));
+  return read_scalar(ctx, d_out);
+}
+
 // Indices i with flag[i] != 0, in increasing order; returns the count.
 inline uint32_t compact_flags(c2gpu_ctx *ctx, const char *tag, const uint8_t *flag, uint32_t n,
                               uint32_t *out) {

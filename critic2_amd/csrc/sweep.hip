@@ -22,14 +22,16 @@
 // recurrence has exactly one solution and every fixpoint satisfies it, so the
 // converged story is the serial sweep.
 //
-// Event lists live in an append-only pool: point y's list is
-// pool[off[y] .. off[y]+cnt[y]).  The first step builds every list; later steps
-// are incremental and proportional to what changed: the lists of points
-// covered by a changed path are rewritten at the pool's end, the known==2
-// times of their neighbourhoods recomputed, and only starts whose inputs
-// changed are decided again.  A start's inputs are the events at the points
-// of its committed path, so the starts to redo are read off those points'
-// lists.  When the pool fills up, it is rebuilt from the story.
+// Event lists live in one pool: point y's list is pool[off[y] .. +cnt[y])
+// inside a reserved block of cap[y] slots.  The first step builds every list;
+// later steps are incremental and proportional to what changed: the lists of
+// points covered by a changed path are rewritten (in place when they fit,
+// else moved to the pool's end with room to grow), the known==2 times of their
+// neighbourhoods recomputed, and only starts whose inputs changed are decided
+// again.  A start's inputs are the events at the points of its committed path,
+// so the starts to redo are read off those points' lists.  Working sets are
+// byte flags gathered into index lists by one streaming pass each, with one
+// atomic per workgroup.  When the pool fills up, it is rebuilt from the story.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -55,13 +57,13 @@ __device__ __forceinline__ int32_t ev_lab(uint64_t e) { return (int32_t)(e & ((1
 __device__ __forceinline__ uint64_t make_event(uint64_t t0, int l, int32_t lab) {
   return ((t0 | (uint64_t)l) << LAB_BITS) | (uint64_t)(uint32_t)lab;
 }
-// the start point (linear index) whose path produced an event
+// the start point (linear index) whose path produced an event; the sweep
+// index of a grid of fewer than 2^32 points fits 32 bits
 __device__ __forceinline__ uint32_t event_start(const BParams &P, uint64_t e) {
-  const uint64_t sg = ev_time(e) >> POS_BITS;
-  const uint32_t k = (uint32_t)(sg % (uint64_t)P.n3);
-  const uint64_t r = sg / (uint64_t)P.n3;
-  const uint32_t j = (uint32_t)(r % (uint64_t)P.n2);
-  const uint32_t i = (uint32_t)(r / (uint64_t)P.n2);
+  const uint32_t sg = (uint32_t)(ev_time(e) >> POS_BITS);
+  uint32_t k, j;
+  const uint32_t r = udiv(sg, (uint32_t)P.n3, P.inv3, k);
+  const uint32_t i = udiv(r, (uint32_t)P.n2, P.inv2, j);
   return lin3(P, (int)i, (int)j, (int)k);
 }
 
@@ -69,6 +71,7 @@ __device__ __forceinline__ uint32_t event_start(const BParams &P, uint64_t e) {
 struct Lists {
   uint64_t *off;
   uint32_t *cnt;
+  uint32_t *cap;
   uint64_t *ev;
   __device__ __forceinline__ uint64_t b(uint32_t y) const { return off[y]; }
   __device__ __forceinline__ uint64_t e(uint32_t y) const { return off[y] + cnt[y]; }
@@ -102,13 +105,70 @@ __device__ __forceinline__ uint64_t first_at_or_after(const uint64_t *__restrict
 }
 
 __device__ __forceinline__ void wave_add(unsigned long long *dst, uint64_t v) {
+  if (!dst) return;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-// Set flag[x]; the first setter appends x to list (each x at most once).
-__device__ __forceinline__ void flag_append(uint32_t *flag, uint32_t x, uint32_t *list, uint32_t *n) {
-  if (flag[x] == 0 && atomicExch(&flag[x], 1u) == 0) list[atomicAdd(n, 1u)] = x;
+// Appends val to list where pred holds: one atomic per wavefront.  Every lane
+// of the wavefront must call it.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t *list, uint32_t *n) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = val;
+}
+
+// Gathers the indices of set byte flags (flags are 0 or 1; the flag array is
+// padded with zeros to a multiple of 16 bytes) into list, in no particular
+// order, clearing the flags if clear != 0.  Each thread takes 16 flags; one
+// atomic per workgroup and pass.
+__global__ void __launch_bounds__(256) k_collect(uint32_t N, uint8_t *__restrict__ flag, int clear,
+                                                 uint32_t *__restrict__ list, uint32_t *n) {
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t bbase;
+  const uint32_t nq = (N + 15) >> 4;
+  uint4 *fq = reinterpret_cast<uint4 *>(flag);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < nq; q0 += gridDim.x * blockDim.x) {
+    const uint32_t q = q0 + threadIdx.x;
+    const uint4 v = (q < nq) ? fq[q] : make_uint4(0u, 0u, 0u, 0u);
+    const bool any = (v.x | v.y | v.z | v.w) != 0u;
+    if (clear && any) fq[q] = make_uint4(0u, 0u, 0u, 0u);
+    const int c = __popc(v.x & 0x01010101u) + __popc(v.y & 0x01010101u) + __popc(v.z & 0x01010101u) +
+                  __popc(v.w & 0x01010101u);
+    int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wv] = (uint32_t)incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      bbase = tot ? atomicAdd(n, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t o = bbase + (uint32_t)(incl - c);
+    for (int w = 0; w < wv; w++) o += wsum[w];
+    if (c) {
+      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++)
+          if ((wd[a] >> (8 * b)) & 1u) list[o++] = 16 * q + 4 * a + b;
+    }
+    __syncthreads();  // wsum and bbase are rewritten by the next pass
+  }
+}
+
+template <class F>
+__global__ void k_clear_list(uint32_t n, const uint32_t *__restrict__ list, F *__restrict__ flag) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) flag[list[t]] = 0;
 }
 
 // ---------------------------------------------------------------- event lists
@@ -131,28 +191,50 @@ __global__ void k_ev_count(uint32_t N, const uint64_t *__restrict__ toff, const 
   for (int l = 0; l < e; l++) atomicAdd(&cnt[tp[o + l]], 1u);
 }
 
+// Reserved block per list: the count plus a quarter, plus one.  capv has a
+// zero sentinel at N for the offset scan.
+__global__ void k_reserve(uint32_t N, const uint32_t *__restrict__ cnt, uint32_t *__restrict__ cap,
+                          uint32_t *__restrict__ capv) {
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y > N) return;
+  if (y == N) {
+    capv[N] = 0;
+    return;
+  }
+  const uint32_t c = cnt[y];
+  const uint32_t r = c + (c >> 2) + 1;
+  cap[y] = r;
+  capv[y] = r;
+}
+
 // Writes the events of the starts in `list` (all starts when list == null)
 // into their points' lists at cursor cur[y].
-__global__ void k_ev_fill(BParams P, uint32_t n, const uint32_t *__restrict__ list,
-                          const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                          const int32_t *__restrict__ ell, const int32_t *__restrict__ lam, Lists L,
-                          uint32_t *__restrict__ cur, int *err) {
+__global__ void __launch_bounds__(256) k_ev_fill(BParams P, uint32_t n, const uint32_t *__restrict__ list,
+                                                 const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
+                                                 const int32_t *__restrict__ ell, const int32_t *__restrict__ lam,
+                                                 Lists L, uint32_t *__restrict__ cur, int *err,
+                                                 unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const uint32_t s = list ? list[t] : t;
-  const int e = ell[s];
-  if (e == 0) return;
-  const uint64_t t0 = sweep_index(P, s) << POS_BITS;
-  const int32_t lab = lam[s];
-  const uint64_t o = toff[s];
-  for (int l = 0; l < e; l++) {
-    const uint32_t y = tp[o + l];
-    const uint32_t slot = atomicAdd(&cur[y], 1u);
-    if (slot < L.cnt[y])
-      L.ev[L.off[y] + slot] = make_event(t0, l, lab);
-    else
-      atomicOr(err, 4);  // count/fill mismatch (never expected): refuse to write
+  const uint32_t s = t < n ? (list ? list[t] : t) : 0u;
+  const int e = t < n ? ell[s] : 0;
+  if (e > 0) {
+    const uint64_t t0 = sweep_index(P, s) << POS_BITS;
+    const int32_t lab = lam[s];
+    const uint64_t o = toff[s];
+    for (int l0 = 0; l0 < e; l0 += 4) {  // four slot reservations in flight at once
+      const int m = (e - l0) < 4 ? e - l0 : 4;
+      uint32_t y[4], slot[4];
+      for (int u = 0; u < 4; u++) y[u] = u < m ? tp[o + l0 + u] : 0u;
+      for (int u = 0; u < 4; u++) slot[u] = u < m ? atomicAdd(&cur[y[u]], 1u) : 0u;
+      for (int u = 0; u < m; u++) {
+        if (slot[u] < L.cnt[y[u]])
+          L.ev[L.off[y[u]] + slot[u]] = make_event(t0, l0 + u, lab);
+        else
+          atomicOr(err, 4);  // count/fill mismatch (never expected): refuse to write
+      }
+    }
   }
+  wave_add(work, (uint64_t)e);
 }
 
 __device__ __forceinline__ bool list_uniform(const uint64_t *__restrict__ ev, uint64_t b, uint64_t e) {
@@ -168,31 +250,33 @@ __device__ __forceinline__ bool list_uniform(const uint64_t *__restrict__ ev, ui
 // queued for the workgroup and radix sorts.
 __global__ void k_ev_sort(uint32_t n, const uint32_t *__restrict__ list, Lists L, uint8_t *__restrict__ uni,
                           uint32_t *__restrict__ longlist, uint32_t *nlong, uint32_t *__restrict__ hugelist,
-                          uint32_t *nhuge) {
+                          uint32_t *nhuge, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const uint32_t y = list ? list[t] : t;
-  const uint64_t b = L.b(y), e = L.e(y);
-  const uint64_t m = e - b;
-  if (m > SORT_LDS) {
-    hugelist[atomicAdd(nhuge, 1u)] = y;
-    return;
-  }
-  if (m > SORT_INLINE) {
-    longlist[atomicAdd(nlong, 1u)] = y;
-    return;
-  }
-  uint64_t *ev = L.ev;
-  for (uint64_t i = b + 1; i < e; i++) {
-    const uint64_t v = ev[i];
-    uint64_t j = i;
-    while (j > b && ev[j - 1] > v) {
-      ev[j] = ev[j - 1];
-      j--;
+  uint64_t sorted = 0;
+  if (t < n) {
+    const uint32_t y = list ? list[t] : t;
+    const uint64_t b = L.b(y), e = L.e(y);
+    const uint64_t m = e - b;
+    if (m > SORT_LDS) {
+      hugelist[atomicAdd(nhuge, 1u)] = y;
+    } else if (m > SORT_INLINE) {
+      longlist[atomicAdd(nlong, 1u)] = y;
+    } else {
+      uint64_t *ev = L.ev;
+      for (uint64_t i = b + 1; i < e; i++) {
+        const uint64_t v = ev[i];
+        uint64_t j = i;
+        while (j > b && ev[j - 1] > v) {
+          ev[j] = ev[j - 1];
+          j--;
+        }
+        ev[j] = v;
+      }
+      uni[y] = list_uniform(ev, b, e);
+      sorted = m;
     }
-    ev[j] = v;
   }
-  uni[y] = list_uniform(ev, b, e);
+  wave_add(work, sorted);
 }
 
 // One workgroup per long list: bitonic sort in LDS.
@@ -372,6 +456,7 @@ __device__ void decide_start(const BParams &P, uint32_t s, const uint64_t *__res
       e = q + 1;
       break;
     }
+  // (e positions of the path were read: tp and Kt)
   const uint32_t end = tp[o + e - 1];
   la = v_at(L, end, ts, false);
   if (la == 0) {
@@ -382,117 +467,176 @@ __device__ void decide_start(const BParams &P, uint32_t s, const uint64_t *__res
 
 // Decides the starts in `list` (every point when list == null); changed
 // decisions are flagged and appended to `chlist`.
-__global__ void k_decide(BParams P, uint32_t n, const uint32_t *__restrict__ list,
-                         const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp, Lists L,
-                         const uint64_t *__restrict__ Kt, const int32_t *__restrict__ maxlab,
-                         const int32_t *__restrict__ ell, const int32_t *__restrict__ lam,
-                         int32_t *__restrict__ ell2, int32_t *__restrict__ lam2, uint8_t *__restrict__ chg,
-                         uint32_t *__restrict__ chlist, uint32_t *nch, int *err) {
+__global__ void __launch_bounds__(256) k_decide(BParams P, uint32_t n, const uint32_t *__restrict__ list,
+                                                const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
+                                                Lists L, const uint64_t *__restrict__ Kt,
+                                                const int32_t *__restrict__ maxlab, const int32_t *__restrict__ ell,
+                                                const int32_t *__restrict__ lam, int32_t *__restrict__ ell2,
+                                                int32_t *__restrict__ lam2, uint8_t *__restrict__ chg,
+                                                uint32_t *__restrict__ chlist, uint32_t *nch, int *err,
+                                                unsigned long long *first_change, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const uint32_t s = list ? list[t] : t;
-  int e;
-  int32_t la;
-  decide_start(P, s, toff, tp, L, Kt, maxlab, e, la, err);
-  ell2[s] = e;
-  lam2[s] = la;
-  if (e != ell[s] || la != lam[s]) {
-    chg[s] = 1;
-    chlist[atomicAdd(nch, 1u)] = s;
+  bool changed = false;
+  uint32_t s = 0;
+  int e = 0;
+  if (t < n) {
+    s = list ? list[t] : t;
+    int32_t la;
+    decide_start(P, s, toff, tp, L, Kt, maxlab, e, la, err);
+    ell2[s] = e;
+    lam2[s] = la;
+    changed = (e != ell[s] || la != lam[s]);
+    if (changed) chg[s] = 1;
   }
+  wave_add(work, (uint64_t)e);
+  if (first_change) {  // earliest changed start in sweep order (diagnostics)
+    uint64_t v = changed ? sweep_index(P, s) : ~0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t w = __shfl_down(v, o, 64);
+      v = w < v ? w : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(first_change, (unsigned long long)v);
+  }
+  wave_append(changed, s, chlist, nch);
 }
 
 // ---------------------------------------------------------------- incremental step
 
 // Count changes per point from the changed starts' old and new paths; every
-// point on either path gets its list rewritten (listed once in dl).
-__global__ void k_delta(uint32_t nc, const uint32_t *__restrict__ chlist, const uint64_t *__restrict__ toff,
-                        const uint32_t *__restrict__ tp, const int32_t *__restrict__ ell,
-                        const int32_t *__restrict__ ell2, int32_t *__restrict__ delta, uint32_t *__restrict__ dflag,
-                        uint32_t *__restrict__ dl, uint32_t *ndl) {
+// point on either path gets its list rewritten (dflag).
+__global__ void __launch_bounds__(256) k_delta(uint32_t nc, const uint32_t *__restrict__ chlist,
+                                               const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
+                                               const int32_t *__restrict__ ell, const int32_t *__restrict__ ell2,
+                                               const int32_t *__restrict__ lam, const int32_t *__restrict__ lam2,
+                                               int32_t *__restrict__ delta, uint8_t *__restrict__ dflag,
+                                               unsigned long long *stats, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nc) return;
-  const uint32_t s = chlist[t];
-  const uint64_t o = toff[s];
-  const int eo = ell[s], en = ell2[s];
-  const int m = eo > en ? eo : en;
-  for (int l = 0; l < m; l++) {
-    const uint32_t y = tp[o + l];
-    const int d = (l < en ? 1 : 0) - (l < eo ? 1 : 0);
-    if (d) atomicAdd(&delta[y], d);
-    flag_append(dflag, y, dl, ndl);
+  uint64_t st[4] = {0, 0, 0, 0};
+  uint64_t upd = 0;
+  if (t < nc) {
+    const uint32_t s = chlist[t];
+    const uint64_t o = toff[s];
+    const int eo = ell[s], en = ell2[s];
+    const int m = eo > en ? eo : en, mn = eo < en ? eo : en;
+    const bool same = lam[s] == lam2[s];
+    for (int l = 0; l < m; l++) {
+      const uint32_t y = tp[o + l];
+      const int d = (l < en ? 1 : 0) - (l < eo ? 1 : 0);
+      if (d) atomicAdd(&delta[y], d);
+      dflag[y] = 1;
+    }
+    st[0] = same && mn > 0;           // changes that keep the label
+    st[1] = same ? (uint64_t)mn : 0;  // unchanged prefix events
+    st[2] = (uint64_t)m;              // positions touched
+    st[3] = same ? 0 : (uint64_t)mn;  // prefix events relabelled
+    upd = (uint64_t)(m - mn);
   }
+  if (stats)
+    for (int q = 0; q < 4; q++) wave_add(stats + q, st[q]);
+  wave_add(work, st[2]);
+  wave_add(work + 1, upd);
 }
 
+// New size of each rewritten list; a list that outgrows its block gets a new
+// block of 1.5x its size (+2) at the pool's end (alloc > 0, scanned).
 __global__ void k_patch_count(uint32_t nd, const uint32_t *__restrict__ dl, const uint32_t *__restrict__ cnt,
-                              int32_t *__restrict__ delta, uint32_t *__restrict__ ncnt) {
+                              const uint32_t *__restrict__ cap, int32_t *__restrict__ delta,
+                              uint32_t *__restrict__ ncnt, uint32_t *__restrict__ alloc, int *err) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > nd) return;
   if (t == nd) {
-    ncnt[nd] = 0;
+    alloc[nd] = 0;
     return;
   }
   const uint32_t y = dl[t];
-  ncnt[t] = (uint32_t)((int64_t)cnt[y] + delta[y]);
+  const int64_t c = (int64_t)cnt[y] + delta[y];
   delta[y] = 0;
+  if (c < 0) atomicOr(err, 4);  // more events removed than present (never expected)
+  const uint32_t nc = c < 0 ? 0u : (uint32_t)c;
+  ncnt[t] = nc;
+  alloc[t] = nc > cap[y] ? nc + (nc >> 1) + 2 : 0u;
 }
 
-// New list of each rewritten point at the pool's end: its old events minus
-// those of changed starts; the changed starts' new events follow (k_ev_fill).
-__global__ void k_patch_keep(BParams P, uint32_t nd, const uint32_t *__restrict__ dl,
-                             const uint64_t *__restrict__ noff, uint64_t base, Lists L,
-                             const uint8_t *__restrict__ chg, uint32_t *__restrict__ cur, int *err) {
+// Rewrites each list: its old events minus those of changed starts, in place
+// (the write index never passes the read index) or in its new block; the
+// changed starts' new events follow (k_ev_fill).
+__global__ void __launch_bounds__(256) k_patch_keep(BParams P, uint32_t nd, const uint32_t *__restrict__ dl,
+                                                    const uint32_t *__restrict__ ncnt,
+                                                    const uint32_t *__restrict__ alloc,
+                                                    const uint64_t *__restrict__ aoff, uint64_t base, Lists L,
+                                                    const uint8_t *__restrict__ chg, uint32_t *__restrict__ cur,
+                                                    int *err, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nd) return;
-  const uint32_t y = dl[t];
-  const uint64_t ob = L.off[y], oe = ob + L.cnt[y];
-  const uint64_t nb = base + noff[t];
-  const uint64_t nn = noff[t + 1] - noff[t];
-  uint32_t kept = 0;
-  for (uint64_t x = ob; x < oe; x++) {
-    const uint64_t v = L.ev[x];
-    if (!chg[event_start(P, v)]) {
-      if (kept < nn)
-        L.ev[nb + kept] = v;
-      else
-        atomicOr(err, 4);
-      kept++;
+  uint64_t nold = 0, nkept = 0;
+  if (t < nd) {
+    const uint32_t y = dl[t];
+    const uint64_t ob = L.off[y], oe = ob + L.cnt[y];
+    const uint32_t nn = ncnt[t];
+    const bool move = alloc[t] != 0;
+    const uint64_t nb = move ? base + aoff[t] : ob;
+    const uint32_t room = move ? alloc[t] : L.cap[y];
+    const uint32_t lim = nn <= room ? nn : room;  // room >= nn by construction
+    if (nn > room) atomicOr(err, 4);
+    uint32_t kept = 0;
+    for (uint64_t x = ob; x < oe; x++) {
+      const uint64_t v = L.ev[x];
+      if (!chg[event_start(P, v)]) {
+        if (kept < lim)
+          L.ev[nb + kept] = v;
+        else
+          atomicOr(err, 4);
+        kept++;
+      }
     }
+    cur[y] = kept;
+    L.off[y] = nb;
+    L.cnt[y] = lim;
+    if (move) L.cap[y] = room;
+    nold = oe - ob;
+    nkept = kept;
   }
-  cur[y] = kept;
-  L.off[y] = nb;
-  L.cnt[y] = (uint32_t)nn;
+  wave_add(work, nold);
+  wave_add(work + 1, nkept);
 }
 
 // Points whose known==2 time may change: the rewritten points and their face
-// neighbours (listed once in ql).
-__global__ void k_mark_kt(BParams P, uint32_t nd, const uint32_t *__restrict__ dl, uint32_t *__restrict__ qflag,
-                          uint32_t *__restrict__ ql, uint32_t *nql) {
+// neighbours.
+__global__ void k_mark_kt(BParams P, uint32_t nd, const uint32_t *__restrict__ dl, uint8_t *__restrict__ qflag) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nd) return;
   const uint32_t y = dl[t];
   int i, j, k;
   coords(P, y, i, j, k);
-  flag_append(qflag, y, ql, nql);
-  flag_append(qflag, lin3(P, wrap1(i + 1, P.n1), j, k), ql, nql);
-  flag_append(qflag, lin3(P, wrap1(i - 1, P.n1), j, k), ql, nql);
-  flag_append(qflag, lin3(P, i, wrap1(j + 1, P.n2), k), ql, nql);
-  flag_append(qflag, lin3(P, i, wrap1(j - 1, P.n2), k), ql, nql);
-  flag_append(qflag, lin3(P, i, j, wrap1(k + 1, P.n3)), ql, nql);
-  flag_append(qflag, lin3(P, i, j, wrap1(k - 1, P.n3)), ql, nql);
+  qflag[y] = 1;
+  qflag[lin3(P, wrap1(i + 1, P.n1), j, k)] = 1;
+  qflag[lin3(P, wrap1(i - 1, P.n1), j, k)] = 1;
+  qflag[lin3(P, i, wrap1(j + 1, P.n2), k)] = 1;
+  qflag[lin3(P, i, wrap1(j - 1, P.n2), k)] = 1;
+  qflag[lin3(P, i, j, wrap1(k + 1, P.n3))] = 1;
+  qflag[lin3(P, i, j, wrap1(k - 1, P.n3))] = 1;
 }
 
 // Starts to decide again: those with an event at a point whose list or
-// known==2 time changed, and those points themselves.
-__global__ void k_mark_decide(BParams P, uint32_t nq, const uint32_t *__restrict__ ql, Lists L,
-                              const uint32_t *__restrict__ dflag, const uint8_t *__restrict__ ktchg,
-                              uint32_t *__restrict__ sflag, uint32_t *__restrict__ sl, uint32_t *nsl) {
+// known==2 time changed, and those points themselves.  (A start's decision
+// reads volnum at the start and at its path's end and the known==2 marks
+// along its committed path, bader@proc.f90:154-211; every such point carries
+// one of its events.)
+__global__ void __launch_bounds__(256) k_mark_decide(BParams P, uint32_t nq, const uint32_t *__restrict__ ql,
+                                                     Lists L, const uint8_t *__restrict__ dflag,
+                                                     const uint8_t *__restrict__ ktchg, uint8_t *__restrict__ sflag,
+                                                     unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nq) return;
-  const uint32_t x = ql[t];
-  if (!(dflag[x] || ktchg[x])) return;
-  flag_append(sflag, x, sl, nsl);
-  for (uint64_t q = L.b(x); q < L.e(x); q++) flag_append(sflag, event_start(P, L.ev[q]), sl, nsl);
+  uint64_t scanned = 0, marked = 0;
+  const uint32_t x = t < nq ? ql[t] : 0u;
+  if (t < nq && (dflag[x] || ktchg[x])) {
+    sflag[x] = 1;
+    const uint64_t b = L.b(x), e = L.e(x);
+    for (uint64_t q = b; q < e; q++) sflag[event_start(P, L.ev[q])] = 1;
+    scanned = e - b;
+    marked = 1;
+  }
+  wave_add(work, scanned);
+  wave_add(work + 1, marked);
 }
 
 __global__ void k_apply(uint32_t nc, const uint32_t *__restrict__ chlist, int32_t *__restrict__ ell,
@@ -504,12 +648,6 @@ __global__ void k_apply(uint32_t nc, const uint32_t *__restrict__ chlist, int32_
   ell[s] = ell2[s];
   lam[s] = lam2[s];
   chg[s] = 0;
-}
-
-template <class F>
-__global__ void k_clear_list(uint32_t n, const uint32_t *__restrict__ list, F *__restrict__ flag) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) flag[list[t]] = 0;
 }
 
 __global__ void k_final_state(uint32_t N, Lists L, const uint64_t *__restrict__ Kt, int32_t *__restrict__ V,
@@ -529,13 +667,13 @@ __global__ void k_final_state(uint32_t N, Lists L, const uint64_t *__restrict__ 
 
 // Sorts the lists of the points in `list` (all points when null).
 static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const Lists &L, uint8_t *uni,
-                       uint32_t *longlist, uint32_t *hugelist) {
+                       uint32_t *longlist, uint32_t *hugelist, unsigned long long *work) {
   hipStream_t st = ctx->stream;
   if (n == 0) return;
   uint32_t *d_n = buf<uint32_t>(ctx, "sweep_nlist", 2);
   C2_HIP(hipMemsetAsync(d_n, 0, 2 * sizeof(uint32_t), st));
   C2_LAUNCH(ctx, k_ev_sort, dim3(grid_for(n, 256)), dim3(256), 0, n, list, L, uni, longlist, d_n, hugelist,
-            d_n + 1);
+            d_n + 1, work);
   uint32_t nlh[2];
   C2_HIP(hipMemcpyAsync(nlh, d_n, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   C2_HIP(hipStreamSynchronize(st));
@@ -573,68 +711,91 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   hipStream_t st = ctx->stream;
   const int B = 256;
   const unsigned gN = grid_for(N, B);
+  const unsigned gW = grid_for(((int64_t)N + 15) / 16, B);
+  const unsigned gCollect = gW < 2048 ? gW : 2048;
   const bool full_every_step = std::getenv("C2GPU_SWEEP_FULL") != nullptr;
   const bool trace = std::getenv("C2GPU_SWEEP_TRACE") != nullptr;
   SweepResult R{0, 0, 0};
+  const size_t NF = ((size_t)N + 16) & ~(size_t)15;  // byte flags padded to whole 16-byte groups
   int32_t *ell = buf<int32_t>(ctx, "sweep_ell", N), *lam = buf<int32_t>(ctx, "sweep_lam", N);
   int32_t *ell2 = buf<int32_t>(ctx, "sweep_ell2", N), *lam2 = buf<int32_t>(ctx, "sweep_lam2", N);
   uint32_t *cntN = buf<uint32_t>(ctx, "sweep_cnt", (size_t)N + 1);
+  uint32_t *capN = buf<uint32_t>(ctx, "sweep_cap", N);
+  uint32_t *capv = buf<uint32_t>(ctx, "sweep_capv", (size_t)N + 1);
   uint64_t *offN = buf<uint64_t>(ctx, "sweep_off", (size_t)N + 1);
   uint32_t *cur = buf<uint32_t>(ctx, "sweep_cur", N);
   int32_t *delta = buf<int32_t>(ctx, "sweep_delta", N);
   uint64_t *Kt = buf<uint64_t>(ctx, "sweep_kt", N);
   uint8_t *uni = buf<uint8_t>(ctx, "sweep_uni", N);
-  uint8_t *chg = buf<uint8_t>(ctx, "sweep_chg", N);
-  uint8_t *ktchg = buf<uint8_t>(ctx, "sweep_ktchg", N);
-  uint32_t *dflag = buf<uint32_t>(ctx, "sweep_dflag", N);  // list rewritten this step
-  uint32_t *qflag = buf<uint32_t>(ctx, "sweep_qflag", N);  // known==2 time to recompute
-  uint32_t *sflag = buf<uint32_t>(ctx, "sweep_sflag", N);  // start to decide again
+  uint8_t *chg = buf<uint8_t>(ctx, "sweep_chg", NF);
+  uint8_t *ktchg = buf<uint8_t>(ctx, "sweep_ktchg", NF);
+  uint8_t *dflag = buf<uint8_t>(ctx, "sweep_dflag", NF);  // list rewritten this step
+  uint8_t *qflag = buf<uint8_t>(ctx, "sweep_qflag", NF);  // known==2 time to recompute
+  uint8_t *sflag = buf<uint8_t>(ctx, "sweep_sflag", NF);  // start to decide again
   uint32_t *dl = buf<uint32_t>(ctx, "sweep_dl", N), *ql = buf<uint32_t>(ctx, "sweep_ql", N);
   uint32_t *sl = buf<uint32_t>(ctx, "sweep_sl", N);
   uint32_t *longlist = buf<uint32_t>(ctx, "sweep_long", N), *hugelist = buf<uint32_t>(ctx, "sweep_huge", N);
   uint32_t *chlist = buf<uint32_t>(ctx, "sweep_chlist", N);
-  uint32_t *ncnt = buf<uint32_t>(ctx, "sweep_ncnt", (size_t)N + 1);
-  uint64_t *noff = buf<uint64_t>(ctx, "sweep_noff", (size_t)N + 1);
+  uint32_t *ncnt = buf<uint32_t>(ctx, "sweep_ncnt", N);
+  uint32_t *alloc = buf<uint32_t>(ctx, "sweep_alloc", (size_t)N + 1);
+  uint64_t *aoff = buf<uint64_t>(ctx, "sweep_aoff", (size_t)N + 1);
   uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 4);  // nch, ndl, nql, nsl
   int *d_err = buf<int>(ctx, "sweep_err", 1);
+  unsigned long long *d_stats = buf<unsigned long long>(ctx, "sweep_stats", 5);
+  unsigned long long *d_work = buf<unsigned long long>(ctx, "sweep_work", 16);  // algorithmic work counters
   C2_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
+  C2_HIP(hipMemsetAsync(d_work, 0, 16 * sizeof(unsigned long long), st));
+  // host-side totals for the algorithmic-bytes report
+  double s_nc = 0, s_nd = 0, s_nq = 0, s_ns = 0, s_coll = 0, n_coll = 0, n_full = 0, n_decall = 0, s_nev = 0;
   C2_HIP(hipMemsetAsync(delta, 0, sizeof(int32_t) * N, st));
-  C2_HIP(hipMemsetAsync(chg, 0, N, st));
-  C2_HIP(hipMemsetAsync(ktchg, 0, N, st));
-  C2_HIP(hipMemsetAsync(dflag, 0, sizeof(uint32_t) * N, st));
-  C2_HIP(hipMemsetAsync(qflag, 0, sizeof(uint32_t) * N, st));
-  C2_HIP(hipMemsetAsync(sflag, 0, sizeof(uint32_t) * N, st));
+  for (uint8_t *f : {chg, ktchg, dflag, qflag, sflag}) C2_HIP(hipMemsetAsync(f, 0, NF, st));
 
   Lists L;
   L.off = offN;
   L.cnt = cntN;
+  L.cap = capN;
   L.ev = nullptr;
   uint64_t pool_cap = 0, pool_top = 0;
 
-  // every list from the story: counts, offsets, events, order, known==2 times
+  auto collect = [&](uint8_t *flag, int clear, uint32_t *list, uint32_t *d_cnt) -> uint32_t {
+    C2_HIP(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st));
+    C2_LAUNCH(ctx, k_collect, dim3(gCollect), dim3(B), 0, N, flag, clear, list, d_cnt);
+    const uint32_t c = read_scalar(ctx, d_cnt);
+    s_coll += c;
+    n_coll += 1;
+    return c;
+  };
+
+  // every list from the story: counts, reserved blocks, events, order,
+  // known==2 times
   auto full_build = [&]() {
     C2_HIP(hipMemsetAsync(cntN, 0, sizeof(uint32_t) * ((size_t)N + 1), st));
     C2_LAUNCH(ctx, k_ev_count, dim3(gN), dim3(B), 0, N, toff, tp, ell, cntN);
-    exclusive_scan(ctx, "ev", cntN, offN, (int64_t)N + 1);
-    const uint64_t nev = read_scalar(ctx, offN + N);
-    if (nev + (uint64_t)N * 4 > pool_cap) {  // room for later rewrites too
-      pool_cap = nev * 3 + (uint64_t)N * 4;
-      L.ev = buf<uint64_t>(ctx, "sweep_pool", pool_cap);
+    const uint64_t nev = sum_u32(ctx, "ev", cntN, N);
+    C2_LAUNCH(ctx, k_reserve, dim3(grid_for((int64_t)N + 1, B)), dim3(B), 0, N, cntN, capN, capv);
+    exclusive_scan(ctx, "ev", capv, offN, (int64_t)N + 1);
+    const uint64_t reserved = read_scalar(ctx, offN + N);
+    const uint64_t want = reserved + (reserved >> 2) + (uint64_t)N;  // room for moved lists
+    if (want > pool_cap) {
+      L.ev = buf<uint64_t>(ctx, "sweep_pool", want);
       pool_cap = ctx->bufs["sweep_pool"].bytes / 8;
     }
-    pool_top = nev;
+    pool_top = reserved;
     C2_HIP(hipMemsetAsync(cur, 0, sizeof(uint32_t) * N, st));
     C2_LAUNCH(ctx, k_ev_fill, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, toff, tp, ell, lam, L, cur,
-              d_err);
-    sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist);
+              d_err, d_work + 4);
+    sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8);
     C2_LAUNCH(ctx, k_kentry, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, L, uni, Kt,
               (uint8_t *)nullptr);
     R.events_total += (int64_t)nev;
+    n_full += 1;
+    s_nev += (double)nev;
   };
   auto decide_all = [&]() -> uint32_t {
     C2_HIP(hipMemsetAsync(d_n, 0, sizeof(uint32_t), st));
     C2_LAUNCH(ctx, k_decide, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, toff, tp, L, Kt, maxlab, ell,
-              lam, ell2, lam2, chg, chlist, d_n, d_err);
+              lam, ell2, lam2, chg, chlist, d_n, d_err, (unsigned long long *)nullptr, d_work + 7);
+    n_decall += 1;
     return read_scalar(ctx, d_n);
   };
 
@@ -648,56 +809,63 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     it++;
     const unsigned gC = grid_for(nc, B);
     // 1. points whose lists change, and their new sizes
-    uint32_t counts[4] = {0, 0, 0, 0};
-    C2_HIP(hipMemsetAsync(d_n + 1, 0, 3 * sizeof(uint32_t), st));
-    C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, delta, dflag, dl, d_n + 1);
-    const uint32_t nd = read_scalar(ctx, d_n + 1);
-    C2_LAUNCH(ctx, k_patch_count, dim3(grid_for((int64_t)nd + 1, B)), dim3(B), 0, nd, dl, cntN, delta, ncnt);
-    exclusive_scan(ctx, "patch", ncnt, noff, (int64_t)nd + 1);
-    const uint64_t added = read_scalar(ctx, noff + nd);
+    if (trace) C2_HIP(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), st));
+    C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, lam, lam2, delta, dflag,
+              trace ? d_stats : nullptr, d_work + 5);
+    s_nc += nc;
+    const uint32_t nd = collect(dflag, 0, dl, d_n + 1);
+    const unsigned gD = grid_for(nd, B);
+    C2_LAUNCH(ctx, k_patch_count, dim3(grid_for((int64_t)nd + 1, B)), dim3(B), 0, nd, dl, cntN, capN, delta, ncnt,
+              alloc, d_err);
+    exclusive_scan(ctx, "patch", alloc, aoff, (int64_t)nd + 1);
+    const uint64_t added = read_scalar(ctx, aoff + nd);
     if (full_every_step || pool_top + added > pool_cap) {
       // rebuild every list from the updated story (compacts the pool)
-      C2_LAUNCH(ctx, k_clear_list<uint32_t>, dim3(grid_for(nd, B)), dim3(B), 0, nd, dl, dflag);
+      C2_LAUNCH(ctx, k_clear_list<uint8_t>, dim3(gD), dim3(B), 0, nd, dl, dflag);
       C2_LAUNCH(ctx, k_apply, dim3(gC), dim3(B), 0, nc, chlist, ell, lam, ell2, lam2, chg);
       full_build();
       nc = decide_all();
       if (trace) std::fprintf(stderr, "[sweep] it %d full rebuild, changed %u\n", it, nc);
       continue;
     }
-    // 2. rewrite those lists at the pool's end; add the changed starts' events
-    C2_LAUNCH(ctx, k_patch_keep, dim3(grid_for(nd, B)), dim3(B), 0, P, nd, dl, noff, pool_top, L, chg, cur, d_err);
-    C2_LAUNCH(ctx, k_ev_fill, dim3(gC), dim3(B), 0, P, nc, chlist, toff, tp, ell2, lam2, L, cur, d_err);
+    // 2. rewrite those lists; add the changed starts' events
+    C2_LAUNCH(ctx, k_patch_keep, dim3(gD), dim3(B), 0, P, nd, dl, ncnt, alloc, aoff, pool_top, L, chg, cur, d_err,
+              d_work + 0);
+    C2_LAUNCH(ctx, k_ev_fill, dim3(gC), dim3(B), 0, P, nc, chlist, toff, tp, ell2, lam2, L, cur, d_err, d_work + 4);
+    s_nd += nd;
     C2_LAUNCH(ctx, k_apply, dim3(gC), dim3(B), 0, nc, chlist, ell, lam, ell2, lam2, chg);
     pool_top += added;
-    R.events_total += (int64_t)added;
-    sort_lists(ctx, nd, dl, L, uni, longlist, hugelist);
+    sort_lists(ctx, nd, dl, L, uni, longlist, hugelist, d_work + 8);
     // 3. known==2 times around the rewritten lists
-    C2_LAUNCH(ctx, k_mark_kt, dim3(grid_for(nd, B)), dim3(B), 0, P, nd, dl, qflag, ql, d_n + 2);
-    const uint32_t nq = read_scalar(ctx, d_n + 2);
-    C2_LAUNCH(ctx, k_kentry, dim3(grid_for(nq, B)), dim3(B), 0, P, nq, ql, L, uni, Kt, ktchg);
+    C2_LAUNCH(ctx, k_mark_kt, dim3(gD), dim3(B), 0, P, nd, dl, qflag);
+    const uint32_t nq = collect(qflag, 1, ql, d_n + 2);
+    const unsigned gQ = grid_for(nq, B);
+    C2_LAUNCH(ctx, k_kentry, dim3(gQ), dim3(B), 0, P, nq, ql, L, uni, Kt, ktchg);
     // 4. starts whose inputs changed decide again
-    C2_LAUNCH(ctx, k_mark_decide, dim3(grid_for(nq, B)), dim3(B), 0, P, nq, ql, L, dflag, ktchg, sflag, sl,
-              d_n + 3);
-    C2_HIP(hipMemcpyAsync(counts, d_n, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    C2_HIP(hipStreamSynchronize(st));
-    const uint32_t ns = counts[3];
+    C2_LAUNCH(ctx, k_mark_decide, dim3(gQ), dim3(B), 0, P, nq, ql, L, dflag, ktchg, sflag, d_work + 2);
+    s_nq += nq;
+    const uint32_t ns = collect(sflag, 1, sl, d_n + 3);
     C2_HIP(hipMemsetAsync(d_n, 0, sizeof(uint32_t), st));
+    if (trace) C2_HIP(hipMemsetAsync(d_stats + 4, 0xff, sizeof(unsigned long long), st));
     if (ns)
       C2_LAUNCH(ctx, k_decide, dim3(grid_for(ns, B)), dim3(B), 0, P, ns, sl, toff, tp, L, Kt, maxlab, ell, lam,
-                ell2, lam2, chg, chlist, d_n, d_err);
+                ell2, lam2, chg, chlist, d_n, d_err, trace ? d_stats + 4 : nullptr, d_work + 7);
+    s_ns += ns;
     // 5. clear the step's flags through their lists
-    C2_LAUNCH(ctx, k_clear_list<uint32_t>, dim3(grid_for(nd, B)), dim3(B), 0, nd, dl, dflag);
-    if (nq) {
-      C2_LAUNCH(ctx, k_clear_list<uint32_t>, dim3(grid_for(nq, B)), dim3(B), 0, nq, ql, qflag);
-      C2_LAUNCH(ctx, k_clear_list<uint8_t>, dim3(grid_for(nq, B)), dim3(B), 0, nq, ql, ktchg);
-    }
-    if (ns) C2_LAUNCH(ctx, k_clear_list<uint32_t>, dim3(grid_for(ns, B)), dim3(B), 0, ns, sl, sflag);
+    C2_LAUNCH(ctx, k_clear_list<uint8_t>, dim3(gD), dim3(B), 0, nd, dl, dflag);
+    C2_LAUNCH(ctx, k_clear_list<uint8_t>, dim3(gQ), dim3(B), 0, nq, ql, ktchg);
     nc = read_scalar(ctx, d_n);
     if (read_scalar(ctx, d_err) & 4) fail(C2GPU_ERR_ARG, "c2gpu_bader: internal inconsistency in the sweep event lists");
-    if (trace)
-      std::fprintf(stderr, "[sweep] it %d changed %u rewritten %u kt %u decided %u added %llu pool %llu/%llu\n", it,
-                   nc, nd, nq, ns, (unsigned long long)added, (unsigned long long)pool_top,
-                   (unsigned long long)pool_cap);
+    R.events_total += (int64_t)added;
+    if (trace) {
+      unsigned long long h[5];
+      C2_HIP(hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost));
+      std::fprintf(stderr,
+                   "[sweep] it %d changed %u rewritten %u kt %u decided %u moved %llu pool %llu/%llu | "
+                   "same-label %llu kept-prefix %llu touched %llu relabel %llu first-change %lld\n",
+                   it, nc, nd, nq, ns, (unsigned long long)added, (unsigned long long)pool_top,
+                   (unsigned long long)pool_cap, h[0], h[1], h[2], h[3], (long long)h[4]);
+    }
   }
   const int serr = read_scalar(ctx, d_err);
   if (serr & 2) fail(C2GPU_ERR_ARG, "c2gpu_bader: internal inconsistency in the sweep state");
@@ -712,6 +880,23 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     C2_HIP(hipMemcpy(a.data(), Kt, 8 * (size_t)N, hipMemcpyDeviceToHost));
     C2_HIP(hipMemcpy(b.data(), Kt_inc, 8 * (size_t)N, hipMemcpyDeviceToHost));
     if (a != b) fail(C2GPU_ERR_NOCONVERGE, "sweep verify: incremental known==2 times differ from a rebuild");
+  }
+  if (ctx->timing) {
+    // algorithmic bytes per kernel: each element a launch must read or write
+    // once (read-modify-write twice), from the host-side launch sizes and the
+    // device work counters
+    unsigned long long w[16];
+    C2_HIP(hipMemcpy(w, d_work, sizeof(w), hipMemcpyDeviceToHost));
+    const double dN = (double)N;
+    add_kernel_bytes(ctx, "k_patch_keep", 56.0 * s_nd + 9.0 * w[0] + 8.0 * w[1]);
+    add_kernel_bytes(ctx, "k_mark_decide", 6.0 * s_nq + 13.0 * w[3] + 9.0 * w[2]);
+    add_kernel_bytes(ctx, "k_ev_fill", 20.0 * (s_nc + dN * n_full) + 32.0 * w[4]);
+    add_kernel_bytes(ctx, "k_delta", 28.0 * s_nc + 5.0 * w[5] + 8.0 * w[6]);
+    add_kernel_bytes(ctx, "k_decide", 77.0 * (s_ns + dN * n_decall) + 12.0 * w[7]);
+    add_kernel_bytes(ctx, "k_kentry", 168.0 * (s_nq + dN * n_full));
+    add_kernel_bytes(ctx, "k_collect", n_coll * (double)NF + 5.0 * s_coll);
+    add_kernel_bytes(ctx, "k_ev_sort", 17.0 * (s_nd + dN * n_full) + 16.0 * w[8]);
+    add_kernel_bytes(ctx, "k_ev_count", n_full * 12.0 * dN + 12.0 * s_nev);
   }
   unsigned long long *d_ns = buf<unsigned long long>(ctx, "sweep_nstarts", 1);
   C2_HIP(hipMemsetAsync(d_ns, 0, 8, st));

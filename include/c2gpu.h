@@ -73,6 +73,12 @@ int c2gpu_memcpy(c2gpu_ctx *ctx, void *dst, const void *src, uint64_t bytes, int
 int c2gpu_ctx_timing(c2gpu_ctx *ctx, int enable);
 int c2gpu_ctx_kernel_time(c2gpu_ctx *ctx, const char *kernel, double *ms, int64_t *launches);
 int c2gpu_ctx_kernel_names(c2gpu_ctx *ctx, char *names_out, uint64_t len);
+/* Algorithmic bytes a kernel had to move while timing was enabled (each
+ * array element it must read or write, counted once; a read-modify-write
+ * counts twice), accumulated over its launches; 0 for kernels that do not
+ * report.  No counterpart in critic2: the measurement side of the timing
+ * table, used for the HBM roofline. */
+int c2gpu_ctx_kernel_bytes(c2gpu_ctx *ctx, const char *kernel, double *bytes);
 
 /* ------------------------------------------------------------------ */
 /* Bader (near-grid, Henkelman et al.)                                 */
