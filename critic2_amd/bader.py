@@ -75,7 +75,7 @@ def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noato
     else:
         rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
         lab = np.zeros(npts, dtype=np.int32)
-    cap = max(opts.nat, 1) * 4 + 64
+    cap = max(opts.nat, 1) * 4 + 4096  # non-nuclear maxima of the NNM/NOATOMS rules append
     xattr = np.zeros(3 * cap)
     nattr = _lib.C.c_int(0)
     st = _lib.BaderStats()
@@ -83,6 +83,8 @@ def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noato
                              _lib.C.byref(opts), flags, _lib.ptr(lab), _lib.C.byref(nattr),
                              xattr.ctypes.data, cap, _lib.C.byref(st))
     ctx.check(rc, "c2gpu_bader")
+    if nattr.value > cap:
+        raise RuntimeError(f"c2gpu_bader: {nattr.value} attractors exceed the {cap} attractor slots")
     bas = BasinData(n=n, atexist=(mode != _lib.ATTR_NOATOMS), ratom=ra, nattr=nattr.value,
                     xattr=xattr[:3 * nattr.value].reshape(-1, 3).copy(), idg=lab,
                     stats=st.as_dict())

@@ -22,6 +22,7 @@ struct SweepResult {
   int iters;
   int64_t events_total;
   int64_t nstarts;
+  const int32_t *ell;  // the converged story's committed path lengths (device)
 };
 SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64_t *toff, const uint32_t *tp,
                       const int32_t *maxlab, int32_t *V, int8_t *known);
@@ -87,6 +88,26 @@ __global__ void k_scatter_labels(uint32_t n, const uint32_t *__restrict__ idx, c
   if (q < n) out[idx[q]] = lab[q];
 }
 
+// Earliest start (sweep index) whose path ends at each point: over the free
+// trajectories (ell == null) or the committed paths of a story.  A path that
+// reaches a maximum ends there, so at a maximum this is the first start to
+// reach it: its discovery (bader@proc.f90:159-199).
+__global__ void k_first_reach(BParams P, uint32_t N, const uint64_t *__restrict__ toff,
+                              const uint32_t *__restrict__ tp, const int32_t *__restrict__ ell,
+                              uint32_t *__restrict__ first) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const int e = ell ? ell[s] : (int)(toff[s + 1] - toff[s]);
+  if (e <= 0) return;
+  atomicMin(&first[tp[toff[s] + e - 1]], (uint32_t)sweep_index(P, s));
+}
+
+__global__ void k_gather_u32(uint32_t n, const uint32_t *__restrict__ idx, const uint32_t *__restrict__ src,
+                             uint32_t *__restrict__ out) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = src[idx[q]];
+}
+
 // ---------------------------------------------------------------- step table
 
 __global__ void k_step_pointers(BParams P, uint32_t N, int64_t *ongrid, int64_t *near0, double *dr1) {
@@ -139,16 +160,20 @@ static BParams make_params(const double *d_rho, const int n[3], const double car
 // Attractor of a new maximum in the default mode (bader@proc.f90:159-199):
 // identify_atom with ratom = 1d40, the nearest atom of the cell list.
 int nearest_atom_label(const c2gpu_attractor_opts *o, const double x[3]);
+// Attractors of maxima taken in discovery order (bader@proc.f90:159-199).
+int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
+                   int32_t *lab, std::vector<double> &xattr);
 
 void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double car2lat[9],
                const double lat_i_dist[27], const c2gpu_attractor_opts *opts, int flags, int32_t *label_out,
                int *nattr, double *xattr_out, int xattr_cap, c2gpu_bader_stats *stats) {
   if (!opts) fail(C2GPU_ERR_ARG, "c2gpu_bader: opts is NULL");
-  if (opts->mode != C2GPU_ATTR_NEAREST_ATOM)
-    fail(C2GPU_ERR_UNSUPPORTED,
-         "c2gpu_bader: only the default attractor assignment (nearest atom, ratom=1d40) is "
-         "implemented on the GPU path");
-  if (opts->nat < 1 || !opts->xat) fail(C2GPU_ERR_ARG, "c2gpu_bader: no atoms");
+  if (opts->mode != C2GPU_ATTR_NEAREST_ATOM && opts->mode != C2GPU_ATTR_RATOM && opts->mode != C2GPU_ATTR_NOATOMS)
+    fail(C2GPU_ERR_ARG, "c2gpu_bader: unknown attractor mode");
+  const bool ordered = opts->mode != C2GPU_ATTR_NEAREST_ATOM;  // labels depend on discovery order
+  if (opts->mode != C2GPU_ATTR_NOATOMS && (opts->nat < 1 || !opts->xat))
+    fail(C2GPU_ERR_ARG, "c2gpu_bader: no atoms");
+  if (opts->nat > 0 && !opts->xat) fail(C2GPU_ERR_ARG, "c2gpu_bader: null atom positions");
   if (opts->nat >= (1 << 20)) fail(C2GPU_ERR_ARG, "c2gpu_bader: too many atoms");
   for (int c = 0; c < 3; c++)
     if (n[c] < 3) fail(C2GPU_ERR_ARG, "c2gpu_bader: grid edges must be >= 3");
@@ -212,28 +237,75 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   std::vector<uint32_t> h_ends(nends);
   C2_HIP(hipMemcpyAsync(h_ends.data(), d_ends, sizeof(uint32_t) * nends, hipMemcpyDeviceToHost, st));
   C2_HIP(hipStreamSynchronize(st));
-  std::vector<int32_t> h_lab(nends);
+  // fractional position of each maximum, dv = (p-1)/n (bader@proc.f90:161)
+  std::vector<double> h_dv(3 * (size_t)nends);
   for (uint32_t q = 0; q < nends; q++) {
     const uint32_t l = h_ends[q];
     const int i = (int)(l % (uint32_t)n[0]);
     const int j = (int)((l / (uint32_t)n[0]) % (uint32_t)n[1]);
     const int k = (int)(l / ((uint32_t)n[0] * (uint32_t)n[1]));
-    const double dv[3] = {(double)i / n[0], (double)j / n[1], (double)k / n[2]};
-    h_lab[q] = nearest_atom_label(opts, dv);
+    h_dv[3 * q] = (double)i / n[0];
+    h_dv[3 * q + 1] = (double)j / n[1];
+    h_dv[3 * q + 2] = (double)k / n[2];
+  }
+  std::vector<int32_t> h_lab(nends);
+  std::vector<double> h_xattr;
+  int nbasin = opts->nat;
+  uint32_t *d_first = nullptr, *d_key = nullptr;
+  // discovery order of the maxima from the earliest start whose path ends at
+  // each (free trajectories first; then the committed paths of the story)
+  auto discovery_labels = [&](const int32_t *ell) {
+    C2_HIP(hipMemsetAsync(d_first, 0xff, sizeof(uint32_t) * N, st));
+    C2_LAUNCH(ctx, k_first_reach, dim3(gN), dim3(B), 0, P, N, d_toff, d_tp, ell, d_first);
+    std::vector<uint32_t> key(nends), order(nends);
+    if (nends) {
+      C2_LAUNCH(ctx, k_gather_u32, dim3(grid_for(nends, B)), dim3(B), 0, nends, d_ends, d_first, d_key);
+      C2_HIP(hipMemcpyAsync(key.data(), d_key, sizeof(uint32_t) * nends, hipMemcpyDeviceToHost, st));
+    }
+    C2_HIP(hipStreamSynchronize(st));
+    for (uint32_t q = 0; q < nends; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    std::vector<int32_t> lab(nends);
+    nbasin = attractor_rule(opts, nends, order.data(), h_dv.data(), lab.data(), h_xattr);
+    if (nbasin >= (1 << 20)) fail(C2GPU_ERR_ARG, "c2gpu_bader: too many attractors");
+    return lab;
+  };
+  if (ordered) {
+    d_first = buf<uint32_t>(ctx, "bader_first", N);
+    d_key = buf<uint32_t>(ctx, "bader_firstkey", nends > 0 ? nends : 1);
+    h_lab = discovery_labels(nullptr);
+  } else {
+    for (uint32_t q = 0; q < nends; q++) h_lab[q] = nearest_atom_label(opts, &h_dv[3 * q]);
+    h_xattr.assign(opts->xat, opts->xat + 3 * (size_t)opts->nat);
   }
   S.nmaxima = (int)nends;
   int32_t *d_maxlab = buf<int32_t>(ctx, "bader_maxlab", N);
   int32_t *d_endlab = buf<int32_t>(ctx, "bader_endlab", nends > 0 ? nends : 1);
-  C2_HIP(hipMemcpyAsync(d_endlab, h_lab.data(), sizeof(int32_t) * nends, hipMemcpyHostToDevice, st));
-  if (nends)
-    C2_LAUNCH(ctx, k_scatter_labels, dim3(grid_for(nends, B)), dim3(B), 0, nends, d_ends, d_endlab, d_maxlab);
+  auto upload_labels = [&]() {
+    C2_HIP(hipMemcpyAsync(d_endlab, h_lab.data(), sizeof(int32_t) * nends, hipMemcpyHostToDevice, st));
+    if (nends)
+      C2_LAUNCH(ctx, k_scatter_labels, dim3(grid_for(nends, B)), dim3(B), 0, nends, d_ends, d_endlab, d_maxlab);
+  };
+  upload_labels();
   S.ms_traj = ttraj.stop();
 
-  // 2) first sweep (bader@proc.f90:151-215)
+  // 2) first sweep (bader@proc.f90:151-215).  With the NNM/RATOM/NOATOMS
+  // rules a maximum's label depends on the order in which the sweep reaches
+  // the maxima; labels and story are made consistent by iteration: the serial
+  // sweep is the unique story whose own discovery order reproduces the labels
+  // it was computed with.
   Timer tsweep(st);
   int32_t *V = buf<int32_t>(ctx, "bader_V", N), *V2 = buf<int32_t>(ctx, "bader_V2", N);
   int8_t *known = buf<int8_t>(ctx, "bader_known", N), *known2 = buf<int8_t>(ctx, "bader_known2", N);
-  const SweepResult SR = sweep_run(ctx, P, N, d_toff, d_tp, d_maxlab, V, known);
+  SweepResult SR = sweep_run(ctx, P, N, d_toff, d_tp, d_maxlab, V, known);
+  for (int outer = 0; ordered; outer++) {
+    std::vector<int32_t> lab = discovery_labels(SR.ell);
+    if (lab == h_lab) break;
+    if (outer >= 8) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_bader: attractor discovery order did not settle");
+    h_lab.swap(lab);
+    upload_labels();
+    SR = sweep_run(ctx, P, N, d_toff, d_tp, d_maxlab, V, known);
+  }
   S.sweep_iters = SR.iters;
   S.events_total = SR.events_total;
   S.nstarts = SR.nstarts;
@@ -241,7 +313,6 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
 
   // 3) refinement (bader@proc.f90:217-224)
   Timer tref(st);
-  const int nbasin = opts->nat;
   const RefineResult RR = refine_run(ctx, P, N, nbasin, V, V2, known, known2, scratch, nth, MAX_PATH);
   S.refine_iters = RR.iters;
   S.edge_points = RR.edge_points;
@@ -256,7 +327,7 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   if (nattr) *nattr = nbasin;
   if (xattr_out)
     for (int a = 0; a < nbasin && a < xattr_cap; a++)
-      for (int c = 0; c < 3; c++) xattr_out[3 * a + c] = opts->xat[3 * a + c];
+      for (int c = 0; c < 3; c++) xattr_out[3 * a + c] = h_xattr[3 * (size_t)a + c];
   S.ms_total = ttot.stop();
   if (stats) *stats = S;
 }

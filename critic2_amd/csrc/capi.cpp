@@ -129,6 +129,66 @@ int nearest_atom_label(const c2gpu_attractor_opts *o, const double x[3]) {
   return best;
 }
 
+// identify_atom(x, icrd_crys, distmax) (crystalmod@env.f90:593-614 through
+// nearest_atom, :516-563): the nearest atom, if within distmax (atoms up to
+// distmax inclusive when distmax < 1 bohr, else the nearest unless farther
+// than distmax); 0 if none.
+static int identify_atom_within(const c2gpu_attractor_opts *o, const double x[3], double distmax) {
+  int best = 0;
+  double bd = DBL_MAX;
+  for (int a = 0; a < o->nat; a++) {
+    const double xd[3] = {x[0] - o->xat[3 * a], x[1] - o->xat[3 * a + 1], x[2] - o->xat[3 * a + 2]};
+    const double d = lattice_dist(o->x2c, xd);
+    if (d < bd) {
+      bd = d;
+      best = a + 1;
+    }
+  }
+  if (best == 0) return 0;
+  if (distmax < 1.0) return bd <= distmax ? best : 0;
+  return bd > distmax ? 0 : best;
+}
+
+// Attractor of each new maximum, taken in discovery order
+// (bader@proc.f90:159-199): an atom within ratom (if atoms exist), else a
+// known attractor within ratom (are_lclose, crystalmod@proc.f90:1118-1133),
+// else a new attractor numbered nbasin+1 at the maximum.  dv: fractional
+// positions of the maxima; order: their discovery order.  Fills lab (per
+// maximum) and xattr (3 per attractor); returns the attractor count.
+int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
+                   int32_t *lab, std::vector<double> &xattr) {
+  const bool atexist = o->mode != C2GPU_ATTR_NOATOMS;
+  const double ratom = o->mode == C2GPU_ATTR_NEAREST_ATOM ? 1e40 : o->ratom;
+  xattr.clear();
+  int nbasin = 0;
+  if (atexist) {
+    nbasin = o->nat;
+    xattr.assign(o->xat, o->xat + 3 * (size_t)o->nat);
+  }
+  for (uint32_t q = 0; q < nmax; q++) {
+    const uint32_t m = order[q];
+    const double *x = dv + 3 * (size_t)m;
+    int pv = 0;
+    if (atexist) pv = identify_atom_within(o, x, ratom);
+    if (pv == 0 && ratom > 1e-40) {  // vsmall (param.F90)
+      for (int l = 0; l < nbasin; l++) {
+        const double xd[3] = {x[0] - xattr[3 * l], x[1] - xattr[3 * l + 1], x[2] - xattr[3 * l + 2]};
+        if (lattice_dist(o->x2c, xd) < ratom) {
+          pv = l + 1;
+          break;
+        }
+      }
+    }
+    if (pv == 0) {
+      nbasin++;
+      xattr.insert(xattr.end(), x, x + 3);
+      pv = nbasin;
+    }
+    lab[m] = pv;
+  }
+  return nbasin;
+}
+
 }  // namespace c2
 
 using namespace c2;

@@ -702,6 +702,7 @@ struct SweepResult {
   int iters;
   int64_t events_total;
   int64_t nstarts;
+  const int32_t *ell;  // the converged story's committed path lengths (device)
 };
 
 // Runs the fixpoint; on return V holds volnum after the sweep and known the
@@ -715,7 +716,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   const unsigned gCollect = gW < 2048 ? gW : 2048;
   const bool full_every_step = std::getenv("C2GPU_SWEEP_FULL") != nullptr;
   const bool trace = std::getenv("C2GPU_SWEEP_TRACE") != nullptr;
-  SweepResult R{0, 0, 0};
+  SweepResult R{0, 0, 0, nullptr};
   const size_t NF = ((size_t)N + 16) & ~(size_t)15;  // byte flags padded to whole 16-byte groups
   int32_t *ell = buf<int32_t>(ctx, "sweep_ell", N), *lam = buf<int32_t>(ctx, "sweep_lam", N);
   int32_t *ell2 = buf<int32_t>(ctx, "sweep_ell2", N), *lam2 = buf<int32_t>(ctx, "sweep_lam2", N);
@@ -903,6 +904,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   C2_LAUNCH(ctx, k_final_state, dim3(gN), dim3(B), 0, N, L, Kt, V, known, d_ns, ell);
   R.nstarts = (int64_t)read_scalar(ctx, d_ns);
   R.iters = it;
+  R.ell = ell;
   return R;
 }
 

@@ -134,7 +134,14 @@ typedef struct {
  * before int_reorder_gridout); xattr_out (host, 3*xattr_cap) receives
  * bas%xattr in fractional coordinates; *nattr receives bas%nattr.
  * car2lat and lat_i_dist are taken as given so that the caller fixes their
- * bits (critic2 computes them with LAPACK).  stats may be NULL. */
+ * bits (critic2 computes them with LAPACK).  stats may be NULL.
+ * Attractors of new maxima follow bader@proc.f90:159-199 in every mode: the
+ * nearest atom (default), or an atom within ratom, else an earlier attractor
+ * within ratom, else a new one (NNM/RATOM: atoms first; NOATOMS: none), the
+ * new ones numbered in the serial sweep's discovery order.  If *nattr exceeds
+ * xattr_cap only the first xattr_cap positions are written.  The DISCARD
+ * expression filter has no counterpart here (the caller keeps critic2's own
+ * path for it). */
 int c2gpu_bader(c2gpu_ctx *ctx, const double *rho, const int n[3],
                 const double car2lat[9], const double lat_i_dist[27],
                 const c2gpu_attractor_opts *opts, int flags, int32_t *label_out,

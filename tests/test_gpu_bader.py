@@ -43,6 +43,29 @@ def test_bader_labels_bit_exact(ctx, name):
     np.testing.assert_array_equal(bas.xattr, xattr)
 
 
+@pytest.mark.parametrize("rule,ratom", [("nnm", 0.1), ("nnm", 1.0), ("noatoms", 1.0), ("noatoms", 0.05),
+                                        ("noatoms", 100.0)])
+def test_bader_attractor_rules_follow_discovery_order(ctx, rule, ratom):
+    """NNM / RATOM r / NOATOMS (bader@proc.f90:159-199): a new maximum is an
+    atom within ratom, else a known attractor within ratom, else a new
+    attractor numbered in the order the serial sweep discovers it.  Labels,
+    the attractor count and their positions must equal the oracle's."""
+    cell = random_cell(13, (8.0, 9.0, 10.0), (85, 95, 100), 5, (1, 6, 7, 8), (32, 36, 40))
+    print(f"[case] attractor rule {rule} ratom={ratom} n={cell.n}", flush=True)
+    f = _grid(ctx, cell)
+    omode = _lib.ATTR_RATOM if rule == "nnm" else _lib.ATTR_NOATOMS
+    ref, info, xattr, _ = oracle.bader(f, cell.n, cell.x2c, cell.xat, mode=omode, ratom=ratom)
+    kw = dict(nnm=True, ratom=ratom) if rule == "nnm" else dict(noatoms=True, ratom=ratom)
+    bas = gb.bader_integrate(ctx, f, cell.n, cell.x2c, cell.xat, **kw)
+    assert int(np.count_nonzero(bas.idg != ref)) == 0
+    assert bas.nattr == info["nattr"]
+    np.testing.assert_array_equal(bas.xattr, xattr)
+    if rule == "noatoms" and ratom > 50:
+        assert bas.nattr == 1
+    if rule == "nnm" and ratom < 0.2:
+        assert bas.nattr > len(cell.zat)  # maxima off the nuclei became non-nuclear attractors
+
+
 @pytest.mark.parametrize("mode", ["C2GPU_SWEEP_VERIFY", "C2GPU_SWEEP_FULL"])
 def test_sweep_incremental_equals_full_rebuild(ctx, mode, monkeypatch):
     """The incremental fixpoint steps against full rebuilds: VERIFY rebuilds
@@ -123,9 +146,6 @@ def test_bad_arguments_return_status(ctx):
     with pytest.raises(_lib.C2GpuError) as e:
         gb.bader_integrate(ctx, f, (2, 2, 2), cell.x2c, cell.xat)
     assert e.value.code == 1
-    with pytest.raises(_lib.C2GpuError) as e:
-        gb.bader_integrate(ctx, np.zeros(27), (3, 3, 3), cell.x2c, cell.xat, nnm=True)
-    assert e.value.code == 4
 
 
 def test_device_flag_rejects_host_pointers(ctx):
