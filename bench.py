@@ -40,15 +40,9 @@ def kernel_bytes(name, N, ev_per_launch, traj_points, nfields=1):
     the kernel must read or write, counted once (atomics count read+write)."""
     E = ev_per_launch
     table = {
-        "k_kentry": 17 * N + 8 * E,         # eoff, uni, Kt; each event once
-        "k_ev_count": 12 * N + 12 * E,      # ell, toff; path points, count RMW
-        "k_ev_fill": 24 * N + 20 * E,       # ell, lam, toff, eoff; point, cursor RMW, event
-        "k_ev_sort": 8 * N + 16 * E,        # eoff; events read and written
-        "k_ev_uniform": 9 * N + 8 * E,
-        "k_decide": 40 * N + 12 * E,        # story in/out, toff, Kt and points of the paths
-        "k_traj_len": 12 * N,               # grid once, lengths
-        "k_traj_fill": 16 * N + 4 * traj_points,
-        "k_promol_f": 8 * N,                # FP32 crho + cgrad
+        "k_traj": 12 * N + 4 * traj_points,          # grid once, position+length, the path points
+        "k_traj_reorder": 16 * N + 8 * traj_points,  # offsets; path points read and written
+        "k_promol_f": 8 * N,                         # FP32 crho + cgrad
         "k_promol_d": 16 * N,
         "k_basin_partials": (4 + 8 * nfields) * N,
         "k_reassign": 8 * N,
@@ -122,6 +116,77 @@ def cpu_baseline(ctx, cell, tables, species):
                       f"512^3 grid ({t_nci * 1e6:.2f} us/pt), 1 thread"}
 
 
+def run_batch(ctx, args):
+    """The batching workload: independent periodic dimer cells with seeded
+    geometry and grid edges drawn from {48,64,96,128,160}, FP64 Slater grids
+    resident in HBM; one step = exact Bader labels + Volume/Pop sums + FP32
+    promolecular NCI of every cell through c2gpu_batch_run."""
+    from critic2_amd import batch as gbt
+    from critic2_amd.synth import batch_cells
+    cells = batch_cells(args.batch)
+    zs = (1, 6, 7, 8)
+    tables = [slater_grid1(z) for z in zs]
+    species = [np.array([zs.index(int(z)) for z in c.zat], dtype=np.int32) for c in cells]
+    grids = []
+    for c in cells:
+        g = ctx.empty((c.npts,), np.float64)
+        gb.synth_slater(ctx, c, device_out=g)
+        grids.append(g)
+    points = sum(c.npts for c in cells)
+    print(f"[bench] batch of {len(cells)} cells, {points} points resident", file=sys.stderr, flush=True)
+
+    def step(cs, gs, sp, chunk=128):
+        # the cells go through c2gpu_batch_run in chunks, with a progress line each
+        agg = {"nstreams": 0, "queue_workgroups": 0, "ms_nci": 0.0}
+        for c0 in range(0, len(cs), chunk):
+            res, st = gbt.batch_run(ctx, cs[c0:c0 + chunk], gs[c0:c0 + chunk], tables, sp[c0:c0 + chunk],
+                                    nstreams=args.nstreams)
+            for r in res:
+                for key in ("labels", "crho", "cgrad"):
+                    if key in r:
+                        r[key].free()
+            agg["nstreams"] = st["nstreams"]
+            agg["queue_workgroups"] = st["queue_workgroups"]
+            agg["ms_nci"] += st["ms_nci"]
+            print(f"[bench]   cells {c0 + len(res)}/{len(cs)}", file=sys.stderr, flush=True)
+        return agg
+
+    for w in range(args.warmup):
+        step(cells[:8], grids[:8], species[:8])
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        st = step(cells, grids, species)
+        print(f"[bench] step {s_ + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    out = {
+        "metric": f"Gpoints/s Bader+NCI, batch of {len(cells)} cells (labels exact)",
+        "value": round(points / dt / 1e9, 6),
+        "unit": "Gpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(cells)} seeded dimer cells (6-40 atoms, gap 2.5-4.0 A), edges from "
+                               "{48,64,96,128,160}; Bader labels + Volume/Pop + FP32 promolecular NCI",
+                   "cells": len(cells), "points": points, "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": None,
+        "batch": {"nstreams": st["nstreams"], "queue_workgroups": st["queue_workgroups"],
+                  "ms_nci_queue": round(st["ms_nci"], 3)},
+    }
+    print(json.dumps(out), flush=True)
+    for g in grids:
+        g.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +195,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--n", type=int, default=512, help="grid edge (default: the 512^3 workload)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead of the 512^3 step, time the batched path on this many dimer cells "
+                         "(512 = the batching workload: edges from {48,64,96,128,160}, 6-40 atoms)")
+    ap.add_argument("--nstreams", type=int, default=0, help="batched path: concurrent Bader workers (0: 8)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a streaming kernel of known traffic (k_sqrt of the FFT |grad| grid: "
                          "8 B read + 8 B written per point, 256^3) to scale rocprof FETCH_SIZE/WRITE_SIZE")
@@ -138,6 +207,9 @@ def main():
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     ctx = _lib.Context(0)
+    if args.batch:
+        run_batch(ctx, args)
+        return
     if args.copy_probe:
         probe = config(2)
         f = gb.synth_slater(ctx, probe)

@@ -21,6 +21,9 @@ void relabel_run(c2gpu_ctx *, int64_t, const int32_t *, int, int, int32_t *);
 void fft_deriv_run(c2gpu_ctx *, const double *, const int *, const double *, int, int, double *);
 void nci_run(c2gpu_ctx *, const double *, const int *, const double *, const double *, int, int,
              const c2gpu_promol *, int, int, void *, void *, int);
+void batch_run(c2gpu_ctx *, int, const c2gpu_cell *, int, double, int, const c2gpu_radial_table *, int, int,
+               c2gpu_batch_stats *);
+void destroy_workers(c2gpu_ctx *);
 void synth_slater_run(c2gpu_ctx *, const double *, const int *, int, const double *, const int *,
                       double, int, double *);
 
@@ -215,6 +218,7 @@ int c2gpu_ctx_destroy(c2gpu_ctx *ctx) {
   if (!ctx) return C2GPU_ERR_ARG;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  destroy_workers(ctx);
   release_all(ctx);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -423,6 +427,14 @@ int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const doubl
                   float *cgrad_out) {
   return guarded(ctx, [&] {
     nci_checked(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out, 1);
+  });
+}
+
+int c2gpu_batch_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int attr_mode, double ratom,
+                    int nspecies, const c2gpu_radial_table *tables, int nstreams, int flags,
+                    c2gpu_batch_stats *stats) {
+  return guarded(ctx, [&] {
+    batch_run(ctx, ncell, cells, attr_mode, ratom, nspecies, tables, nstreams, flags, stats);
   });
 }
 

@@ -37,6 +37,10 @@ struct c2gpu_ctx {
     double bytes = 0;  // algorithmic bytes reported by the launching code
   };
   std::map<std::string, KTime> ktime;
+  // batches: worker contexts (one stream each) and the tuned workgroups per
+  // CU of the work-queue NCI kernel (0 = not tuned yet)
+  std::vector<c2gpu_ctx *> subs;
+  int queue_wg_per_cu = 0;
 };
 
 namespace c2 {

@@ -220,23 +220,18 @@ __global__ void __launch_bounds__(256) k_promol_d(Geo G, PromolD T, const double
 }
 
 // FP32 promolecular NCI: same sums with per-interval cubic tables.
-__global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const float4 *__restrict__ imgf,
-                                                  const float *__restrict__ cut2,
-                                                  const uint64_t *__restrict__ off,
-                                                  const uint32_t *__restrict__ lst, NciOut O) {
-  const int b = blockIdx.x;
-  const int b1 = b % G.nb1, b2 = (b / G.nb1) % G.nb2, b3 = b / (G.nb1 * G.nb2);
-  const int t = threadIdx.x;
-  const int i = b1 * BX + (t % BX), j = b2 * BY + ((t / BX) % BY), k = b3 * BZ + t / (BX * BY);
-  if (i >= G.n1 || j >= G.n2 || k >= G.n3) return;
+// FP32 promolecular NCI at node (i,j,k) from the image list lst[qb..qe)
+// (the atom sum of crystalmod@env.f90:622-753 and nci@proc.f90:565-599).
+__device__ __forceinline__ void promol_f_node(const Geo &G, const PromolF &T, const float4 *__restrict__ imgf,
+                                              const float *__restrict__ cut2, const uint32_t *__restrict__ lst,
+                                              uint64_t qb, uint64_t qe, int i, int j, int k, const NciOut &O) {
   double xcd[3];
   node_cart(G, i, j, k, xcd);
   const float xc0 = (float)xcd[0], xc1 = (float)xcd[1], xc2 = (float)xcd[2];
   float f = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
   float h00 = 0.f, h11 = 0.f, h22 = 0.f, h01 = 0.f, h02 = 0.f, h12 = 0.f;
   bool isnuc = false;
-  const uint64_t qe = off[b + 1];
-  for (uint64_t q = off[b]; q < qe; q++) {
+  for (uint64_t q = qb; q < qe; q++) {
     const float4 p = imgf[lst[q]];
     const int s = (int)p.w;
     const float x0 = xc0 - p.x, x1 = xc1 - p.y, x2 = xc2 - p.z;
@@ -289,6 +284,18 @@ __global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const float4
   const float a = fabsf(f) * 100.f;
   const float crho = (l2 < 0.f || (l2 == 0.f && signbit(l2))) ? -a : a;
   store_out<float>(O, out_index(G, O.zfast, i, j, k), crho, s);
+}
+
+__global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const float4 *__restrict__ imgf,
+                                                  const float *__restrict__ cut2,
+                                                  const uint64_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ lst, NciOut O) {
+  const int b = blockIdx.x;
+  const int b1 = b % G.nb1, b2 = (b / G.nb1) % G.nb2, b3 = b / (G.nb1 * G.nb2);
+  const int t = threadIdx.x;
+  const int i = b1 * BX + (t % BX), j = b2 * BY + ((t / BX) % BY), k = b3 * BZ + t / (BX * BY);
+  if (i >= G.n1 || j >= G.n2 || k >= G.n3) return;
+  promol_f_node(G, T, imgf, cut2, lst, off[b], off[b + 1], i, j, k, O);
 }
 
 // ---------------------------------------------------------------- FFT scheme
@@ -727,6 +734,239 @@ void fft_deriv_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const doub
     C2_LAUNCH(ctx, k_real_part, dim3(g), dim3(256), 0, w, d_out, N);
   }
   if (!(flags & C2GPU_PTR_DEVICE)) C2_HIP(hipMemcpyAsync(out, d_out, 8 * N, hipMemcpyDeviceToHost, st));
+  C2_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------- batches of cells
+
+// A cell of a batch on the device: its geometry, its first brick in the
+// batch's brick numbering, its images in the batch image array, and where its
+// NCI outputs go.
+struct BatchCell {
+  Geo G;
+  uint32_t brick0;
+  uint32_t img0, nimg;
+  double brad;
+  NciOut O;
+};
+
+__device__ __forceinline__ int cell_of_brick(const BatchCell *__restrict__ cells, int ncell, uint32_t gb) {
+  int lo = 0, hi = ncell - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cells[mid].brick0 <= gb)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// k_brick_list over the bricks of every cell of a batch (one wavefront per
+// brick); list entries are batch image indices.
+__global__ void __launch_bounds__(64) k_brick_list_batch(const BatchCell *__restrict__ cells, int ncell,
+                                                         const double4 *__restrict__ img,
+                                                         const double *__restrict__ rcut,
+                                                         uint32_t *__restrict__ cnt, const uint64_t *__restrict__ off,
+                                                         uint32_t *__restrict__ out) {
+  const uint32_t gb = blockIdx.x;
+  const BatchCell &C = cells[cell_of_brick(cells, ncell, gb)];
+  const Images I{img + C.img0, rcut, (int)C.nimg, C.brad};
+  const int b = (int)(gb - C.brick0);
+  const int lane = threadIdx.x;
+  uint32_t total = 0;
+  for (int m0 = 0; m0 < I.nimg; m0 += 64) {
+    const int m = m0 + lane;
+    const bool hit = (m < I.nimg) && image_hits_brick(C.G, I, b, m);
+    const uint64_t mask = __ballot(hit);
+    if (out && hit)
+      out[off[gb] + total + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = C.img0 + (uint32_t)m;
+    total += (uint32_t)__popcll(mask);
+  }
+  if (!out && lane == 0) cnt[gb] = total;
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// Persistent work-queue kernel over the bricks of a batch: each workgroup
+// takes the next brick until the queue is drained; the last take of every
+// workgroup returns a brick number past the end, so all of them exit.
+__global__ void __launch_bounds__(256) k_promol_f_queue(const BatchCell *__restrict__ cells, int ncell,
+                                                        uint32_t nbrick, PromolF T, const float4 *__restrict__ imgf,
+                                                        const float *__restrict__ cut2,
+                                                        const uint64_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ lst, uint32_t *queue) {
+  __shared__ uint32_t next;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous brick number
+    if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t gb = next;
+    if (gb >= nbrick) return;
+    const BatchCell &C = cells[cell_of_brick(cells, ncell, gb)];
+    const int b = (int)(gb - C.brick0);
+    const int b1 = b % C.G.nb1, b2 = (b / C.G.nb1) % C.G.nb2, b3 = b / (C.G.nb1 * C.G.nb2);
+    const int t = threadIdx.x;
+    const int i = b1 * BX + (t % BX), j = b2 * BY + ((t / BX) % BY), k = b3 * BZ + t / (BX * BY);
+    if (i < C.G.n1 && j < C.G.n2 && k < C.G.n3)
+      promol_f_node(C.G, T, imgf, cut2, lst, off[gb], off[gb + 1], i, j, k, C.O);
+  }
+}
+
+// FP32 promolecular NCI of every cell of a batch that asks for it (crho_out
+// and cgrad_out set): one graph launch of the work-queue kernel.  The grid of
+// that kernel (workgroups per CU) is tuned once per context on the first
+// batch by timing the candidates.
+void batch_nci_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int nspecies,
+                   const c2gpu_radial_table *tables, int flags, double *ms_out, int *wg_out) {
+  hipStream_t st = ctx->stream;
+  const bool dev = flags & C2GPU_PTR_DEVICE;
+  std::vector<int> sel;
+  for (int c = 0; c < ncell; c++)
+    if (cells[c].crho_out && cells[c].cgrad_out) sel.push_back(c);
+  if (ms_out) *ms_out = 0.0;
+  if (sel.empty()) return;
+  if (nspecies < 1 || !tables) fail(C2GPU_ERR_ARG, "c2gpu_batch_run: NCI asked for but no radial tables");
+  const c2gpu_cell &c0 = cells[sel[0]];
+  if (c0.nat < 1 || !c0.xat || !c0.species) fail(C2GPU_ERR_ARG, "c2gpu_batch_run: NCI cell without atoms");
+  const c2gpu_promol pm0{nspecies, tables, c0.nat, c0.xat, c0.species};
+  const PromolDev P = upload_promol(ctx, &pm0, c0.x2c);  // the tables (shared by all cells)
+  double rcmax = 0;
+  for (int s = 0; s < nspecies; s++) {
+    const double rc = tables[s].cutrad < tables[s].rmax ? tables[s].cutrad : tables[s].rmax;
+    rcmax = rc > rcmax ? rc : rcmax;
+  }
+  std::vector<BatchCell> hc;
+  std::vector<double4> himg;
+  std::vector<float4> himgf;
+  uint64_t nbrick = 0;
+  size_t nstage = 0;
+  std::vector<size_t> stage_at;
+  for (int c : sel) {
+    const c2gpu_cell &C = cells[c];
+    for (int q = 0; q < 3; q++)
+      if (C.n[q] < 1) fail(C2GPU_ERR_ARG, "c2gpu_batch_run: bad grid");
+    if (C.nat < 1 || !C.xat || !C.species) fail(C2GPU_ERR_ARG, "c2gpu_batch_run: NCI cell without atoms");
+    for (int a = 0; a < C.nat; a++)
+      if (C.species[a] < 0 || C.species[a] >= nspecies) fail(C2GPU_ERR_ARG, "c2gpu_batch_run: bad species");
+    BatchCell B;
+    B.G = make_geo(C.n, C.x2c);
+    std::vector<double> pos;
+    std::vector<int> isp, iat;
+    expand_images(C.x2c, C.nat, C.xat, C.species, rcmax, pos, isp, iat);
+    B.brick0 = (uint32_t)nbrick;
+    B.img0 = (uint32_t)himg.size();
+    B.nimg = (uint32_t)isp.size();
+    B.brad = brick_radius(B.G);
+    for (size_t m = 0; m < isp.size(); m++) {
+      himg.push_back(make_double4(pos[3 * m], pos[3 * m + 1], pos[3 * m + 2], (double)isp[m]));
+      himgf.push_back(make_float4((float)pos[3 * m], (float)pos[3 * m + 1], (float)pos[3 * m + 2], (float)isp[m]));
+    }
+    B.O.is_f32 = 1;
+    B.O.zfast = 1;
+    const size_t npt = (size_t)C.n[0] * C.n[1] * C.n[2];
+    stage_at.push_back(nstage);
+    if (dev) {
+      B.O.crho = C.crho_out;
+      B.O.cgrad = C.cgrad_out;
+    } else {
+      nstage += npt;
+    }
+    nbrick += (uint64_t)B.G.nb1 * B.G.nb2 * B.G.nb3;
+    if (nbrick >= (1ull << 31)) fail(C2GPU_ERR_ARG, "c2gpu_batch_run: too many node bricks");
+    hc.push_back(B);
+  }
+  float *stage = nullptr;
+  if (!dev) {
+    stage = buf<float>(ctx, "bq_stage", 2 * (nstage > 0 ? nstage : 1));
+    for (size_t q = 0; q < hc.size(); q++) {
+      hc[q].O.crho = stage + stage_at[q];
+      hc[q].O.cgrad = stage + nstage + stage_at[q];
+    }
+  }
+  const int ncs = (int)hc.size();
+  BatchCell *d_cells = buf<BatchCell>(ctx, "bq_cells", hc.size());
+  double4 *d_img = buf<double4>(ctx, "bq_img", himg.size() > 0 ? himg.size() : 1);
+  float4 *d_imgf = buf<float4>(ctx, "bq_imgf", himgf.size() > 0 ? himgf.size() : 1);
+  C2_HIP(hipMemcpyAsync(d_cells, hc.data(), sizeof(BatchCell) * hc.size(), hipMemcpyHostToDevice, st));
+  C2_HIP(hipMemcpyAsync(d_img, himg.data(), sizeof(double4) * himg.size(), hipMemcpyHostToDevice, st));
+  C2_HIP(hipMemcpyAsync(d_imgf, himgf.data(), sizeof(float4) * himgf.size(), hipMemcpyHostToDevice, st));
+  uint32_t *cnt = buf<uint32_t>(ctx, "bq_bcnt", nbrick + 1);
+  uint64_t *off = buf<uint64_t>(ctx, "bq_boff", nbrick + 1);
+  C2_HIP(hipMemsetAsync(cnt + nbrick, 0, sizeof(uint32_t), st));
+  C2_LAUNCH(ctx, k_brick_list_batch, dim3((unsigned)nbrick), dim3(64), 0, d_cells, ncs, d_img, P.rcut, cnt,
+            (const uint64_t *)nullptr, (uint32_t *)nullptr);
+  size_t tmp = 0;
+  C2_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, off, (int)nbrick + 1, st));
+  void *t = buffer(ctx, "bq_bscan", tmp);
+  C2_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, cnt, off, (int)nbrick + 1, st));
+  uint64_t total = 0;
+  C2_HIP(hipMemcpyAsync(&total, off + nbrick, 8, hipMemcpyDeviceToHost, st));
+  C2_HIP(hipStreamSynchronize(st));
+  uint32_t *lst = buf<uint32_t>(ctx, "bq_blst", total > 0 ? total : 1);
+  C2_LAUNCH(ctx, k_brick_list_batch, dim3((unsigned)nbrick), dim3(64), 0, d_cells, ncs, d_img, P.rcut, cnt, off,
+            lst);
+  uint32_t *queue = buf<uint32_t>(ctx, "bq_queue", 1);
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  // one graph launch with `wpc` workgroups per CU; returns its device time
+  auto run = [&](int wpc) -> double {
+    const unsigned nwg = (unsigned)(ncu * wpc);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    C2_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, queue, 0u);  // no memset nodes in graphs
+    hipLaunchKernelGGL(k_promol_f_queue, dim3(nwg), dim3(256), 0, st, d_cells, ncs, (uint32_t)nbrick, P.F,
+                       (const float4 *)d_imgf, (const float *)P.cut2f, (const uint64_t *)off,
+                       (const uint32_t *)lst, queue);
+    C2_HIP(hipStreamEndCapture(st, &g));
+    C2_HIP(hipGetLastError());
+    C2_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipEvent_t a = event_get(ctx), b = event_get(ctx);
+    C2_HIP(hipEventRecord(a, st));
+    C2_HIP(hipGraphLaunch(ge, st));
+    C2_HIP(hipEventRecord(b, st));
+    C2_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    C2_HIP(hipEventElapsedTime(&ms, a, b));
+    ctx->evpool.push_back(a);
+    ctx->evpool.push_back(b);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    return ms;
+  };
+  if (ctx->queue_wg_per_cu == 0) {
+    // tune the persistent grid: the candidates write identical results
+    double best = 1e300;
+    for (int wpc : {2, 4, 8, 16}) {
+      const double ms = run(wpc);
+      if (ms < best) {
+        best = ms;
+        ctx->queue_wg_per_cu = wpc;
+      }
+    }
+  }
+  const double ms = run(ctx->queue_wg_per_cu);
+  if (ctx->timing) {
+    auto &k = ctx->ktime["k_promol_f_queue"];
+    k.ms += ms;
+    k.launches += 1;
+    double pts = 0;
+    for (const BatchCell &B : hc) pts += (double)B.G.n1 * B.G.n2 * B.G.n3;
+    k.bytes += 8.0 * pts;  // FP32 crho + cgrad per node
+  }
+  if (ms_out) *ms_out = ms;
+  if (wg_out) *wg_out = ncu * ctx->queue_wg_per_cu;
+  if (!dev) {
+    for (size_t q = 0; q < hc.size(); q++) {
+      const c2gpu_cell &C = cells[sel[q]];
+      const size_t npt = (size_t)C.n[0] * C.n[1] * C.n[2];
+      C2_HIP(hipMemcpyAsync(C.crho_out, stage + stage_at[q], 4 * npt, hipMemcpyDeviceToHost, st));
+      C2_HIP(hipMemcpyAsync(C.cgrad_out, stage + nstage + stage_at[q], 4 * npt, hipMemcpyDeviceToHost, st));
+    }
+  }
   C2_HIP(hipStreamSynchronize(st));
 }
 

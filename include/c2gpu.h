@@ -230,6 +230,58 @@ int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const doubl
                   int flags, float *crho_out, float *cgrad_out);
 
 /* ------------------------------------------------------------------ */
+/* Batches of cells                                                    */
+/* ------------------------------------------------------------------ */
+
+/* One periodic cell of a batch.  Inputs: the FP64 grid (x fastest), the
+ * cell, the grid metrics as c2gpu_bader_metrics returns them, and the atoms
+ * (fractional coordinates; species = radial-table index for NCI).  Outputs
+ * (NULL skips that part for this cell): Bader labels (N), the attractor
+ * count and positions, per-basin point counts and sums of rho (xattr_cap
+ * entries each, host memory), and the promolecular NCI signed density x100
+ * and reduced gradient in FP32, z-fastest as nci@proc.f90:502-505. */
+typedef struct {
+  const double *rho;
+  int n[3];
+  double x2c[9];
+  double car2lat[9];
+  double lat_i_dist[27];
+  int nat;
+  const double *xat;
+  const int *species;
+  int32_t *label_out;
+  int *nattr_out;
+  double *xattr_out;
+  int xattr_cap;
+  int64_t *counts_out;
+  double *sums_out;
+  float *crho_out, *cgrad_out;
+} c2gpu_cell;
+
+typedef struct {
+  double ms_total;      /* wall time of the call                          */
+  double ms_nci;        /* device time of the NCI work-queue launch        */
+  int64_t points;       /* grid points of all cells                       */
+  int nstreams;         /* concurrent Bader workers used                   */
+  int queue_workgroups; /* workgroups of the NCI work-queue kernel         */
+} c2gpu_batch_stats;
+
+/* Bader (bader_integrate + the Volume/Pop sums of intgrid_fields) and the
+ * promolecular NCI block over many independent cells in one call; no
+ * counterpart in critic2, which runs them one system at a time.  Results per
+ * cell equal c2gpu_bader / c2gpu_basin_sums / c2gpu_nci_f32 (FP32 promol) on
+ * that cell alone.  NCI: every cell's 8x8x4 node bricks go through one
+ * persistent work-queue kernel launched as a HIP graph.  Bader: cells are
+ * taken from a queue by nstreams workers (0: 8), each on its own stream.
+ * attr_mode/ratom as in c2gpu_attractor_opts.  tables: the radial tables
+ * (shared by all cells; may be NULL when no cell asks for NCI).  With
+ * C2GPU_PTR_DEVICE, rho, label_out, crho_out and cgrad_out are device
+ * memory; the other outputs are always host memory.  stats may be NULL. */
+int c2gpu_batch_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int attr_mode, double ratom,
+                    int nspecies, const c2gpu_radial_table *tables, int nstreams, int flags,
+                    c2gpu_batch_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic input (no counterpart in critic2: test and benchmark input) */
 /* ------------------------------------------------------------------ */
 
