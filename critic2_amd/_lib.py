@@ -18,7 +18,7 @@ OK = 0
 ERR_NAMES = {
     1: "C2GPU_ERR_ARG", 2: "C2GPU_ERR_HIP", 3: "C2GPU_ERR_ALLOC", 4: "C2GPU_ERR_UNSUPPORTED",
     5: "C2GPU_ERR_NOCONVERGE", 6: "C2GPU_ERR_NEWMAX", 7: "C2GPU_ERR_PATHLEN",
-    8: "C2GPU_ERR_DEGENERATE",
+    8: "C2GPU_ERR_DEGENERATE", 9: "C2GPU_ERR_FORMAT",
 }
 PTR_DEVICE = 1
 ATTR_NEAREST_ATOM, ATTR_RATOM, ATTR_NOATOMS = 0, 1, 2
@@ -32,7 +32,8 @@ EXPORTS = (
     "c2gpu_ctx_kernel_time", "c2gpu_ctx_kernel_names", "c2gpu_ctx_kernel_bytes",
     "c2gpu_bader_metrics", "c2gpu_bader_step_pointers", "c2gpu_bader", "c2gpu_relabel",
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
-    "c2gpu_synth_slater",
+    "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
+    "c2gpu_bincube_read",
 )
 
 
@@ -56,6 +57,11 @@ class BaderStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CubeHeader(C.Structure):
+    _fields_ = [("nat", C.c_int), ("ismo", C.c_int), ("n", C.c_int * 3), ("x0", C.c_double * 3),
+                ("step", C.c_double * 9), ("data_offset", C.c_uint64)]
 
 
 class RadialTable(C.Structure):
@@ -103,6 +109,10 @@ def load():
         "c2gpu_nci": [vp, vp, vp, vp, i, i, C.POINTER(Promol), i, i, vp, vp],
         "c2gpu_nci_f32": [vp, vp, vp, vp, i, i, C.POINTER(Promol), i, i, vp, vp],
         "c2gpu_synth_slater": [vp, vp, vp, i, vp, vp, d, i, vp],
+        "c2gpu_cube_read_header": [vp, C.c_char_p, i, C.POINTER(CubeHeader)],
+        "c2gpu_cube_read": [vp, C.c_char_p, i, C.POINTER(CubeHeader), vp],
+        "c2gpu_cube_parse": [vp, vp, C.c_uint64, i, C.POINTER(CubeHeader), vp],
+        "c2gpu_bincube_read": [vp, C.c_char_p, i, C.POINTER(CubeHeader), vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

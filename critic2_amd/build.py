@@ -23,6 +23,7 @@ SOURCES = [
     ("sums.hip", ["-ffp-contract=off"]),
     ("nci.hip", ["-ffp-contract=fast"]),
     ("ncigrid.hip", ["-ffp-contract=off"]),
+    ("cube.hip", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
 ]

@@ -24,6 +24,10 @@ void nci_run(c2gpu_ctx *, const double *, const int *, const double *, const dou
 void batch_run(c2gpu_ctx *, int, const c2gpu_cell *, int, double, int, const c2gpu_radial_table *, int, int,
                c2gpu_batch_stats *);
 void destroy_workers(c2gpu_ctx *);
+void cube_header(const char *, c2gpu_cube_header *, int);
+void cube_parse_run(c2gpu_ctx *, const char *, uint64_t, int, c2gpu_cube_header *, double *);
+void cube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
+void bincube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
 void synth_slater_run(c2gpu_ctx *, const double *, const int *, int, const double *, const int *,
                       double, int, double *);
 
@@ -445,6 +449,27 @@ int c2gpu_synth_slater(c2gpu_ctx *ctx, const double x2c[9], const int n[3], int 
     if (flags & C2GPU_PTR_DEVICE) require_device_ptr(rho_out, "rho_out");
     synth_slater_run(ctx, x2c, n, nat, xat, zat, rimg, flags, rho_out);
   });
+}
+
+int c2gpu_cube_read_header(c2gpu_ctx *ctx, const char *path, int binary, c2gpu_cube_header *hdr) {
+  return guarded(ctx, [&] {
+    if (!hdr) fail(C2GPU_ERR_ARG, "hdr is NULL");
+    memset(hdr, 0, sizeof(*hdr));
+    cube_header(path, hdr, binary);
+  });
+}
+
+int c2gpu_cube_read(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_header *hdr, double *f) {
+  return guarded(ctx, [&] { cube_read_run(ctx, path, flags, hdr, f); });
+}
+
+int c2gpu_cube_parse(c2gpu_ctx *ctx, const char *text, uint64_t len, int flags, c2gpu_cube_header *hdr,
+                     double *f) {
+  return guarded(ctx, [&] { cube_parse_run(ctx, text, len, flags, hdr, f); });
+}
+
+int c2gpu_bincube_read(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_header *hdr, double *f) {
+  return guarded(ctx, [&] { bincube_read_run(ctx, path, flags, hdr, f); });
 }
 
 }  // extern "C"

@@ -38,8 +38,11 @@ enum {
   C2GPU_ERR_NEWMAX = 6,       /* bader@proc.f90:400-402: new maximum in
                                  edge refinement (fatal in critic2 too)    */
   C2GPU_ERR_PATHLEN = 7,      /* an ascent path exceeded the path limit    */
-  C2GPU_ERR_DEGENERATE = 8    /* plateau case whose serial outcome depends
+  C2GPU_ERR_DEGENERATE = 8,   /* plateau case whose serial outcome depends
                                  on visiting order inside one sweep        */
+  C2GPU_ERR_FORMAT = 9        /* malformed or truncated input file (the
+                                 "Error reading file" of grid3mod@proc.f90:
+                                 527-571)                                  */
 };
 
 enum { C2GPU_PTR_DEVICE = 1 };
@@ -280,6 +283,49 @@ typedef struct {
 int c2gpu_batch_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int attr_mode, double ratom,
                     int nspecies, const c2gpu_radial_table *tables, int nstreams, int flags,
                     c2gpu_batch_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* Gaussian cube input                                                  */
+/* ------------------------------------------------------------------ */
+
+/* What the header of a cube holds.  x0 and step are as in the file
+ * (Cartesian bohr); critic2 then sets x2cl(:,i) = step(:,i)*n(i)
+ * (grid3mod@proc.f90:546-547) and converts x0 to crystallographic
+ * coordinates with the caller's cell (:540-541). */
+typedef struct {
+  int nat;               /* number of atom records (|natoms| of the file)  */
+  int ismo;              /* natoms < 0: orbital cube, one more record       */
+  int n[3];              /* grid points along each axis                     */
+  double x0[3];          /* origin                                          */
+  double step[9];        /* step(:,i) = the file's axis-i vector, col-major */
+  uint64_t data_offset;  /* byte offset of the grid values (text), or of the
+                            grid record's first marker (binary)             */
+} c2gpu_cube_header;
+
+/* Header only (the host reads it; no device work).  binary: 0 text cube
+ * (grid3mod@proc.f90:533-551), 1 binary cube (:592-609). */
+int c2gpu_cube_read_header(c2gpu_ctx *ctx, const char *path, int binary, c2gpu_cube_header *hdr);
+
+/* grid3%read_cube (grid3mod@proc.f90:512-572): the text cube at path, its
+ * grid values converted on the GPU with correct rounding (the value the
+ * Fortran list-directed read gives) into f(n1,n2,n3), x fastest (the file
+ * holds them z fastest).  f: n1*n2*n3 doubles, device memory with
+ * C2GPU_PTR_DEVICE.  Values past n1*n2*n3 are ignored; fewer values, a
+ * non-numeric token, or a list-directed form other than blank-separated
+ * constants (r*c repeats, commas, slashes) give C2GPU_ERR_FORMAT. */
+int c2gpu_cube_read(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_header *hdr, double *f);
+
+/* The same over the bytes of a text cube already in memory: text holds len
+ * bytes (device memory with C2GPU_PTR_DEVICE, as is f then).  f NULL: only
+ * hdr is filled. */
+int c2gpu_cube_parse(c2gpu_ctx *ctx, const char *text, uint64_t len, int flags, c2gpu_cube_header *hdr,
+                     double *f);
+
+/* grid3%read_bincube (grid3mod@proc.f90:575-635): unformatted sequential
+ * records with 4-byte markers (subrecords of records over 2 GiB followed):
+ * (natoms, x0), (n, step), one record per atom, then f in Fortran order,
+ * streamed to f (device memory with C2GPU_PTR_DEVICE). */
+int c2gpu_bincube_read(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_header *hdr, double *f);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic input (no counterpart in critic2: test and benchmark input) */

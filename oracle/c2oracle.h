@@ -135,6 +135,13 @@ void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
                         const double *xat, const int *zat, double rimg,
                         double *out);
 
+/* Cube readers (grid3mod@proc.f90:512-635); f == NULL: header only.
+ * step: column i = the file's axis-i vector.  Return 0, or 1 on a read
+ * error. */
+int oracle_read_cube(const char *path, int *nat, int *ismo, int n[3], double x0[3], double step[9], double *f);
+int oracle_read_bincube(const char *path, int *nat, int *ismo, int n[3], double x0[3], double step[9],
+                        double *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -315,3 +315,33 @@ def nci_grid_interp(f, n, x2c):
     lib.oracle_nci_grid(f.ctypes.data, nn.ctypes.data, x.ctypes.data, c2x.ctypes.data, crho.ctypes.data,
                         s.ctypes.data)
     return crho, s
+
+
+def _cube_call(fn_name, path):
+    lib = load()
+    fn = getattr(lib, fn_name)
+    fn.argtypes = [C.c_char_p] + [C.c_void_p] * 6
+    fn.restype = C.c_int
+    nat, ismo = C.c_int(), C.c_int()
+    n = np.zeros(3, dtype=np.int32)
+    x0 = np.zeros(3)
+    step = np.zeros(9)
+    p = os.fsencode(path)
+    if fn(p, C.byref(nat), C.byref(ismo), n.ctypes.data, x0.ctypes.data, step.ctypes.data, None):
+        raise ValueError(f"{fn_name}: error reading {path}")
+    f = np.empty(int(np.prod(n)), dtype=np.float64)
+    if fn(p, C.byref(nat), C.byref(ismo), n.ctypes.data, x0.ctypes.data, step.ctypes.data, f.ctypes.data):
+        raise ValueError(f"{fn_name}: error reading {path}")
+    hdr = {"nat": nat.value, "ismo": bool(ismo.value), "n": tuple(int(v) for v in n), "x0_cart": x0,
+           "step": step.reshape(3, 3).T}
+    return hdr, f
+
+
+def read_cube(path):
+    """grid3%read_cube restated (strtod conversion): (header, f x-fastest)."""
+    return _cube_call("oracle_read_cube", path)
+
+
+def read_bincube(path):
+    """grid3%read_bincube restated: (header, f x-fastest)."""
+    return _cube_call("oracle_read_bincube", path)

@@ -1,0 +1,90 @@
+"""Cube files for the reader tests: text and binary writers following
+critic2's writegrid_cube (crystalmod@write.f90:3476-3570) and an
+independent expected-value path (Python's float(), correctly rounded)."""
+import re
+import struct
+
+import numpy as np
+
+_BARE_EXP = re.compile(r"^([+-]?(?:\d+\.?\d*|\.\d+))([+-]\d+)$")
+
+
+def fortran_float(tok: str) -> float:
+    """A Fortran real constant to binary64 with Python's float (D/Q exponent
+    letters, bare signed exponents)."""
+    t = tok
+    for a in "DdQq":
+        t = t.replace(a, "E")
+    m = _BARE_EXP.match(t)
+    if m:
+        t = m.group(1) + "E" + m.group(2)
+    return float(t)
+
+
+def expected_grid(text: bytes, n, data_offset: int) -> np.ndarray:
+    """f(n1,n2,n3) x fastest from the value tokens after the header."""
+    toks = text[data_offset:].split()
+    n1, n2, n3 = n
+    npts = n1 * n2 * n3
+    v = np.array([fortran_float(t.decode()) for t in toks[:npts]], dtype=np.float64)
+    # file order (((f(i,j,k),k),j),i): k fastest
+    return v.reshape(n1, n2, n3).transpose(2, 1, 0).reshape(-1)
+
+
+def header(n, step, x0=(0.0, 0.0, 0.0), atoms=(), ismo=False, precise=False):
+    nat = len(atoms)
+    lines = ["critic2-cube", "critic2-cube"]
+    if precise:
+        f = lambda i, v: f"{i:5d} " + " ".join(f"{x:22.14E}" for x in v) + " "
+    else:
+        f = lambda i, v: f"{i:5d}" + "".join(f"{x:12.6f}" for x in v)
+    lines.append(f(-nat if ismo else nat, x0))
+    for i in range(3):
+        lines.append(f(n[i], step[i]))
+    for z, x in atoms:
+        lines.append(f"{z:4d}{0.0:5.1f}{x[0]:11.6f}{x[1]:11.6f}{x[2]:11.6f}")
+    if ismo:
+        lines.append("    1    7")
+    return "\n".join(lines) + "\n"
+
+
+def body(tokens_file_order, per_line=6, n3=None, sep=" ", eol="\n"):
+    """Value records: per (i,j) row of n3 values, per_line per line."""
+    out = []
+    toks = list(tokens_file_order)
+    rows = [toks[r:r + n3] for r in range(0, len(toks), n3)] if n3 else [toks]
+    for row in rows:
+        for s in range(0, len(row), per_line):
+            out.append(sep + sep.join(row[s:s + per_line]) + eol)
+    return "".join(out)
+
+
+def fortran_records(*records: bytes, split=None) -> bytes:
+    """gfortran unformatted sequential records; split=(record index, parts)
+    writes that record as subrecords (negative markers mark continuation)."""
+    out = b""
+    for ir, rec in enumerate(records):
+        if split and split[0] == ir:
+            parts = split[1]
+            cuts = np.linspace(0, len(rec), parts + 1).astype(int)
+            for p in range(parts):
+                chunk = rec[cuts[p]:cuts[p + 1]]
+                lead = -len(chunk) if p < parts - 1 else len(chunk)
+                trail = -len(chunk) if p > 0 else len(chunk)
+                out += struct.pack("<i", lead) + chunk + struct.pack("<i", trail)
+        else:
+            out += struct.pack("<i", len(rec)) + rec + struct.pack("<i", len(rec))
+    return out
+
+
+def bincube(f_xfast, n, step, x0=(0.0, 0.0, 0.0), atoms=(), ismo=False, split=None) -> bytes:
+    """step: 3x3, column i = axis-i vector (written as Fortran xd(3,3))."""
+    nat = len(atoms)
+    recs = [struct.pack("<i3d", -nat if ismo else nat, *x0),
+            struct.pack("<3i9d", *n, *np.asarray(step, dtype=np.float64).reshape(-1, order="F"))]
+    for z, x in atoms:
+        recs.append(struct.pack("<i4d", z, 0.0, *x))
+    recs.append(np.ascontiguousarray(f_xfast, dtype="<f8").tobytes())
+    if split is not None:
+        split = (len(recs) - 1, split)
+    return fortran_records(*recs, split=split)
