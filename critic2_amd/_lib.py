@@ -24,6 +24,7 @@ PTR_DEVICE = 1
 ATTR_NEAREST_ATOM, ATTR_RATOM, ATTR_NOATOMS = 0, 1, 2
 FT_GRAD, FT_XX, FT_YY, FT_ZZ, FT_LAP = 0, 1, 2, 3, 4
 NCI_FFT, NCI_GRID, NCI_PROMOL = 0, 1, 2
+CUBE_E22_14, CUBE_1P_E12_5, CUBE_1P_E13_5 = 0, 1, 2
 FP64, FP32 = 0, 1
 
 EXPORTS = (
@@ -33,7 +34,7 @@ EXPORTS = (
     "c2gpu_bader_metrics", "c2gpu_bader_step_pointers", "c2gpu_bader", "c2gpu_relabel",
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
-    "c2gpu_bincube_read",
+    "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write",
 )
 
 
@@ -113,6 +114,8 @@ def load():
         "c2gpu_cube_read": [vp, C.c_char_p, i, C.POINTER(CubeHeader), vp],
         "c2gpu_cube_parse": [vp, vp, C.c_uint64, i, C.POINTER(CubeHeader), vp],
         "c2gpu_bincube_read": [vp, C.c_char_p, i, C.POINTER(CubeHeader), vp],
+        "c2gpu_cube_format": [vp, vp, vp, i, i, i, vp, C.c_uint64, C.POINTER(C.c_uint64)],
+        "c2gpu_cube_write": [vp, C.c_char_p, C.POINTER(CubeHeader), vp, vp, vp, i, i],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

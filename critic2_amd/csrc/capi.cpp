@@ -28,6 +28,9 @@ void cube_header(const char *, c2gpu_cube_header *, int);
 void cube_parse_run(c2gpu_ctx *, const char *, uint64_t, int, c2gpu_cube_header *, double *);
 void cube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
 void bincube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
+void cube_format_run(c2gpu_ctx *, const double *, const int *, int, int, int, char *, uint64_t, uint64_t *);
+void cube_write_run(c2gpu_ctx *, const char *, const c2gpu_cube_header *, const int *, const double *,
+                    const double *, int, int);
 void synth_slater_run(c2gpu_ctx *, const double *, const int *, int, const double *, const int *,
                       double, int, double *);
 
@@ -470,6 +473,16 @@ int c2gpu_cube_parse(c2gpu_ctx *ctx, const char *text, uint64_t len, int flags, 
 
 int c2gpu_bincube_read(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_header *hdr, double *f) {
   return guarded(ctx, [&] { bincube_read_run(ctx, path, flags, hdr, f); });
+}
+
+int c2gpu_cube_format(c2gpu_ctx *ctx, const double *f, const int n[3], int style, int zfast, int flags,
+                      char *text, uint64_t cap, uint64_t *len) {
+  return guarded(ctx, [&] { cube_format_run(ctx, f, n, style, zfast, flags, text, cap, len); });
+}
+
+int c2gpu_cube_write(c2gpu_ctx *ctx, const char *path, const c2gpu_cube_header *hdr, const int *zat,
+                     const double *xat, const double *f, int style, int flags) {
+  return guarded(ctx, [&] { cube_write_run(ctx, path, hdr, zat, xat, f, style, flags); });
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@
 
 #include "common.hpp"
 #include "devutil.hpp"
+#include "format.hpp"
 #include "number.hpp"
 
 namespace c2 {
@@ -319,6 +320,51 @@ __global__ void __launch_bounds__(256) k_cube_transpose(const double *vals, doub
   }
 }
 
+// ---- device: cube body text (writegrid_cube, crystalmod@write.f90:3557-3568;
+// nciplot's write_cube_body, nci@proc.f90:916-932)
+//
+// Per (i,j) row, the n3 values in records of six, each " " + the edit
+// descriptor's field; a record that ends before the sixth value also holds
+// the next " " (the format's character edit descriptor is processed before
+// the data list is found exhausted).  Every field has the style's fixed
+// width, so each value's byte offset follows from its index.
+
+struct BodyLayout {
+  int W;          // field width
+  int n3;
+  uint64_t row;   // bytes per (i,j) row
+};
+
+__host__ __device__ inline BodyLayout body_layout(int n3, int style) {
+  BodyLayout L;
+  L.W = fmt_width(style);
+  L.n3 = n3;
+  const uint64_t nrec = (uint64_t)(n3 + 5) / 6;
+  L.row = (uint64_t)n3 * (L.W + 1) + nrec + (n3 % 6 ? 1 : 0);
+  return L;
+}
+
+// zf: values in file order (k fastest); one thread per value.
+__global__ void __launch_bounds__(256) k_cube_format(const double *zf, uint64_t npts, BodyLayout L, int style,
+                                                     const uint64_t *pow5, const int16_t *pow5s, char *text,
+                                                     unsigned int *err) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  const uint64_t r = t / (uint64_t)L.n3;
+  const int k = (int)(t - r * (uint64_t)L.n3);
+  char *o = text + r * L.row + (uint64_t)k * (L.W + 1) + (uint64_t)(k / 6);
+  char fld[24];
+  if (!fmt_value(zf[t], style, pow5, pow5s, fld)) atomicOr(err, 1u);
+  o[0] = ' ';
+  for (int c = 0; c < L.W; c++) o[1 + c] = fld[c];
+  if (k % 6 == 5) {
+    o[L.W + 1] = '\n';
+  } else if (k == L.n3 - 1) {
+    o[L.W + 1] = ' ';
+    o[L.W + 2] = '\n';
+  }
+}
+
 // Converts the grid values in [lo, hi) of base (device memory, base 16-byte
 // aligned) into f (device, x fastest).
 void parse_values(c2gpu_ctx *ctx, const uint8_t *base, uint64_t lo, uint64_t hi, const int n[3], double *f) {
@@ -360,6 +406,79 @@ void parse_values(c2gpu_ctx *ctx, const uint8_t *base, uint64_t lo, uint64_t hi,
 }
 
 uint64_t npts_of(const c2gpu_cube_header &h) { return (uint64_t)h.n[0] * h.n[1] * h.n[2]; }
+
+uint64_t body_bytes(const int n[3], int style) {
+  return (uint64_t)n[0] * n[1] * body_layout(n[2], style).row;
+}
+
+// Body text of f (device; x fastest unless zfast) into text (device).
+void format_body(c2gpu_ctx *ctx, const double *f, const int n[3], int style, int zfast, char *text) {
+  const uint64_t npts = (uint64_t)n[0] * n[1] * n[2];
+  const double *zf = f;
+  if (!zfast) {
+    double *t = buf<double>(ctx, "cube_zf", npts);
+    // the transpose kernel with the axes swapped: x fastest -> z fastest
+    dim3 grid((unsigned)(((n[0] + TR - 1) / TR) * (uint64_t)n[1]), (n[2] + TR - 1) / TR);
+    C2_LAUNCH(ctx, k_cube_transpose, grid, dim3(256), 0, f, t, n[2], n[1], n[0]);
+    add_kernel_bytes(ctx, "k_cube_transpose", 16.0 * (double)npts);
+    zf = t;
+  }
+  const uint64_t *p5;
+  const int16_t *p5s;
+  pow5_device(ctx, p5, p5s);
+  unsigned int *err = buf<unsigned int>(ctx, "cube_fmt_err", 1);
+  C2_HIP(hipMemsetAsync(err, 0, 4, ctx->stream));
+  const BodyLayout L = body_layout(n[2], style);
+  C2_LAUNCH(ctx, k_cube_format, dim3(grid_for((int64_t)npts, 256)), dim3(256), 0, zf, npts, L, style, p5, p5s,
+            text, err);
+  add_kernel_bytes(ctx, "k_cube_format", 8.0 * (double)npts + (double)body_bytes(n, style));
+  if (read_scalar(ctx, err)) fail(C2GPU_ERR_HIP, "cube: value formatting overflowed its big integers");
+}
+
+// Header of writegrid_cube (crystalmod@write.f90:3537-3556): precise
+// (E22.14E3 fields) for FMT_E22_14, else the F12.6 / F11.6 form.
+std::string grid_header(const c2gpu_cube_header &h, const int *zat, const double *xat, int style) {
+  const Pow5Host &p = pow5_host();
+  std::string s = "critic2-cube\ncritic2-cube\n";
+  char b[64];
+  auto e22 = [&](double v) {
+    char f[24];
+    fmt_value(v, FMT_E22_14, p.t.data(), p.s.data(), f);
+    s.append(f, 22);
+  };
+  const bool precise = style == FMT_E22_14;
+  auto line = [&](int iv, const double *v3) {
+    snprintf(b, sizeof b, "%5d", iv);
+    s += b;
+    for (int a = 0; a < 3; a++) {
+      if (precise) {
+        s += ' ';
+        e22(v3[a]);
+      } else {
+        snprintf(b, sizeof b, "%12.6f", v3[a]);
+        s += b;
+      }
+    }
+    s += precise ? " \n" : "\n";
+  };
+  line(h.nat, h.x0);
+  for (int i = 0; i < 3; i++) line(h.n[i], h.step + 3 * i);
+  for (int a = 0; a < h.nat; a++) {
+    if (precise) {
+      snprintf(b, sizeof b, "%5d %5.1f", zat[a], 0.0);
+      s += b;
+      for (int c = 0; c < 3; c++) {
+        s += ' ';
+        e22(xat[3 * a + c]);
+      }
+      s += " \n";
+    } else {
+      snprintf(b, sizeof b, "%4d%5.1f%11.6f%11.6f%11.6f\n", zat[a], 0.0, xat[3 * a], xat[3 * a + 1], xat[3 * a + 2]);
+      s += b;
+    }
+  }
+  return s;
+}
 
 // Device output: the caller's (PTR_DEVICE) or a context buffer.
 double *out_grid(c2gpu_ctx *ctx, int flags, double *f, uint64_t npts) {
@@ -613,6 +732,99 @@ void bincube_read_run(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_he
   });
   if (got < need) fail(C2GPU_ERR_FORMAT, "binary cube: grid record holds fewer values than n1*n2*n3");
   C2_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void cube_format_run(c2gpu_ctx *ctx, const double *f, const int n[3], int style, int zfast, int flags, char *text,
+                     uint64_t cap, uint64_t *len) {
+  if (!n || !len) fail(C2GPU_ERR_ARG, "cube: NULL argument");
+  if (style < FMT_E22_14 || style > FMT_1P_E13_5) fail(C2GPU_ERR_ARG, "cube: unknown body style");
+  for (int a = 0; a < 3; a++)
+    if (n[a] < 1) fail(C2GPU_ERR_ARG, "cube: grid dimensions must be positive");
+  const uint64_t nb = body_bytes(n, style);
+  *len = nb;
+  if (!text) return;  // size query
+  if (!f) fail(C2GPU_ERR_ARG, "cube: f is NULL");
+  if (cap < nb) fail(C2GPU_ERR_ARG, "cube: text buffer holds " + std::to_string(cap) + " bytes, need " +
+                                        std::to_string(nb));
+  const uint64_t npts = (uint64_t)n[0] * n[1] * n[2];
+  if (flags & C2GPU_PTR_DEVICE) {
+    require_device_ptr(f, "f");
+    require_device_ptr(text, "text");
+    format_body(ctx, f, n, style, zfast, text);
+  } else {
+    double *df = buf<double>(ctx, "cube_fin", npts);
+    C2_HIP(hipMemcpyAsync(df, f, npts * 8, hipMemcpyHostToDevice, ctx->stream));
+    char *dt = buf<char>(ctx, "cube_text_out", nb);
+    format_body(ctx, df, n, style, zfast, dt);
+    C2_HIP(hipMemcpyAsync(text, dt, nb, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  C2_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void cube_write_run(c2gpu_ctx *ctx, const char *path, const c2gpu_cube_header *h, const int *zat,
+                    const double *xat, const double *f, int style, int flags) {
+  if (!path || !h) fail(C2GPU_ERR_ARG, "cube: NULL argument");
+  if (style != FMT_E22_14 && style != FMT_1P_E12_5) fail(C2GPU_ERR_ARG, "cube: grid files use style 0 or 1");
+  if (h->nat < 0 || (h->nat > 0 && (!zat || !xat))) fail(C2GPU_ERR_ARG, "cube: atoms missing");
+  for (int a = 0; a < 3; a++)
+    if (h->n[a] < 1) fail(C2GPU_ERR_ARG, "cube: grid dimensions must be positive");
+  const std::string head = grid_header(*h, zat, xat, style);
+  char *dt = nullptr;
+  uint64_t nb = 0;
+  if (f) {  // f NULL: header only (writegrid_cube's onlyheader)
+    const uint64_t npts = npts_of(*h);
+    nb = body_bytes(h->n, style);
+    dt = buf<char>(ctx, "cube_text_out", nb);
+    if (flags & C2GPU_PTR_DEVICE) {
+      require_device_ptr(f, "f");
+      format_body(ctx, f, h->n, style, 0, dt);
+    } else {
+      double *df = buf<double>(ctx, "cube_fin", npts);
+      C2_HIP(hipMemcpyAsync(df, f, npts * 8, hipMemcpyHostToDevice, ctx->stream));
+      format_body(ctx, df, h->n, style, 0, dt);
+    }
+  }
+  FILE *fp = fopen(path, "wb");
+  if (!fp) fail(C2GPU_ERR_FORMAT, std::string("Error writing file: ") + path);
+  struct Closer {
+    FILE *fp;
+    ~Closer() {
+      if (fp) fclose(fp);
+    }
+  } closer{fp};
+  if (fwrite(head.data(), 1, head.size(), fp) != head.size()) fail(C2GPU_ERR_FORMAT, "cube: write failed");
+  if (nb) {
+    Pinned pin(nb < CHUNK ? nb : CHUNK);
+    uint64_t done = 0;
+    int which = 0;
+    // copy chunk c+1 while chunk c is written
+    size_t pending = 0;
+    int pend_buf = -1;
+    while (done < nb || pend_buf >= 0) {
+      size_t want = 0;
+      if (done < nb) {
+        want = (size_t)((nb - done) < pin.bytes ? (nb - done) : pin.bytes);
+        C2_HIP(hipMemcpyAsync(pin.p[which], dt + done, want, hipMemcpyDeviceToHost, ctx->stream));
+        C2_HIP(hipEventRecord(pin.ev[which], ctx->stream));
+      }
+      if (pend_buf >= 0) {
+        if (fwrite(pin.p[pend_buf], 1, pending, fp) != pending) fail(C2GPU_ERR_FORMAT, "cube: write failed");
+        pend_buf = -1;
+      }
+      if (want) {
+        C2_HIP(hipEventSynchronize(pin.ev[which]));
+        pending = want;
+        pend_buf = which;
+        done += want;
+        which ^= 1;
+      }
+    }
+  }
+  if (fclose(fp) != 0) {
+    closer.fp = nullptr;
+    fail(C2GPU_ERR_FORMAT, "cube: write failed");
+  }
+  closer.fp = nullptr;
 }
 
 }  // namespace c2

@@ -23,7 +23,7 @@
 
 namespace c2 {
 
-static constexpr int POW5_QMIN = -400, POW5_QMAX = 310;
+static constexpr int POW5_QMIN = -400, POW5_QMAX = 340;  // reading needs up to 310, writing 338
 static constexpr int NUM_MAXDIG = 40;
 
 struct DecNum {

@@ -86,3 +86,46 @@ def read_bincube(ctx: Context, path: str, x2c=None, molx0=None, device: bool = F
     ctx.check(ctx.lib.c2gpu_bincube_read(ctx.h, os.fsencode(path), flags, C.byref(h), _lib.ptr(f)),
               "c2gpu_bincube_read")
     return _header_dict(h, x2c, molx0), f
+
+
+def format_body(ctx: Context, f, n, style: int = _lib.CUBE_E22_14, zfast: bool = False) -> bytes:
+    """The body text of a cube for f (host numpy array, x fastest unless
+    zfast): critic2's writegrid_cube (styles 0, 1) or nciplot's
+    write_cube_body (style 2) records."""
+    nn = np.asarray(n, dtype=np.int32)
+    ln = C.c_uint64()
+    ctx.check(ctx.lib.c2gpu_cube_format(ctx.h, None, nn.ctypes.data, style, int(zfast), 0, None, 0, C.byref(ln)),
+              "c2gpu_cube_format")
+    out = np.empty(ln.value, dtype=np.uint8)
+    if hasattr(f, "data_ptr"):
+        d = ctx.empty((ln.value,), np.uint8)
+        ctx.check(ctx.lib.c2gpu_cube_format(ctx.h, f.data_ptr(), nn.ctypes.data, style, int(zfast),
+                                            _lib.PTR_DEVICE, d.data_ptr(), ln.value, C.byref(ln)),
+                  "c2gpu_cube_format")
+        return d.numpy().tobytes()
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    ctx.check(ctx.lib.c2gpu_cube_format(ctx.h, f.ctypes.data, nn.ctypes.data, style, int(zfast), 0,
+                                        out.ctypes.data, ln.value, C.byref(ln)), "c2gpu_cube_format")
+    return out.tobytes()
+
+
+def write_cube(ctx: Context, path: str, n, x0, step, zat=(), xat=(), f=None, style: int = _lib.CUBE_E22_14):
+    """writegrid_cube, text form: step 3x3 with column i = axis-i vector,
+    xat Cartesian (nat, 3); f x fastest (host or DeviceArray) or None for
+    the header alone."""
+    h = CubeHeader()
+    zat = np.ascontiguousarray(zat, dtype=np.int32).reshape(-1)
+    xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1)
+    h.nat = len(zat)
+    for a in range(3):
+        h.n[a] = int(n[a])
+        h.x0[a] = float(x0[a])
+    st = np.asarray(step, dtype=np.float64)
+    for i in range(3):
+        for a in range(3):
+            h.step[a + 3 * i] = float(st[a, i])
+    flags = _lib.PTR_DEVICE if hasattr(f, "data_ptr") else 0
+    fp = None if f is None else (f if flags else np.ascontiguousarray(f, dtype=np.float64).reshape(-1))
+    ctx.check(ctx.lib.c2gpu_cube_write(ctx.h, os.fsencode(path), C.byref(h), zat.ctypes.data if len(zat) else None,
+                                       xat.ctypes.data if len(xat) else None, _lib.ptr(fp), style, flags),
+              "c2gpu_cube_write")

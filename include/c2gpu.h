@@ -285,7 +285,7 @@ int c2gpu_batch_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int attr
                     c2gpu_batch_stats *stats);
 
 /* ------------------------------------------------------------------ */
-/* Gaussian cube input                                                  */
+/* Gaussian cube input and output                                       */
 /* ------------------------------------------------------------------ */
 
 /* What the header of a cube holds.  x0 and step are as in the file
@@ -326,6 +326,39 @@ int c2gpu_cube_parse(c2gpu_ctx *ctx, const char *text, uint64_t len, int flags, 
  * (natoms, x0), (n, step), one record per atom, then f in Fortran order,
  * streamed to f (device memory with C2GPU_PTR_DEVICE). */
 int c2gpu_bincube_read(c2gpu_ctx *ctx, const char *path, int flags, c2gpu_cube_header *hdr, double *f);
+
+/* Cube body styles: the Fortran edit descriptor of each grid value.
+ * Values are correctly rounded (nearest, ties to even on the exact binary
+ * value), negative values carry '-', fields too narrow hold asterisks, as
+ * gfortran's formatted WRITE gives them. */
+enum {
+  C2GPU_CUBE_E22_14 = 0,   /* '(6(" ",E22.14E3))': writegrid_cube with
+                              precisecube, the default
+                              (crystalmod@write.f90:3563)              */
+  C2GPU_CUBE_1P_E12_5 = 1, /* '(1p,6(" ",E12.5E3))': writegrid_cube
+                              without precisecube (:3565)              */
+  C2GPU_CUBE_1P_E13_5 = 2  /* '(6(" ",1p,e13.5e3))': nciplot's
+                              write_cube_body (nci@proc.f90:927)       */
+};
+
+/* The body text of a cube: for each (i,j) in file order, the n3 values
+ * f(i,j,1..n3) in records of six, a record closed early also holding the
+ * format's next " " (crystalmod@write.f90:3557-3568, nci@proc.f90:925-929).
+ * zfast: 0 f is x fastest (critic2's grids), 1 f is already z fastest
+ * (c2gpu_nci's zfast outputs).  *len receives the byte count; text NULL
+ * queries it, else text must hold cap >= *len bytes.  With
+ * C2GPU_PTR_DEVICE f and text are device memory. */
+int c2gpu_cube_format(c2gpu_ctx *ctx, const double *f, const int n[3], int style, int zfast, int flags,
+                      char *text, uint64_t cap, uint64_t *len);
+
+/* grid3 writer writegrid_cube (crystalmod@write.f90:3476-3570), text form:
+ * two "critic2-cube" titles, the header from hdr (nat, x0, n, step) and the
+ * atoms (zat, xat: Cartesian bohr, as the caller shifted them), then the
+ * body of f (x fastest; NULL writes the header only, critic2's onlyheader).
+ * style: C2GPU_CUBE_E22_14 (precisecube header and body) or
+ * C2GPU_CUBE_1P_E12_5 (the F12.6 header and 1p body). */
+int c2gpu_cube_write(c2gpu_ctx *ctx, const char *path, const c2gpu_cube_header *hdr, const int *zat,
+                     const double *xat, const double *f, int style, int flags);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic input (no counterpart in critic2: test and benchmark input) */

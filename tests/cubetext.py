@@ -88,3 +88,47 @@ def bincube(f_xfast, n, step, x0=(0.0, 0.0, 0.0), atoms=(), ismo=False, split=No
     if split is not None:
         split = (len(recs) - 1, split)
     return fortran_records(*recs, split=split)
+
+
+def fortran_e(x: float, style: int) -> str:
+    """One field of the cube body formats with Python's correctly rounded
+    formatting: style 0 E22.14E3, 1 1p,E12.5E3, 2 1p,e13.5e3."""
+    import math
+    W = {0: 22, 1: 12, 2: 13}[style]
+    P = 14 if style == 0 else 6
+    if math.isnan(x):
+        t = "NaN"
+    elif math.isinf(x):
+        t = "-Infinity" if x < 0 else "Infinity"
+    else:
+        s = f"{x:.{P - 1}e}"
+        neg = s.startswith("-")
+        mant, ex = s.lstrip("-").split("e")
+        digits = mant.replace(".", "")
+        e = 0 if x == 0 else int(ex) + (1 if style == 0 else 0)
+        es = f"E{'-' if e < 0 else '+'}{abs(e):03d}"
+        body = ("0." + digits if style == 0 else digits[0] + "." + digits[1:]) + es
+        t = ("-" if neg else "") + body
+    return t.rjust(W) if len(t) <= W else "*" * W
+
+
+def expected_body(f_xfast, n, style: int) -> str:
+    """Body records of writegrid_cube / write_cube_body from Python."""
+    n1, n2, n3 = n
+    z = np.asarray(f_xfast, dtype=np.float64).reshape(n3, n2, n1).transpose(2, 1, 0)  # [i, j, k]
+    out = []
+    for i in range(n1):
+        for j in range(n2):
+            row = [fortran_e(float(v), style) for v in z[i, j]]
+            for s in range(0, n3, 6):
+                chunk = row[s:s + 6]
+                out.append("".join(" " + c for c in chunk) + (" " if len(chunk) < 6 else "") + "\n")
+    return "".join(out)
+
+
+def atoms_of(text: bytes, nat: int, ismo: bool = False):
+    """(z, Cartesian positions) from the atom records of a text cube."""
+    lines = text.split(b"\n")[6:6 + nat]
+    z = [int(l.split()[0]) for l in lines]
+    x = [[fortran_float(t.decode()) for t in l.split()[2:5]] for l in lines]
+    return np.array(z, dtype=np.int32), np.array(x, dtype=np.float64).reshape(-1, 3)

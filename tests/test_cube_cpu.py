@@ -13,7 +13,7 @@ from decimal import Decimal, getcontext
 import numpy as np
 import pytest
 
-from tests.cubetext import bincube, expected_grid, fortran_float
+from tests.cubetext import bincube, expected_grid, fortran_e, fortran_float
 from tests.helpers import GOLDEN
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -29,8 +29,11 @@ def number_check(tmp_path_factory):
     subprocess.run([cxx, "-O2", "-std=c++17", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-x", "c++",
                     os.path.join(ROOT, "tests", "number_check.cpp"), "-o", exe], check=True)
 
-    def run(tokens):
-        out = subprocess.run([exe], input="\n".join(tokens) + "\n", capture_output=True, text=True, check=True)
+    def run(tokens, mode=None):
+        args = [exe] + ([mode] if mode else [])
+        out = subprocess.run(args, input="\n".join(tokens) + "\n", capture_output=True, text=True, check=True)
+        if mode == "fmt":
+            return [l[1:-1] for l in out.stdout.splitlines()]
         return out.stdout.split()
     return run
 
@@ -158,3 +161,41 @@ def test_oracle_read_bincube_records(tmp_path, split):
     assert hdr["n"] == n and hdr["nat"] == 1
     assert np.array_equal(g, f)
     assert np.array_equal(hdr["step"], step)
+
+
+def _fmt_values(seed, count):
+    rng = random.Random(seed)
+    vals = [0.0, -0.0, 1.0, -1.5, 0.1, 1e308, 1.7976931348623157e308, 5e-324, 2.2250738585072014e-308,
+            999999.5, 9999995.0, 1234565.0, 1234575.0, 0.5, 12345650000000.0, 99999950000000.0, 1e-5, 1e22,
+            float("nan"), float("inf"), float("-inf"), 9.5367431640625e-07]
+    while len(vals) < count:
+        k = rng.randrange(4)
+        if k == 0:
+            x = struct.unpack("<d", struct.pack("<Q", rng.getrandbits(64)))[0]
+            if x != x:
+                continue
+        elif k == 1:
+            x = rng.getrandbits(53) * 2.0 ** rng.randrange(-80, 40) * rng.choice([-1, 1])
+        elif k == 2:  # decimal ties at the 7th or 15th digit
+            x = float((rng.getrandbits(40) * 10 + 5) * 10 ** rng.randrange(0, 4))
+        else:
+            x = rng.getrandbits(rng.randrange(1, 60)) * 2.0 ** -1074
+        vals.append(x)
+    return vals
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_formatting_matches_python(number_check, seed):
+    vals = _fmt_values(seed, 20000)
+    lines = [f"{st} {struct.unpack('<Q', struct.pack('<d', v))[0]:x}" for v in vals for st in (0, 1, 2)]
+    got = number_check(lines, mode="fmt")
+    want = [fortran_e(v, st) for v in vals for st in (0, 1, 2)]
+    bad = [(l, g, w) for l, g, w in zip(lines, got, want) if g != w]
+    assert not bad, bad[:5]
+
+
+def test_formatting_planted_error_is_caught(number_check):
+    """A field one unit off in the last digit must not match."""
+    got = number_check(["0 " + struct.pack(">d", 0.1).hex()], mode="fmt")[0]
+    assert got == fortran_e(0.1, 0)
+    assert got != fortran_e(0.1 + 1e-14, 0)  # 0.10000000000001E+000

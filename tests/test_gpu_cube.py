@@ -11,7 +11,8 @@ import pytest
 
 from critic2_amd import cube
 from critic2_amd._lib import C2GpuError
-from tests.cubetext import bincube, body, expected_grid, fortran_float, header
+from tests.cubetext import (atoms_of, bincube, body, expected_body, expected_grid, fortran_e, fortran_float,
+                            header)
 from tests.helpers import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -23,6 +24,10 @@ def _same(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+GRID_CUBES = [p for p in CUBES if "nciplot" not in p]
+NCI_CUBES = [p for p in CUBES if "nciplot" in p]
 
 
 @pytest.mark.parametrize("path", [p for p in CUBES if "header" not in p], ids=os.path.basename)
@@ -189,3 +194,82 @@ def test_bincube_short_record_is_error(ctx, tmp_path):
     with pytest.raises(C2GpuError) as e:
         cube.read_bincube(ctx, str(p))
     assert e.value.code == 9
+
+
+# ---- writer
+
+@pytest.mark.parametrize("path", GRID_CUBES, ids=os.path.basename)
+def test_writer_reproduces_reference_grid_files(ctx, tmp_path, path):
+    """writegrid_cube output of the reference, rewritten from the values read
+    back, byte for byte (header, atoms, records, trailing blanks)."""
+    print("case", os.path.basename(path), flush=True)
+    text = open(path, "rb").read()
+    precise = b"E+" in text.split(b"\n")[2]
+    style = 0 if precise else 1
+    if "header" in path:
+        hdr, f = cube.read_header(ctx, path), None
+    else:
+        hdr, f = cube.read_cube(ctx, path)
+    z, x = atoms_of(text, hdr["nat"])
+    out = tmp_path / "w.cube"
+    cube.write_cube(ctx, str(out), hdr["n"], hdr["x0_cart"], hdr["step"], z, x, f, style=style)
+    assert out.read_bytes() == text
+
+
+@pytest.mark.parametrize("path", NCI_CUBES, ids=os.path.basename)
+def test_nci_body_reproduces_reference(ctx, path):
+    """nciplot's write_cube_body records (1p,e13.5e3) of the reference."""
+    text = open(path, "rb").read()
+    hdr, f = cube.read_cube(ctx, path)
+    assert cube.format_body(ctx, f, hdr["n"], style=2) == text[hdr["data_offset"]:]
+
+
+@pytest.mark.parametrize("style", [0, 1, 2])
+@pytest.mark.parametrize("n,zfast,device", [((7, 5, 13), False, False), ((3, 4, 12), True, False),
+                                            ((9, 2, 31), False, True), ((1, 1, 1), True, True)])
+def test_format_matches_python(ctx, style, n, zfast, device):
+    rng = random.Random(hash((style, n)) & 0xffff)
+    npts = n[0] * n[1] * n[2]
+    vals = []
+    specials = [0.0, -0.0, 5e-324, 1.7976931348623157e308, float("inf"), float("-inf"), float("nan"), 1234565.0]
+    for t in range(npts):
+        k = rng.randrange(5)
+        if k == 0 and t < len(specials) * 3:
+            vals.append(specials[t % len(specials)])
+        elif k == 1:
+            vals.append(struct.unpack("<d", struct.pack("<Q", rng.getrandbits(63) | (rng.getrandbits(1) << 63)))[0])
+        else:
+            vals.append(rng.uniform(-1, 1) * 10.0 ** rng.randrange(-20, 5))
+    f = np.array(vals)
+    # expected from the x-fastest view
+    fx = f if not zfast else f.reshape(n[0], n[1], n[2]).transpose(2, 1, 0).reshape(-1)
+    want = expected_body(fx, n, style).encode()
+    src = ctx.to_device(f) if device else f
+    got = cube.format_body(ctx, src, n, style=style, zfast=zfast)
+    assert got == want
+
+
+def test_write_read_round_trip_large(ctx, tmp_path):
+    """Precise cube of a 96x80x64 grid: reading back gives each value
+    rounded to 14 digits; writing that again gives the same bytes."""
+    n = (96, 80, 64)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(n[0] * n[1] * n[2]) * np.exp(rng.uniform(-40, 40, n[0] * n[1] * n[2]))
+    p1, p2 = tmp_path / "a.cube", tmp_path / "b.cube"
+    step = np.diag([0.1, 0.11, 0.12])
+    cube.write_cube(ctx, str(p1), n, (0.0, 0.0, 0.0), step, f=f, style=0)
+    hdr, g = cube.read_cube(ctx, str(p1))
+    want = np.array([float(f"{v:.13e}") for v in f[:5000]])
+    assert _same(g[:5000], want)
+    cube.write_cube(ctx, str(p2), n, (0.0, 0.0, 0.0), step, f=ctx.to_device(g), style=0)
+    assert p1.read_bytes() == p2.read_bytes()
+    from oracle import oracle
+    _, og = oracle.read_cube(str(p1))
+    assert _same(g, og)
+
+
+def test_standard_style_negative_values_overflow_field(ctx):
+    """1p,E12.5E3 has no room for a sign: negative values print asterisks."""
+    got = cube.format_body(ctx, np.array([-1.0, 2.0]), (1, 1, 2), style=1)
+    assert got == b" ************ 2.00000E+000 \n"
+    assert fortran_e(-1.0, 1) == "*" * 12
