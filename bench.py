@@ -187,6 +187,120 @@ def run_batch(ctx, args):
         g.free()
 
 
+def run_cube(ctx, args):
+    """Text cube input: the 512^3 Slater-sum grid written as critic2's
+    default (precisecube) cube text by the GPU writer, resident in HBM; one
+    step = c2gpu_cube_parse of that text into an x-fastest FP64 grid
+    (tokenise, scan, correctly rounded conversion, transpose).  The
+    cpu_baseline is the oracle's strtod restatement of grid3%read_cube on
+    the 256^3 precisecube of the 64-atom Slater-sum cell."""
+    from critic2_amd import cube as gc
+    cell = config(3, n_override=(args.n,) * 3)
+    N = cell.npts
+    rho = ctx.empty((N,), np.float64)
+    gb.synth_slater(ctx, cell, device_out=rho)
+    n = cell.n
+    step = np.asarray(cell.x2c) / np.array(n, dtype=np.float64)[None, :]
+    import ctypes as C
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:  # the header, as the writer prints it
+        hp = os.path.join(td, "h.cube")
+        gc.write_cube(ctx, hp, n, (0.0, 0.0, 0.0), step, f=None, style=0)
+        head = open(hp, "rb").read()
+    nn = np.asarray(n, dtype=np.int32)
+    ln = C.c_uint64()
+    ctx.check(ctx.lib.c2gpu_cube_format(ctx.h, None, nn.ctypes.data, 0, 0, 0, None, 0, C.byref(ln)), "size")
+    nb = len(head) + ln.value
+    text = ctx.empty((nb,), np.uint8)
+    hb = np.frombuffer(head, dtype=np.uint8)
+    ctx.check(ctx.lib.c2gpu_memcpy(ctx.h, text.data_ptr(), hb.ctypes.data, len(head), 1), "c2gpu_memcpy")
+    ctx.check(ctx.lib.c2gpu_cube_format(ctx.h, rho.data_ptr(), nn.ctypes.data, 0, 0, _lib.PTR_DEVICE,
+                                        text.data_ptr() + len(head), ln.value, C.byref(ln)), "c2gpu_cube_format")
+    f = ctx.empty((N,), np.float64)
+    h = _lib.CubeHeader()
+    print(f"[bench] cube text of {nb / 1e9:.3f} GB resident ({N} values)", file=sys.stderr, flush=True)
+
+    def step_():
+        ctx.check(ctx.lib.c2gpu_cube_parse(ctx.h, text.data_ptr(), nb, _lib.PTR_DEVICE, C.byref(h), f.data_ptr()),
+                  "c2gpu_cube_parse")
+
+    for w in range(args.warmup):
+        step_()
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        step_()
+        print(f"[bench] step {s_ + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    kstats = ctx.kernel_stats()
+    ctx.timing(False)
+    # the parse must return the grid the text was written from, to 14 digits
+    g = f.numpy()
+    src = rho.numpy()
+    rel = float(np.max(np.abs(g - src) / np.maximum(np.abs(src), 1e-300)))
+    name = max(kstats.items(), key=lambda kv: kv[1][0])[0]
+    ms_tot, launches, nbytes = kstats[name]
+    avg_ms = ms_tot / launches
+    ach = nbytes / launches / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
+            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps,
+            "algorithmic_bytes_per_launch": int(nbytes / launches)}
+    # the writer over the same grid (formatting into the resident text buffer)
+    ctx.timing(True)
+    ctx.sync()
+    t2 = time.perf_counter()
+    for s_ in range(args.steps):
+        ctx.check(ctx.lib.c2gpu_cube_format(ctx.h, rho.data_ptr(), nn.ctypes.data, 0, 0, _lib.PTR_DEVICE,
+                                            text.data_ptr() + len(head), ln.value, C.byref(ln)), "c2gpu_cube_format")
+    ctx.sync()
+    dtw = (time.perf_counter() - t2) / args.steps
+    wst = ctx.kernel_stats()
+    ctx.timing(False)
+    wms, wl, wb = wst["k_cube_format"]
+    wach = wb / wl / (wms / wl * 1e-3) / 1e9
+    writer = {"value": round(N / dtw / 1e6, 3), "unit": "Mvalues/s", "ms_per_step": round(dtw * 1e3, 3),
+              "roofline": {"bound": "hbm", "kernel": "k_cube_format", "achieved": round(wach, 2),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(wach / HBM_PEAK_GBS, 4),
+                           "avg_launch_ms": round(wms / wl, 4), "algorithmic_bytes_per_launch": int(wb / wl)},
+              "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in
+                                      sorted(wst.items(), key=lambda kv: -kv[1][0])[:4]}}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        small = config(2)
+        rs = gb.synth_slater(ctx, small)
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "s.cube")
+            stp = np.asarray(small.x2c) / np.array(small.n, dtype=np.float64)[None, :]
+            gc.write_cube(ctx, path, small.n, (0.0, 0.0, 0.0), stp, f=rs, style=0)
+            t1 = time.perf_counter()
+            _, og = oracle.read_cube(path)
+            tc = time.perf_counter() - t1
+        cpu = {"value": round(small.npts / tc / 1e6, 3), "unit": "Mvalues/s", "cores": 1, "kind": "port",
+               "sample": f"oracle read_cube (strtod) of a {small.n[0]}^3 precise cube, {small.npts} values"}
+    value = N / dt / 1e6
+    out = {
+        "metric": f"Mvalues/s text cube parse @{n[0]}^3 (correctly rounded, x-fastest FP64 out)",
+        "value": round(value, 3), "unit": "Mvalues/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64+f64", "data": "synthetic",
+        "config": {"workload": f"precisecube text (E22.14E3, 6 per record) of the {n[0]}^3 Slater-sum grid, "
+                               f"{nb} bytes resident in HBM", "grid": list(n), "bytes": nb,
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "max_rel_diff_vs_written_grid": rel,
+        "writer": writer,
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in
+                                sorted(kstats.items(), key=lambda kv: -kv[1][0])[:6]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +313,9 @@ def main():
                     help="instead of the 512^3 step, time the batched path on this many dimer cells "
                          "(512 = the batching workload: edges from {48,64,96,128,160}, 6-40 atoms)")
     ap.add_argument("--nstreams", type=int, default=0, help="batched path: concurrent Bader workers (0: 8)")
+    ap.add_argument("--cube", action="store_true",
+                    help="instead of the Bader+NCI step, time the text cube parser on the --n^3 grid's "
+                         "precisecube text")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a streaming kernel of known traffic (k_sqrt of the FFT |grad| grid: "
                          "8 B read + 8 B written per point, 256^3) to scale rocprof FETCH_SIZE/WRITE_SIZE")
@@ -209,6 +326,9 @@ def main():
     ctx = _lib.Context(0)
     if args.batch:
         run_batch(ctx, args)
+        return
+    if args.cube:
+        run_cube(ctx, args)
         return
     if args.copy_probe:
         probe = config(2)

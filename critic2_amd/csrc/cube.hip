@@ -11,6 +11,8 @@
 //   k_tok_parse   the workgroup's 4 KiB span (plus a 256-byte tail) staged
 //                 in LDS, each token converted to binary64 (correctly
 //                 rounded, number.hpp) and stored at its index, file order;
+//                 the rare tokens that need the exact big-integer tier (or
+//                 do not fit the staged window) go to k_tok_slow;
 //   k_cube_transpose  file order (z fastest, critic2's read loop order
 //                 (((f(i,j,k),k=1,n3),j=1,n2),i=1,n1)) -> f(n1,n2,n3), x
 //                 fastest, through 64x64 LDS tiles.
@@ -30,23 +32,12 @@
 #include "devutil.hpp"
 #include "format.hpp"
 #include "number.hpp"
+#include "pow5tab.hpp"
 
 namespace c2 {
 
-namespace {
+// ---- the 5^q table
 
-constexpr int TOK_BLOCK = 256;
-constexpr int TOK_SEG = 16;                    // bytes per thread
-constexpr int TOK_SPAN = TOK_BLOCK * TOK_SEG;  // bytes per workgroup
-constexpr int TOK_TAIL = 256;                  // staged bytes past the span
-constexpr int TOK_MAXG = 4096;                 // longest token read from global memory
-
-// ---- host: 5^q table, built once per process, uploaded once per context
-
-struct Pow5Host {
-  std::vector<uint64_t> t;
-  std::vector<int16_t> s;
-};
 const Pow5Host &pow5_host() {
   static Pow5Host h;
   static std::once_flag once;
@@ -71,6 +62,14 @@ void pow5_device(c2gpu_ctx *ctx, const uint64_t *&t, const int16_t *&s) {
   t = dt;
   s = ds;
 }
+
+namespace {
+
+constexpr int TOK_BLOCK = 256;
+constexpr int TOK_SEG = 16;                    // bytes per thread
+constexpr int TOK_SPAN = TOK_BLOCK * TOK_SEG;  // bytes per workgroup
+constexpr int TOK_TAIL = 256;                  // staged bytes past the span
+constexpr int TOK_MAXG = 4096;                 // longest token read from global memory
 
 // ---- host: list-directed header records
 
@@ -241,11 +240,19 @@ __global__ void __launch_bounds__(TOK_BLOCK) k_tok_count(const uint8_t *base, ui
   if (threadIdx.x == 0) blockcnt[blockIdx.x] = t;
 }
 
+// Tokens that end inside the staged window and need no exact tier are
+// converted from LDS here; the others (the exact tier, or a token longer
+// than the window) are appended to `slow` as (index, byte position) for
+// k_tok_slow.  stat[0]: smallest (index << 2 | error kind); stat[1]: the
+// number of slow tokens (entries past cap are dropped and the launch
+// repeated with room for all).
 __global__ void __launch_bounds__(TOK_BLOCK) k_tok_parse(const uint8_t *base, uint64_t lo, uint64_t hi,
                                                          const uint64_t *blockoff, uint64_t npts,
                                                          const uint64_t *pow5, const int16_t *pow5s,
-                                                         double *vals, unsigned long long *err) {
+                                                         double *vals, unsigned long long *stat, uint64_t *slow,
+                                                         uint64_t cap) {
   __shared__ uint4 stage[(TOK_SPAN + TOK_TAIL) / 16];
+  __shared__ uint16_t tokpos[TOK_SPAN / 2];  // a token and its blank take >= 2 bytes
   __shared__ uint32_t wtot[TOK_BLOCK / 64];
   const uint64_t gb = (uint64_t)blockIdx.x * TOK_SPAN;
   for (int k = threadIdx.x; k < (TOK_SPAN + TOK_TAIL) / 16; k += TOK_BLOCK)
@@ -255,7 +262,9 @@ __global__ void __launch_bounds__(TOK_BLOCK) k_tok_parse(const uint8_t *base, ui
   const int l0 = threadIdx.x * TOK_SEG;
   const uint8_t prev = threadIdx.x == 0 ? (gb == 0 ? (uint8_t)' ' : byte_at(base, lo, hi, gb - 1)) : s[l0 - 1];
   uint32_t mask = start_mask(s + l0, prev);
-  // exclusive prefix of the token counts over the workgroup
+  // exclusive prefix of the token counts over the workgroup; the token
+  // starts are listed in LDS so that lane j converts the j-th token (even
+  // work per lane, and neighbouring lanes read bytes about a token apart)
   const uint32_t cnt = __popc(mask);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t inc = cnt;
@@ -265,38 +274,69 @@ __global__ void __launch_bounds__(TOK_BLOCK) k_tok_parse(const uint8_t *base, ui
   }
   if (lane == 63) wtot[w] = inc;
   __syncthreads();
-  uint32_t before = 0;
-  for (int i = 0; i < w; i++) before += wtot[i];
-  uint64_t t = blockoff[blockIdx.x] + before + inc - cnt;
-  while (mask && t < npts) {
-    const int k = __ffs(mask) - 1;
-    mask &= mask - 1;
-    const int loc = l0 + k;
-    const uint64_t gpos = gb + loc;
-    // the token's bytes: from LDS when it ends inside the staged window
-    // (bytes past hi read as blanks there), else from global memory
-    int L = 0;
-    const int avail = TOK_SPAN + TOK_TAIL - loc;
-    while (L < avail && !num_sep((char)s[loc + L])) L++;
-    const char *p;
-    int n;
-    if (L < avail) {
-      p = reinterpret_cast<const char *>(s + loc);
-      n = avail;
-    } else {
-      p = reinterpret_cast<const char *>(base + gpos);
-      const uint64_t rem = hi - gpos;
-      n = rem < (uint64_t)TOK_MAXG ? (int)rem : TOK_MAXG;
-    }
-    DecNum d;
-    int rc = dec_parse(p, n, d);
-    if (rc == 0 && d.len >= n && n == TOK_MAXG) rc = 2;  // token longer than the read window
-    double v = 0;
-    if (rc == 0) rc = dec_to_double(p, d, pow5, pow5s, v) ? 2 : 0;
-    if (rc) atomicMin(err, (unsigned long long)((t << 2) | (uint64_t)rc));
-    vals[t] = v;
-    t++;
+  uint32_t before = 0, total = 0;
+  for (int i = 0; i < TOK_BLOCK / 64; i++) {
+    if (i < w) before += wtot[i];
+    total += wtot[i];
   }
+  uint32_t j = before + inc - cnt;
+  while (mask) {
+    tokpos[j++] = (uint16_t)(l0 + __ffs(mask) - 1);
+    mask &= mask - 1;
+  }
+  __syncthreads();
+  const uint64_t t0 = blockoff[blockIdx.x];
+  for (uint32_t q = threadIdx.x; q < total; q += TOK_BLOCK) {
+    const uint64_t t = t0 + q;
+    if (t >= npts) break;
+    const int loc = tokpos[q];
+    const int avail = TOK_SPAN + TOK_TAIL - loc;
+    int L = 0;
+    while (L < avail && !num_sep((char)s[loc + L])) L++;
+    bool done = false;
+    if (L < avail) {  // the token and its terminator are staged (bytes past hi read as blanks)
+      DecNum d;
+      const int rc = dec_parse(reinterpret_cast<const char *>(s + loc), avail, d);
+      if (rc) {
+        atomicMin(stat, (unsigned long long)((t << 2) | (uint64_t)rc));
+        done = true;
+      } else {
+        double v;
+        uint64_t m;
+        int eb;
+        if (dec_fast(d, pow5, pow5s, v, m, eb) == 0) {
+          vals[t] = v;
+          done = true;
+        }
+      }
+    }
+    if (!done) {
+      const unsigned long long qq = atomicAdd(stat + 1, 1ull);
+      if (qq < cap) {
+        slow[2 * qq] = t;
+        slow[2 * qq + 1] = gb + loc;
+      }
+    }
+  }
+}
+
+// The slow tokens, read from global memory with all three tiers.
+__global__ void __launch_bounds__(256) k_tok_slow(const uint8_t *base, uint64_t hi, const uint64_t *slow,
+                                                  uint64_t count, const uint64_t *pow5, const int16_t *pow5s,
+                                                  double *vals, unsigned long long *stat) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t t = slow[2 * i], gpos = slow[2 * i + 1];
+  const char *p = reinterpret_cast<const char *>(base + gpos);
+  const uint64_t rem = hi - gpos;
+  const int n = rem < (uint64_t)TOK_MAXG ? (int)rem : TOK_MAXG;
+  DecNum d;
+  int rc = dec_parse(p, n, d);
+  if (rc == 0 && d.len >= n && n == TOK_MAXG) rc = 2;  // token longer than the read window
+  double v = 0;
+  if (rc == 0) rc = dec_to_double(p, d, pow5, pow5s, v) ? 2 : 0;
+  if (rc) atomicMin(stat, (unsigned long long)((t << 2) | (uint64_t)rc));
+  vals[t] = v;
 }
 
 constexpr int TR = 64;  // transpose tile
@@ -344,17 +384,10 @@ __host__ __device__ inline BodyLayout body_layout(int n3, int style) {
   return L;
 }
 
-// zf: values in file order (k fastest); one thread per value.
-__global__ void __launch_bounds__(256) k_cube_format(const double *zf, uint64_t npts, BodyLayout L, int style,
-                                                     const uint64_t *pow5, const int16_t *pow5s, char *text,
-                                                     unsigned int *err) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npts) return;
+__device__ __forceinline__ void put_field(char *text, const BodyLayout &L, uint64_t t, const char *fld) {
   const uint64_t r = t / (uint64_t)L.n3;
   const int k = (int)(t - r * (uint64_t)L.n3);
   char *o = text + r * L.row + (uint64_t)k * (L.W + 1) + (uint64_t)(k / 6);
-  char fld[24];
-  if (!fmt_value(zf[t], style, pow5, pow5s, fld)) atomicOr(err, 1u);
   o[0] = ' ';
   for (int c = 0; c < L.W; c++) o[1 + c] = fld[c];
   if (k % 6 == 5) {
@@ -363,6 +396,35 @@ __global__ void __launch_bounds__(256) k_cube_format(const double *zf, uint64_t 
     o[L.W + 1] = ' ';
     o[L.W + 2] = '\n';
   }
+}
+
+// zf: values in file order (k fastest); one thread per value.  Values whose
+// rounding needs the exact tier go to `slow` (count in stat[1], entries past
+// cap dropped and the launch repeated) for k_cube_format_slow.
+__global__ void __launch_bounds__(256) k_cube_format(const double *zf, uint64_t npts, BodyLayout L, int style,
+                                                     const uint64_t *pow5, const int16_t *pow5s, char *text,
+                                                     unsigned long long *stat, uint64_t *slow, uint64_t cap) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  char fld[24];
+  if (fmt_value<false>(zf[t], style, pow5, pow5s, fld) == 0) {
+    put_field(text, L, t, fld);
+  } else {
+    const unsigned long long q = atomicAdd(stat + 1, 1ull);
+    if (q < cap) slow[q] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cube_format_slow(const double *zf, const uint64_t *slow, uint64_t count,
+                                                          BodyLayout L, int style, const uint64_t *pow5,
+                                                          const int16_t *pow5s, char *text,
+                                                          unsigned long long *stat) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t t = slow[i];
+  char fld[24];
+  if (fmt_value<true>(zf[t], style, pow5, pow5s, fld)) atomicOr(stat, 1ull);
+  put_field(text, L, t, fld);
 }
 
 // Converts the grid values in [lo, hi) of base (device memory, base 16-byte
@@ -374,13 +436,12 @@ void parse_values(c2gpu_ctx *ctx, const uint8_t *base, uint64_t lo, uint64_t hi,
   if (nblk > 0x7fffffffull) fail(C2GPU_ERR_ARG, "cube text too large");
   uint32_t *cnt = buf<uint32_t>(ctx, "cube_blockcnt", nblk + 1);
   uint64_t *off = buf<uint64_t>(ctx, "cube_blockoff", nblk + 1);
-  unsigned long long *err = buf<unsigned long long>(ctx, "cube_err", 1);
+  unsigned long long *stat = buf<unsigned long long>(ctx, "cube_stat", 2);
   double *vals = buf<double>(ctx, "cube_vals", npts);
   const uint64_t *p5;
   const int16_t *p5s;
   pow5_device(ctx, p5, p5s);
   C2_HIP(hipMemsetAsync(cnt + nblk, 0, 4, ctx->stream));
-  C2_HIP(hipMemsetAsync(err, 0xff, 8, ctx->stream));
   C2_LAUNCH(ctx, k_tok_count, dim3((unsigned)nblk), dim3(TOK_BLOCK), 0, base, lo, hi, cnt);
   add_kernel_bytes(ctx, "k_tok_count", (double)(hi - lo) + 4.0 * nblk);
   exclusive_scan(ctx, "cube", cnt, off, (int64_t)nblk + 1);
@@ -388,10 +449,25 @@ void parse_values(c2gpu_ctx *ctx, const uint8_t *base, uint64_t lo, uint64_t hi,
   if (ntok < npts)
     fail(C2GPU_ERR_FORMAT, "cube: file holds " + std::to_string(ntok) + " grid values, expected " +
                                std::to_string(npts));
-  C2_LAUNCH(ctx, k_tok_parse, dim3((unsigned)nblk), dim3(TOK_BLOCK), 0, base, lo, hi, off, npts, p5, p5s,
-            vals, err);
-  add_kernel_bytes(ctx, "k_tok_parse", (double)(hi - lo) + 8.0 * nblk + 8.0 * (double)npts);
-  const unsigned long long e = read_scalar(ctx, err);
+  uint64_t cap = 1ull << 16;
+  unsigned long long st[2];
+  for (;;) {
+    uint64_t *slow = buf<uint64_t>(ctx, "cube_slow", 2 * cap);
+    const unsigned long long init[2] = {~0ull, 0ull};
+    C2_HIP(hipMemcpyAsync(stat, init, sizeof init, hipMemcpyHostToDevice, ctx->stream));
+    C2_LAUNCH(ctx, k_tok_parse, dim3((unsigned)nblk), dim3(TOK_BLOCK), 0, base, lo, hi, off, npts, p5, p5s,
+              vals, stat, slow, cap);
+    add_kernel_bytes(ctx, "k_tok_parse", (double)(hi - lo) + 8.0 * nblk + 8.0 * (double)npts);
+    C2_HIP(hipMemcpyAsync(st, stat, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    if (st[1] <= cap) break;
+    cap = st[1];  // room for every slow token, then once more
+  }
+  if (st[1]) {
+    C2_LAUNCH(ctx, k_tok_slow, dim3(grid_for((int64_t)st[1], 256)), dim3(256), 0, base, hi,
+              buf<uint64_t>(ctx, "cube_slow", 2 * cap), (uint64_t)st[1], p5, p5s, vals, stat);
+  }
+  const unsigned long long e = st[1] ? read_scalar(ctx, stat) : st[0];
   if (e != ~0ull) {
     const uint64_t t = e >> 2;
     const int kk = (int)(t % n[2]), jj = (int)((t / n[2]) % n[1]), ii = (int)(t / ((uint64_t)n[2] * n[1]));
@@ -426,13 +502,27 @@ void format_body(c2gpu_ctx *ctx, const double *f, const int n[3], int style, int
   const uint64_t *p5;
   const int16_t *p5s;
   pow5_device(ctx, p5, p5s);
-  unsigned int *err = buf<unsigned int>(ctx, "cube_fmt_err", 1);
-  C2_HIP(hipMemsetAsync(err, 0, 4, ctx->stream));
+  unsigned long long *stat = buf<unsigned long long>(ctx, "cube_fmt_stat", 2);
   const BodyLayout L = body_layout(n[2], style);
-  C2_LAUNCH(ctx, k_cube_format, dim3(grid_for((int64_t)npts, 256)), dim3(256), 0, zf, npts, L, style, p5, p5s,
-            text, err);
-  add_kernel_bytes(ctx, "k_cube_format", 8.0 * (double)npts + (double)body_bytes(n, style));
-  if (read_scalar(ctx, err)) fail(C2GPU_ERR_HIP, "cube: value formatting overflowed its big integers");
+  uint64_t cap = 1ull << 16;
+  unsigned long long st[2];
+  for (;;) {
+    uint64_t *slow = buf<uint64_t>(ctx, "cube_fmt_slow", cap);
+    const unsigned long long init[2] = {0ull, 0ull};
+    C2_HIP(hipMemcpyAsync(stat, init, sizeof init, hipMemcpyHostToDevice, ctx->stream));
+    C2_LAUNCH(ctx, k_cube_format, dim3(grid_for((int64_t)npts, 256)), dim3(256), 0, zf, npts, L, style, p5, p5s,
+              text, stat, slow, cap);
+    add_kernel_bytes(ctx, "k_cube_format", 8.0 * (double)npts + (double)body_bytes(n, style));
+    C2_HIP(hipMemcpyAsync(st, stat, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    if (st[1] <= cap) break;
+    cap = st[1];
+  }
+  if (st[1]) {
+    C2_LAUNCH(ctx, k_cube_format_slow, dim3(grid_for((int64_t)st[1], 256)), dim3(256), 0, zf,
+              buf<uint64_t>(ctx, "cube_fmt_slow", cap), (uint64_t)st[1], L, style, p5, p5s, text, stat);
+    if (read_scalar(ctx, stat)) fail(C2GPU_ERR_HIP, "cube: value formatting overflowed its big integers");
+  }
 }
 
 // Header of writegrid_cube (crystalmod@write.f90:3537-3556): precise

@@ -5,6 +5,9 @@
 //                             (crystalmod@write.f90:3563)
 //   FMT_1P_E12_5  1p,E12.5E3  writegrid_cube without (:3565)
 //   FMT_1P_E13_5  1p,e13.5e3  nciplot's write_cube_body (nci@proc.f90:927)
+// and the two-digit-exponent field of nciplot's .dat rows:
+//   FMT_1P_E15_7  1p,E15.7    (nci@proc.f90:638); an exponent beyond 99
+//                             drops the letter: d.dddddddE-05, d.ddddddd-100
 // P significant digits (14 or 6) of x: n = round(x 10^k), k = P-1-E with
 // E = floor(log10 x), from the 192-bit product of the normalised
 // significand and the same 128-bit truncation T(k) of 5^k the decimal
@@ -16,12 +19,14 @@
 
 namespace c2 {
 
-enum { FMT_E22_14 = 0, FMT_1P_E12_5 = 1, FMT_1P_E13_5 = 2 };
+enum { FMT_E22_14 = 0, FMT_1P_E12_5 = 1, FMT_1P_E13_5 = 2, FMT_1P_E15_7 = 3 };
 
-__host__ __device__ __forceinline__ int fmt_width(int style) {
-  return style == FMT_E22_14 ? 22 : (style == FMT_1P_E12_5 ? 12 : 13);
+__host__ __device__ __forceinline__ constexpr int fmt_width(int style) {
+  return style == FMT_E22_14 ? 22 : (style == FMT_1P_E12_5 ? 12 : (style == FMT_1P_E13_5 ? 13 : 15));
 }
-__host__ __device__ __forceinline__ int fmt_digits(int style) { return style == FMT_E22_14 ? 14 : 6; }
+__host__ __device__ __forceinline__ constexpr int fmt_digits(int style) {
+  return style == FMT_E22_14 ? 14 : (style == FMT_1P_E15_7 ? 8 : 6);
+}
 
 __host__ __device__ __forceinline__ uint64_t pow10_u64(int p) {
   uint64_t v = 1;
@@ -45,10 +50,12 @@ __host__ __device__ inline int cmp_mid(uint64_t m, int e, int k, uint64_t I, boo
 }
 
 // P significant digits of x > 0 (finite): n in [10^(P-1), 10^P) and the
-// decimal exponent E with x ~= n 10^(E-P+1).  Returns false only on an
-// internal overflow of the exact tier (not reachable for binary64).
-__host__ __device__ inline bool sig_digits(double x, int P, const uint64_t *pow5, const int16_t *pow5s,
-                                           uint64_t &n, int &E) {
+// decimal exponent E with x ~= n 10^(E-P+1).  Returns 0; 1 when the exact
+// tier is needed and EXACT is false; 2 on an internal overflow of the exact
+// tier (not reachable for binary64).
+template <bool EXACT = true>
+__host__ __device__ inline int sig_digits(double x, int P, const uint64_t *pow5, const int16_t *pow5s,
+                                          uint64_t &n, int &E) {
   union {
     double d;
     uint64_t u;
@@ -126,9 +133,10 @@ __host__ __device__ inline bool sig_digits(double x, int P, const uint64_t *pow5
     }
     if (up == 2) up = (I & 1) ? 1 : 0;
     if (up < 0) {
+      if (!EXACT) return 1;
       bool ok;
       const int cm = cmp_mid(mn, en, k, I, ok);
-      if (!ok) return false;
+      if (!ok) return 2;
       up = cm > 0 ? 1 : (cm < 0 ? 0 : (int)(I & 1));
     }
     n = I + (uint64_t)up;
@@ -137,16 +145,20 @@ __host__ __device__ inline bool sig_digits(double x, int P, const uint64_t *pow5
       n /= 10;  // 10^P -> 10^(P-1)
       E++;
     }
-    return true;
+    return 0;
   }
-  return false;
+  return 2;
 }
 
-// Writes exactly fmt_width(style) characters of x at out.
-__host__ __device__ inline bool fmt_value(double x, int style, const uint64_t *pow5, const int16_t *pow5s,
-                                          char *out) {
-  const int W = fmt_width(style), P = fmt_digits(style);
-  const bool p1 = style != FMT_E22_14;
+// Writes exactly fmt_width(style) characters of x at out.  Returns 0; 1
+// (nothing written) when the exact tier is needed and EXACT is false; 2 on
+// an internal overflow.  The fields have fixed layouts (sign slot, digits,
+// exponent), so every character goes to a position known at compile time.
+template <bool EXACT, int STYLE>
+__host__ __device__ inline int fmt_value_s(double x, const uint64_t *pow5, const int16_t *pow5s, char *out) {
+  constexpr int W = fmt_width(STYLE);
+  constexpr int P = fmt_digits(STYLE);
+  constexpr bool p1 = STYLE != FMT_E22_14;
   union {
     double d;
     uint64_t u;
@@ -154,54 +166,80 @@ __host__ __device__ inline bool fmt_value(double x, int style, const uint64_t *p
   cv.d = x;
   const bool neg = cv.u >> 63;
   const uint64_t mag = cv.u & 0x7fffffffffffffffull;
-  char t[32];
-  int len = 0;
   if (mag >= 0x7ff0000000000000ull) {  // Infinity / NaN, right-justified
-    const char *w = mag > 0x7ff0000000000000ull ? "NaN" : "Infinity";
-    if (mag == 0x7ff0000000000000ull && neg) t[len++] = '-';
-    for (; *w; w++) t[len++] = *w;
-  } else {
-    uint64_t n = 0;
-    int E = 0;
-    if (mag != 0) {
-      double ax;
-      cv.u = mag;
-      ax = cv.d;
-      if (!sig_digits(ax, P, pow5, pow5s, n, E)) return false;
-    }
-    char dg[16];
-    for (int i = P - 1; i >= 0; i--) {
-      dg[i] = (char)('0' + n % 10);
-      n /= 10;
-    }
-    if (neg) t[len++] = '-';
-    int ex;
-    if (p1) {  // d.ddddd, exponent E
-      t[len++] = dg[0];
-      t[len++] = '.';
-      for (int i = 1; i < P; i++) t[len++] = dg[i];
-      ex = mag ? E : 0;
-    } else {  // 0.dddd..., exponent E + 1
-      t[len++] = '0';
-      t[len++] = '.';
-      for (int i = 0; i < P; i++) t[len++] = dg[i];
-      ex = mag ? E + 1 : 0;
-    }
-    t[len++] = 'E';
-    t[len++] = ex < 0 ? '-' : '+';
-    const int a = ex < 0 ? -ex : ex;
-    t[len++] = (char)('0' + a / 100);
-    t[len++] = (char)('0' + (a / 10) % 10);
-    t[len++] = (char)('0' + a % 10);
+    const bool nan = mag > 0x7ff0000000000000ull;
+    const char *w = nan ? "NaN" : (neg ? "-Infinity" : "Infinity");
+    const int len = nan ? 3 : (neg ? 9 : 8);
+    for (int i = 0; i < W; i++) out[i] = i < W - len ? ' ' : w[i - (W - len)];
+    return 0;
   }
-  if (len > W) {  // the field overflows: W asterisks
+  if (STYLE == FMT_1P_E12_5 && neg) {  // no room for the sign: W asterisks
     for (int i = 0; i < W; i++) out[i] = '*';
-    return true;
+    return 0;
   }
-  const int pad = W - len;
-  for (int i = 0; i < pad; i++) out[i] = ' ';
-  for (int i = 0; i < len; i++) out[pad + i] = t[i];
-  return true;
+  uint64_t n = 0;
+  int E = 0;
+  if (mag != 0) {
+    cv.u = mag;
+    const int rc = sig_digits<EXACT>(cv.d, P, pow5, pow5s, n, E);
+    if (rc) return rc;
+  }
+  const int ex = mag ? (p1 ? E : E + 1) : 0;
+  // layout: [blank] [sign slot] lead digits, exponent
+  constexpr int B = STYLE == FMT_1P_E15_7 ? 1 : 0;  // leading blank (field wider than needed)
+  constexpr int S = STYLE == FMT_1P_E12_5 ? 0 : 1;  // sign slot width
+  if (B) out[0] = ' ';
+  if (S) out[B] = neg ? '-' : ' ';
+  // digits from two 32-bit halves (n < 10^14): 32-bit divisions by the
+  // constant 10 are a multiply-high and a shift, 64-bit ones are not
+  uint32_t lo7 = (uint32_t)(n % 10000000ull), hi7 = (uint32_t)(n / 10000000ull);
+  char dg[P];
+#pragma unroll
+  for (int i = P - 1; i >= 0; i--) {
+    if (i == P - 8) lo7 = hi7;
+    dg[i] = (char)('0' + lo7 % 10u);
+    lo7 /= 10u;
+  }
+  int o = B + S;
+  if (p1) {  // d.ddddd
+    out[o++] = dg[0];
+    out[o++] = '.';
+#pragma unroll
+    for (int i = 1; i < P; i++) out[o++] = dg[i];
+  } else {  // 0.dddd...
+    out[o++] = '0';
+    out[o++] = '.';
+#pragma unroll
+    for (int i = 0; i < P; i++) out[o++] = dg[i];
+  }
+  const uint32_t a = (uint32_t)(ex < 0 ? -ex : ex);
+  if (STYLE == FMT_1P_E15_7) {  // Ew.d: E+dd, or +ddd past 99
+    if (a <= 99u) {
+      out[o++] = 'E';
+      out[o++] = ex < 0 ? '-' : '+';
+    } else {
+      out[o++] = ex < 0 ? '-' : '+';
+      out[o++] = (char)('0' + a / 100u);
+    }
+    out[o++] = (char)('0' + (a / 10u) % 10u);
+    out[o++] = (char)('0' + a % 10u);
+    return 0;
+  }
+  out[o++] = 'E';
+  out[o++] = ex < 0 ? '-' : '+';
+  out[o++] = (char)('0' + a / 100u);
+  out[o++] = (char)('0' + (a / 10u) % 10u);
+  out[o++] = (char)('0' + a % 10u);
+  return 0;
+}
+
+template <bool EXACT = true>
+__host__ __device__ inline int fmt_value(double x, int style, const uint64_t *pow5, const int16_t *pow5s,
+                                         char *out) {
+  if (style == FMT_E22_14) return fmt_value_s<EXACT, FMT_E22_14>(x, pow5, pow5s, out);
+  if (style == FMT_1P_E12_5) return fmt_value_s<EXACT, FMT_1P_E12_5>(x, pow5, pow5s, out);
+  if (style == FMT_1P_E13_5) return fmt_value_s<EXACT, FMT_1P_E13_5>(x, pow5, pow5s, out);
+  return fmt_value_s<EXACT, FMT_1P_E15_7>(x, pow5, pow5s, out);
 }
 
 }  // namespace c2

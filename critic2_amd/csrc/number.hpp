@@ -248,9 +248,12 @@ __host__ __device__ __forceinline__ uint64_t pack_double(uint64_t m, int eb, int
 // T(q): 128-bit truncation of 5^q scaled into [2^127, 2^128), and its
 // binary scale s: 5^q ~= T * 2^s, T <= 5^q * 2^-s < T + 1.  pow5: (hi, lo)
 // per q from POW5_QMIN; pow5s: s per q.
-// Returns 0 (out set), or 2 if the exact tier ran out of big-integer room.
-__host__ __device__ inline int dec_to_double(const char *s, const DecNum &d, const uint64_t *pow5,
-                                             const int16_t *pow5s, double &out) {
+//
+// Tiers 1 and 2 (no access to the text): returns 0 with out set, or 3 when
+// the exact tier must decide, with the candidate significand m (floor of the
+// value at scale 2^eb, up to +-1) and eb.
+__host__ __device__ inline int dec_fast(const DecNum &d, const uint64_t *pow5, const int16_t *pow5s, double &out,
+                                        uint64_t &m_out, int &eb_out) {
   const int neg = d.neg;
   const uint64_t sign = neg ? (1ull << 63) : 0;
   if (d.w == 0) {
@@ -330,7 +333,16 @@ __host__ __device__ inline int dec_to_double(const char *s, const DecNum &d, con
     out = bits_to_double(pack_double(m, e2, neg) | 0);
     return 0;
   }
-  // tier 3: exact.  The candidate m is floor(value / 2^eb) up to +-1.
+  m_out = m;
+  eb_out = eb;
+  return 3;
+}
+
+// Tier 3: exact big-integer comparisons of the token's digits at s against
+// the candidate's neighbours and midpoint.  Returns 0, or 2 if the big
+// integers ran out of room.
+__host__ __device__ inline int dec_exact(const char *s, const DecNum &d, uint64_t m, int eb, double &out) {
+  const int neg = d.neg;
   const int q10 = d.exp10 - (d.dot >= 0 ? (d.mant_end - d.dot - 1) : 0);
   bool ok = true;
   uint64_t mm = m;
@@ -373,6 +385,15 @@ __host__ __device__ inline int dec_to_double(const char *s, const DecNum &d, con
   }
   out = bits_to_double(pack_double(mm, ebx, neg));
   return 0;
+}
+
+// Tiers 1-3.  Returns 0 (out set), or 2 if the exact tier ran out of room.
+__host__ __device__ inline int dec_to_double(const char *s, const DecNum &d, const uint64_t *pow5,
+                                             const int16_t *pow5s, double &out) {
+  uint64_t m;
+  int eb;
+  if (dec_fast(d, pow5, pow5s, out, m, eb) == 0) return 0;
+  return dec_exact(s, d, m, eb, out);
 }
 
 // ---- host: the 5^q table used by tier 2

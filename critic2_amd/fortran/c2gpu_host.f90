@@ -6,10 +6,12 @@
 ! integrated properties.
 !
 ! Input file (text): n1 n2 n3 / x2c(3,3) column by column / nat / xat(3,i)
-! per atom, then the grid as an unformatted stream file next to it.
-! Usage: c2gpu_host <case.txt> <grid.bin> <labels.bin> <props.txt>
+! per atom, then the grid as an unformatted stream file next to it, or as a
+! Gaussian cube (a name ending in .cube) read through c2gpu_cube_read, the
+! drop-in for grid3%read_cube.
+! Usage: c2gpu_host <case.txt> <grid.bin|grid.cube> <labels.bin> <props.txt>
 program c2gpu_host
-  use iso_c_binding, only: c_ptr, c_int, c_int32_t, c_int64_t, c_double, c_loc, c_null_ptr
+  use iso_c_binding, only: c_ptr, c_int, c_int32_t, c_int64_t, c_double, c_loc, c_null_ptr, c_null_char
   use c2gpu
   implicit none
   character(len=512) :: fcase, fgrid, flab, fprop
@@ -22,7 +24,8 @@ program c2gpu_host
   type(c_ptr) :: fields(1)
   type(c2gpu_attractor_opts) :: opts
   type(c2gpu_bader_stats) :: stats
-  integer :: lu, i
+  type(c2gpu_cube_header) :: hdr
+  integer :: lu, i, lg
   integer(c_int64_t) :: npts
 
   call get_command_argument(1, fcase)
@@ -38,12 +41,18 @@ program c2gpu_host
   read(lu,*) xat
   close(lu)
   allocate(f(n(1),n(2),n(3)), idg(n(1),n(2),n(3)))
-  open(newunit=lu, file=trim(fgrid), access='stream', form='unformatted', status='old')
-  read(lu) f
-  close(lu)
-
   ier = c2gpu_ctx_create(ctx, 0_c_int)
   if (ier /= C2GPU_OK) call die("c2gpu_ctx_create", ier)
+  lg = len_trim(fgrid)
+  if (lg > 5 .and. fgrid(max(lg-4,1):lg) == '.cube') then
+     ier = c2gpu_cube_read(ctx, trim(fgrid) // c_null_char, 0_c_int, hdr, f)
+     if (ier /= C2GPU_OK) call die("c2gpu_cube_read", ier)
+     if (any(hdr%n /= n)) call die("cube grid size", 1_c_int)
+  else
+     open(newunit=lu, file=trim(fgrid), access='stream', form='unformatted', status='old')
+     read(lu) f
+     close(lu)
+  end if
   ier = c2gpu_bader_metrics(x2c, n, lat2car, car2lat, lat_i_dist)
   if (ier /= C2GPU_OK) call die("c2gpu_bader_metrics", ier)
 

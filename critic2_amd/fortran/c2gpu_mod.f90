@@ -30,8 +30,21 @@ module c2gpu
      real(c_double) :: ms_traj, ms_sweep, ms_refine, ms_total
   end type c2gpu_bader_stats
 
+  integer(c_int), parameter, public :: C2GPU_CUBE_E22_14 = 0, C2GPU_CUBE_1P_E12_5 = 1, &
+     C2GPU_CUBE_1P_E13_5 = 2
+
+  ! header of a cube file (grid3mod@proc.f90:533-551)
+  type, bind(c), public :: c2gpu_cube_header
+     integer(c_int) :: nat, ismo
+     integer(c_int) :: n(3)
+     real(c_double) :: x0(3)
+     real(c_double) :: step(9)
+     integer(c_int64_t) :: data_offset
+  end type c2gpu_cube_header
+
   public :: c2gpu_ctx_create, c2gpu_ctx_destroy, c2gpu_bader_metrics, c2gpu_bader
   public :: c2gpu_basin_sums, c2gpu_fft_deriv, c2gpu_relabel
+  public :: c2gpu_cube_read_header, c2gpu_cube_read, c2gpu_bincube_read, c2gpu_cube_write
 
   interface
      function c2gpu_ctx_create(ctx, device) bind(c, name="c2gpu_ctx_create")
@@ -107,6 +120,47 @@ module c2gpu
        real(c_double), intent(out) :: out(*)
        integer(c_int) :: c2gpu_fft_deriv
      end function c2gpu_fft_deriv
+
+     function c2gpu_cube_read_header(ctx, path, binary, hdr) bind(c, name="c2gpu_cube_read_header")
+       import :: c_ptr, c_int, c_char, c2gpu_cube_header
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: binary
+       type(c2gpu_cube_header), intent(out) :: hdr
+       integer(c_int) :: c2gpu_cube_read_header
+     end function c2gpu_cube_read_header
+
+     function c2gpu_cube_read(ctx, path, flags, hdr, f) bind(c, name="c2gpu_cube_read")
+       import :: c_ptr, c_int, c_char, c_double, c2gpu_cube_header
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: flags
+       type(c2gpu_cube_header), intent(out) :: hdr
+       real(c_double), intent(out) :: f(*)
+       integer(c_int) :: c2gpu_cube_read
+     end function c2gpu_cube_read
+
+     function c2gpu_bincube_read(ctx, path, flags, hdr, f) bind(c, name="c2gpu_bincube_read")
+       import :: c_ptr, c_int, c_char, c_double, c2gpu_cube_header
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: flags
+       type(c2gpu_cube_header), intent(out) :: hdr
+       real(c_double), intent(out) :: f(*)
+       integer(c_int) :: c2gpu_bincube_read
+     end function c2gpu_bincube_read
+
+     function c2gpu_cube_write(ctx, path, hdr, zat, xat, f, style, flags) bind(c, name="c2gpu_cube_write")
+       import :: c_ptr, c_int, c_char, c_double, c2gpu_cube_header
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       type(c2gpu_cube_header), intent(in) :: hdr
+       integer(c_int), intent(in) :: zat(*)
+       real(c_double), intent(in) :: xat(*)
+       real(c_double), intent(in) :: f(*)
+       integer(c_int), value :: style, flags
+       integer(c_int) :: c2gpu_cube_write
+     end function c2gpu_cube_write
   end interface
 
 end module c2gpu

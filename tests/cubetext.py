@@ -92,10 +92,11 @@ def bincube(f_xfast, n, step, x0=(0.0, 0.0, 0.0), atoms=(), ismo=False, split=No
 
 def fortran_e(x: float, style: int) -> str:
     """One field of the cube body formats with Python's correctly rounded
-    formatting: style 0 E22.14E3, 1 1p,E12.5E3, 2 1p,e13.5e3."""
+    formatting: style 0 E22.14E3, 1 1p,E12.5E3, 2 1p,e13.5e3, 3 1p,E15.7
+    (two-digit exponent; past 99 the letter is dropped)."""
     import math
-    W = {0: 22, 1: 12, 2: 13}[style]
-    P = 14 if style == 0 else 6
+    W = {0: 22, 1: 12, 2: 13, 3: 15}[style]
+    P = {0: 14, 3: 8}.get(style, 6)
     if math.isnan(x):
         t = "NaN"
     elif math.isinf(x):
@@ -107,6 +108,8 @@ def fortran_e(x: float, style: int) -> str:
         digits = mant.replace(".", "")
         e = 0 if x == 0 else int(ex) + (1 if style == 0 else 0)
         es = f"E{'-' if e < 0 else '+'}{abs(e):03d}"
+        if style == 3:
+            es = f"E{'-' if e < 0 else '+'}{abs(e):02d}" if abs(e) <= 99 else f"{'-' if e < 0 else '+'}{abs(e):03d}"
         body = ("0." + digits if style == 0 else digits[0] + "." + digits[1:]) + es
         t = ("-" if neg else "") + body
     return t.rjust(W) if len(t) <= W else "*" * W

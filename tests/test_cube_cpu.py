@@ -187,9 +187,9 @@ def _fmt_values(seed, count):
 @pytest.mark.parametrize("seed", [1, 2])
 def test_formatting_matches_python(number_check, seed):
     vals = _fmt_values(seed, 20000)
-    lines = [f"{st} {struct.unpack('<Q', struct.pack('<d', v))[0]:x}" for v in vals for st in (0, 1, 2)]
+    lines = [f"{st} {struct.unpack('<Q', struct.pack('<d', v))[0]:x}" for v in vals for st in (0, 1, 2, 3)]
     got = number_check(lines, mode="fmt")
-    want = [fortran_e(v, st) for v in vals for st in (0, 1, 2)]
+    want = [fortran_e(v, st) for v in vals for st in (0, 1, 2, 3)]
     bad = [(l, g, w) for l, g, w in zip(lines, got, want) if g != w]
     assert not bad, bad[:5]
 
