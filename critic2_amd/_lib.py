@@ -34,7 +34,7 @@ EXPORTS = (
     "c2gpu_bader_metrics", "c2gpu_bader_step_pointers", "c2gpu_bader", "c2gpu_relabel",
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
-    "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write",
+    "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff",
 )
 
 
@@ -63,6 +63,11 @@ class BaderStats(C.Structure):
 class CubeHeader(C.Structure):
     _fields_ = [("nat", C.c_int), ("ismo", C.c_int), ("n", C.c_int * 3), ("x0", C.c_double * 3),
                 ("step", C.c_double * 9), ("data_offset", C.c_uint64)]
+
+
+class NciCut(C.Structure):
+    _fields_ = [("rhocut", C.c_double), ("dimcut", C.c_double), ("rhoplot", C.c_double),
+                ("srhorange", C.c_double * 2), ("onlyneg", C.c_int), ("rho_void", C.c_double)]
 
 
 class RadialTable(C.Structure):
@@ -116,6 +121,8 @@ def load():
         "c2gpu_bincube_read": [vp, C.c_char_p, i, C.POINTER(CubeHeader), vp],
         "c2gpu_cube_format": [vp, vp, vp, i, i, i, vp, C.c_uint64, C.POINTER(C.c_uint64)],
         "c2gpu_cube_write": [vp, C.c_char_p, C.POINTER(CubeHeader), vp, vp, vp, i, i],
+        "c2gpu_nci_cutoff": [vp, vp, vp, vp, vp, i, C.POINTER(NciCut), i, vp, C.c_uint64, C.POINTER(C.c_uint64),
+                             vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

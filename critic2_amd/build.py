@@ -24,6 +24,7 @@ SOURCES = [
     ("nci.hip", ["-ffp-contract=fast"]),
     ("ncigrid.hip", ["-ffp-contract=off"]),
     ("cube.hip", ["-ffp-contract=off"]),
+    ("ncicut.hip", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
 ]

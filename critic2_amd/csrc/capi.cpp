@@ -28,6 +28,8 @@ void cube_header(const char *, c2gpu_cube_header *, int);
 void cube_parse_run(c2gpu_ctx *, const char *, uint64_t, int, c2gpu_cube_header *, double *);
 void cube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
 void bincube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
+void nci_cutoff_run(c2gpu_ctx *, const void *, void *, const void *, const int *, int, const c2gpu_nci_cut *, int,
+                    char *, uint64_t, uint64_t *, double *);
 void cube_format_run(c2gpu_ctx *, const double *, const int *, int, int, int, char *, uint64_t, uint64_t *);
 void cube_write_run(c2gpu_ctx *, const char *, const c2gpu_cube_header *, const int *, const double *,
                     const double *, int, int);
@@ -483,6 +485,12 @@ int c2gpu_cube_format(c2gpu_ctx *ctx, const double *f, const int n[3], int style
 int c2gpu_cube_write(c2gpu_ctx *ctx, const char *path, const c2gpu_cube_header *hdr, const int *zat,
                      const double *xat, const double *f, int style, int flags) {
   return guarded(ctx, [&] { cube_write_run(ctx, path, hdr, zat, xat, f, style, flags); });
+}
+
+int c2gpu_nci_cutoff(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *rhoat, const int n[3],
+                     int precision, const c2gpu_nci_cut *opts, int flags, char *dat, uint64_t cap,
+                     uint64_t *len, double sums[3]) {
+  return guarded(ctx, [&] { nci_cutoff_run(ctx, crho, cgrad, rhoat, n, precision, opts, flags, dat, cap, len, sums); });
 }
 
 }  // extern "C"

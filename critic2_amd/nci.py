@@ -84,16 +84,44 @@ def nciplot_promol(ctx: Context, n, x2c, tables: list[Grid1], xat, species, prec
     return res
 
 
-def cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e30, 1e30),
-            onlyneg=False):
-    """Cutoff pass of nci@proc.f90:615-643 (no fragments): returns the
-    (rho, s) rows of the .dat file and cgrad with 100 outside the plot range."""
-    crho = np.asarray(crho, dtype=np.float64)
-    cg = np.array(cgrad, dtype=np.float64, copy=True)
-    sel = (np.abs(crho) < rhocut * 100.0) & (cg < dimcut)
-    dat = np.stack([crho[sel] / 100.0, cg[sel]], axis=1)
-    cg[np.abs(crho) > rhoplot * 100.0] = 100.0
-    cg[(crho < srhorange[0] * 100.0) | (crho > srhorange[1] * 100.0)] = 100.0
-    if onlyneg:
-        cg[crho > 0] = 100.0
-    return dat, cg
+def cutoffs(ctx: Context, crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e30, 1e30),
+            onlyneg=False, rhoat=None, rho_void=-1.0, dat=True):
+    """nciplot's cutoff pass (nci@proc.f90:612-654, no fragments) on the GPU
+    through c2gpu_nci_cutoff.  crho, cgrad (and rhoat with rho_void > 0):
+    z-fastest arrays of one type (float64 or float32), host numpy arrays or
+    DeviceArrays.  cgrad is masked in place (s <- 100).  Returns (dat_text,
+    sums): the .dat rows '(1p,E15.7,E15.7)' as bytes (None if dat is False)
+    and the unscaled void sums (charge, promolecular charge, node count)."""
+    import ctypes as C
+    dev = hasattr(crho, "data_ptr")
+    dt = np.dtype(crho.dtype)
+    prec = _lib.FP64 if dt == np.float64 else _lib.FP32
+    if not dev:
+        if not (isinstance(cgrad, np.ndarray) and cgrad.dtype == dt and cgrad.flags.c_contiguous):
+            raise ValueError("cgrad must be a contiguous array of crho's type (it is masked in place)")
+        crho = np.ascontiguousarray(crho)
+        if rhoat is not None:
+            rhoat = np.ascontiguousarray(rhoat, dtype=dt)
+    npts = int(np.prod(crho.shape))
+    n = np.array([npts, 1, 1], dtype=np.int32)
+    o = _lib.NciCut(rhocut, dimcut, rhoplot, (C.c_double * 2)(*srhorange), int(bool(onlyneg)), rho_void)
+    flags = _lib.PTR_DEVICE if dev else 0
+    ln = C.c_uint64()
+    sums = np.zeros(3)
+    args = (ctx.h, _lib.ptr(crho), _lib.ptr(cgrad), _lib.ptr(rhoat), n.ctypes.data, prec, C.byref(o), flags)
+    ctx.check(ctx.lib.c2gpu_nci_cutoff(*args, None, 0, C.byref(ln), None), "c2gpu_nci_cutoff")
+    size = max(ln.value, 1)
+    buf = ctx.empty((size,), np.uint8) if dev else np.empty(size, dtype=np.uint8)
+    ctx.check(ctx.lib.c2gpu_nci_cutoff(*args, _lib.ptr(buf), size, C.byref(ln), sums.ctypes.data),
+              "c2gpu_nci_cutoff")
+    text = None
+    if dat:
+        text = (buf.numpy() if dev else buf)[:ln.value].tobytes()
+    return text, sums
+
+
+def dat_rows(text: bytes) -> np.ndarray:
+    """The (rho, s) pairs of .dat text."""
+    if not text:
+        return np.zeros((0, 2))
+    return np.array(text.split(), dtype=np.float64).reshape(-1, 2)

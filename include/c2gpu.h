@@ -232,6 +232,31 @@ int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const doubl
                   int scheme, int precision, const c2gpu_promol *promol, int zfast,
                   int flags, float *crho_out, float *cgrad_out);
 
+/* Options of nciplot's cutoff pass (nci@proc.f90:612-643; keywords
+ * CUTOFFS rhocut dimcut, CUTPLOT rhoplot, SRHORANGE, ONLYNEG, RHO_VOID). */
+typedef struct {
+  double rhocut, dimcut;  /* .dat rows: |rho| < rhocut and s < dimcut       */
+  double rhoplot;         /* s <- 100 where |rho| > rhoplot                 */
+  double srhorange[2];    /* s <- 100 where sign(l2) rho is outside          */
+  int onlyneg;            /* s <- 100 where sign(l2) rho > 0                 */
+  double rho_void;        /* > 0: only nodes with promolecular density below
+                             it count; the void sums are returned           */
+} c2gpu_nci_cut;
+
+/* nciplot's cutoff pass (nci@proc.f90:612-654, without fragments) over crho
+ * (sign(l2) rho * 100) and cgrad (s) as c2gpu_nci returns them, z fastest:
+ * the .dat rows '(1p,E15.7,E15.7)' rho, s of the selected nodes in node
+ * order (31 bytes each) into dat, then cgrad <- 100 where masked (in place).
+ * rhoat: promolecular density (needed with rho_void > 0).  precision:
+ * C2GPU_FP64 or C2GPU_FP32, the type of the three arrays.  *len receives
+ * the .dat byte count; dat NULL only queries it (nothing is modified).
+ * sums (may be NULL): unscaled void charge, promolecular charge and node
+ * count (critic2 multiplies them by Omega/N).  With C2GPU_PTR_DEVICE the
+ * arrays and dat are device memory. */
+int c2gpu_nci_cutoff(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *rhoat, const int n[3],
+                     int precision, const c2gpu_nci_cut *opts, int flags, char *dat, uint64_t cap,
+                     uint64_t *len, double sums[3]);
+
 /* ------------------------------------------------------------------ */
 /* Batches of cells                                                    */
 /* ------------------------------------------------------------------ */

@@ -345,3 +345,25 @@ def read_cube(path):
 def read_bincube(path):
     """grid3%read_bincube restated: (header, f x-fastest)."""
     return _cube_call("oracle_read_bincube", path)
+
+
+def nci_cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e30, 1e30), onlyneg=False,
+                rhoat=None, rho_void=-1.0):
+    """Cutoff pass of nci@proc.f90:612-643 (no fragments), restated with
+    numpy: the (rho, s) rows of the .dat file in node order, cgrad with 100
+    outside the plot range, and the unscaled void sums."""
+    crho = np.asarray(crho, dtype=np.float64).reshape(-1)
+    cg = np.array(cgrad, dtype=np.float64, copy=True).reshape(-1)
+    inter = np.ones(crho.shape, dtype=bool)
+    sums = np.zeros(3)
+    if rho_void > 0:
+        ra = np.asarray(rhoat, dtype=np.float64).reshape(-1)
+        inter = ra < rho_void
+        sums = np.array([np.sum(np.abs(crho[inter]) / 100.0), np.sum(ra[inter]), float(np.count_nonzero(inter))])
+    sel = (np.abs(crho) < rhocut * 100.0) & (cg < dimcut) & inter
+    dat = np.stack([crho[sel] / 100.0, cg[sel]], axis=1)
+    cg[(np.abs(crho) > rhoplot * 100.0) | ~inter] = 100.0
+    cg[(crho < srhorange[0] * 100.0) | (crho > srhorange[1] * 100.0)] = 100.0
+    if onlyneg:
+        cg[crho > 0] = 100.0
+    return dat, cg, sums

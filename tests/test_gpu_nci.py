@@ -2,6 +2,7 @@
 golden urea values and the oracle; FP32 against FP64 (error reported);
 FFT scheme against the oracle's grid3%fft restatement."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -12,7 +13,8 @@ from critic2_amd import nci as gn
 from critic2_amd.radial import slater_grid1
 from critic2_amd.synth import config, random_cell
 from oracle import oracle
-from tests.helpers import urea, fmt6, images, node_points
+from tests.cubetext import fortran_e
+from tests.helpers import GOLDEN, urea, fmt6, images, node_points
 
 pytestmark = pytest.mark.gpu
 
@@ -22,9 +24,20 @@ def test_urea_golden_on_gpu_fp64(ctx):
     d, x2c, tables, xat, species = urea()
     crho, s = gn.nciplot_promol(ctx, d["nstep"], x2c, tables, xat, species, precision="fp64")
     o = d["options"]
-    dat, cg = gn.cutoffs(crho, s, rhocut=o["rhocut"], dimcut=o["dimcut"], rhoplot=o["rhoplot"],
+    cg = s.copy()
+    text, _ = gn.cutoffs(ctx, crho, cg, rhocut=o["rhocut"], dimcut=o["dimcut"], rhoplot=o["rhoplot"],
                          srhorange=o["srhorange"])
+    dat = gn.dat_rows(text)
     assert [fmt6(v) for v in crho] == d["dens_cube"]
+    # the .dat file of the reference: the density column byte for byte, the
+    # RDG column where it is not round-off noise at a symmetric node
+    ref_lines = open(os.path.join(GOLDEN, "cube", "005_nciplot_basic-gen.dat"), "rb").read().splitlines()
+    got_lines = text.splitlines()
+    assert len(got_lines) == len(ref_lines)
+    for g, r in zip(got_lines, ref_lines):
+        assert g[:15] == r[:15]
+        if float(r[15:]) > 1e-10:
+            assert g[15:] == r[15:]
     for v, ref in zip(cg, d["grad_cube"]):
         if abs(ref) > 1e-10:
             assert fmt6(v) == ref
@@ -138,3 +151,35 @@ def test_nci_grid_scheme_rejects_fp32_and_tiny_grids(ctx):
         gn._run(ctx, f, cell.n, cell.x2c, _lib.NCI_GRID, _lib.FP32, None, True, False)
     with pytest.raises(_lib.C2GpuError):
         gn.nciplot_grid(ctx, np.ones(8), (8, 1, 1), cell.x2c, interpolation="grid")
+
+
+@pytest.mark.parametrize("dtype,device,opts", [
+    (np.float64, False, {}),
+    (np.float64, True, {"onlyneg": True, "srhorange": (-0.3, 0.2)}),
+    (np.float32, False, {"rho_void": 0.05, "rhoplot": 0.2}),
+    (np.float32, True, {"rho_void": 0.02, "rhocut": 0.1, "dimcut": 1.0}),
+])
+def test_cutoff_pass_matches_oracle(ctx, dtype, device, opts):
+    """c2gpu_nci_cutoff against the oracle's restatement: the same rows in
+    node order (each field the correctly rounded 1p,E15.7 of the oracle's
+    value), the same masked RDG, the void sums."""
+    rng = np.random.default_rng(3)
+    npts = 20000
+    crho = (rng.standard_normal(npts) * 10.0 ** rng.uniform(-5, 1.5, npts)).astype(dtype)
+    cg = (rng.uniform(0, 3, npts) * 10.0 ** rng.uniform(-3, 0.5, npts)).astype(dtype)
+    rhoat = (10.0 ** rng.uniform(-4, 0, npts)).astype(dtype)
+    kw = {"rhocut": 0.2, "dimcut": 2.0, "rhoplot": 0.1, "srhorange": (-1e30, 1e30), "onlyneg": False,
+          "rho_void": -1.0}
+    kw.update(opts)
+    dat, cgo, sums_o = oracle.nci_cutoffs(crho.astype(np.float64), cg.astype(np.float64), rhoat=rhoat, **kw)
+    want = "".join(fortran_e(float(a), 3) + fortran_e(float(b), 3) + "\n" for a, b in dat).encode()
+    g = cg.copy()
+    if device:
+        dc, dg, da = ctx.to_device(crho), ctx.to_device(g), ctx.to_device(rhoat)
+        text, sums = gn.cutoffs(ctx, dc, dg, rhoat=da, **kw)
+        g = dg.numpy()
+    else:
+        text, sums = gn.cutoffs(ctx, crho, g, rhoat=rhoat, **kw)
+    assert text == want
+    assert np.array_equal(g.astype(np.float64), cgo)
+    np.testing.assert_allclose(sums, sums_o, rtol=1e-12)

@@ -6,7 +6,6 @@ import pytest
 
 from oracle import oracle
 from critic2_amd import radial
-from critic2_amd.nci import cutoffs
 from tests.helpers import urea, fmt6, images, node_points
 
 
@@ -20,8 +19,8 @@ def test_urea_promolecular_nci_matches_reference_golden():
     out = oracle.promol_nci(pts, pos, species[idx], otabs)
     crho, s = out[:, 3], out[:, 2]
     o = d["options"]
-    dat, cg = cutoffs(crho, s, rhocut=o["rhocut"], dimcut=o["dimcut"], rhoplot=o["rhoplot"],
-                      srhorange=o["srhorange"])
+    dat, cg, _ = oracle.nci_cutoffs(crho, s, rhocut=o["rhocut"], dimcut=o["dimcut"], rhoplot=o["rhoplot"],
+                                    srhorange=o["srhorange"])
     assert [fmt6(v) for v in crho] == d["dens_cube"]
     # RDG: exact digits where the value is not round-off noise at a symmetric node
     for v, ref in zip(cg, d["grad_cube"]):
