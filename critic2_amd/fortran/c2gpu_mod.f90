@@ -45,6 +45,17 @@ module c2gpu
   public :: c2gpu_ctx_create, c2gpu_ctx_destroy, c2gpu_bader_metrics, c2gpu_bader
   public :: c2gpu_basin_sums, c2gpu_fft_deriv, c2gpu_relabel
   public :: c2gpu_cube_read_header, c2gpu_cube_read, c2gpu_bincube_read, c2gpu_cube_write
+  public :: c2gpu_nci_cutoff
+
+  integer(c_int), parameter, public :: C2GPU_FP64 = 0, C2GPU_FP32 = 1
+
+  ! nciplot cutoff options (nci@proc.f90:612-643)
+  type, bind(c), public :: c2gpu_nci_cut
+     real(c_double) :: rhocut, dimcut, rhoplot
+     real(c_double) :: srhorange(2)
+     integer(c_int) :: onlyneg
+     real(c_double) :: rho_void
+  end type c2gpu_nci_cut
 
   interface
      function c2gpu_ctx_create(ctx, device) bind(c, name="c2gpu_ctx_create")
@@ -161,6 +172,24 @@ module c2gpu
        integer(c_int), value :: style, flags
        integer(c_int) :: c2gpu_cube_write
      end function c2gpu_cube_write
+
+     function c2gpu_nci_cutoff(ctx, crho, cgrad, rhoat, n, precision, opts, flags, dat, cap, len, sums) &
+        bind(c, name="c2gpu_nci_cutoff")
+       import :: c_ptr, c_int, c_int64_t, c_double, c_char, c2gpu_nci_cut
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: crho(*)
+       real(c_double), intent(inout) :: cgrad(*)
+       real(c_double), intent(in) :: rhoat(*)
+       integer(c_int), intent(in) :: n(3)
+       integer(c_int), value :: precision
+       type(c2gpu_nci_cut), intent(in) :: opts
+       integer(c_int), value :: flags
+       character(kind=c_char), intent(out) :: dat(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: len
+       real(c_double), intent(out) :: sums(3)
+       integer(c_int) :: c2gpu_nci_cutoff
+     end function c2gpu_nci_cutoff
   end interface
 
 end module c2gpu
