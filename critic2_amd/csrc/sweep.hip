@@ -68,6 +68,9 @@ __device__ __forceinline__ uint32_t event_start(const BParams &P, uint64_t e) {
 }
 
 // The per-point event lists.
+// Lists longer than this are filtered / scanned by a whole workgroup.
+constexpr uint32_t LIST_LONG = 512;
+
 struct Lists {
   uint64_t *off;
   uint32_t *cnt;
@@ -565,10 +568,13 @@ __global__ void __launch_bounds__(256) k_patch_keep(BParams P, uint32_t nd, cons
                                                     const uint32_t *__restrict__ alloc,
                                                     const uint64_t *__restrict__ aoff, uint64_t base, Lists L,
                                                     const uint8_t *__restrict__ chg, uint32_t *__restrict__ cur,
-                                                    int *err, unsigned long long *work) {
+                                                    uint32_t *__restrict__ longq, uint32_t *nlong, int *err,
+                                                    unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t nold = 0, nkept = 0;
-  if (t < nd) {
+  const bool lng = t < nd && L.cnt[dl[t]] > LIST_LONG;
+  wave_append(lng, t, longq, nlong);  // a workgroup each (k_patch_keep_long)
+  if (t < nd && !lng) {
     const uint32_t y = dl[t];
     const uint64_t ob = L.off[y], oe = ob + L.cnt[y];
     const uint32_t nn = ncnt[t];
@@ -599,6 +605,82 @@ __global__ void __launch_bounds__(256) k_patch_keep(BParams P, uint32_t nd, cons
   wave_add(work + 1, nkept);
 }
 
+// Exclusive prefix of pred over the workgroup (256 threads) and its total;
+// every thread must call it.
+__device__ __forceinline__ uint32_t block_prefix(bool pred, uint32_t *lds4, uint32_t &total) {
+  const uint64_t m = __ballot(pred);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds4[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t before = 0;
+  total = 0;
+  for (int i = 0; i < 4; i++) {
+    if (i < w) before += lds4[i];
+    total += lds4[i];
+  }
+  __syncthreads();
+  return before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// k_patch_keep for the long lists, one workgroup per list at a time: the
+// list is filtered in chunks of 256 events (coalesced reads, one flag gather
+// each, a workgroup prefix for the write positions).  In place the writes of
+// a chunk land at or below its own positions, which every lane has read
+// before the prefix's barrier, and below the next chunk's.
+__global__ void __launch_bounds__(256) k_patch_keep_long(BParams P, const uint32_t *__restrict__ longq,
+                                                         const uint32_t *nlong, const uint32_t *__restrict__ dl,
+                                                         const uint32_t *__restrict__ ncnt,
+                                                         const uint32_t *__restrict__ alloc,
+                                                         const uint64_t *__restrict__ aoff, uint64_t base, Lists L,
+                                                         const uint8_t *__restrict__ chg,
+                                                         uint32_t *__restrict__ cur, int *err,
+                                                         unsigned long long *work) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t nl = *nlong;
+  for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    const uint32_t t = longq[q];
+    const uint32_t y = dl[t];
+    const uint64_t ob = L.off[y], oe = ob + L.cnt[y];
+    const uint32_t nn = ncnt[t];
+    const bool move = alloc[t] != 0;
+    const uint64_t nb = move ? base + aoff[t] : ob;
+    const uint32_t room = move ? alloc[t] : L.cap[y];
+    const uint32_t lim = nn <= room ? nn : room;
+    uint32_t kept = 0;
+    for (uint64_t c = ob; c < oe; c += 256) {
+      const uint64_t x = c + threadIdx.x;
+      uint64_t v = 0;
+      bool keep = false;
+      if (x < oe) {
+        v = L.ev[x];
+        keep = !chg[event_start(P, v)];
+      }
+      uint32_t tot;
+      const uint32_t pos = kept + block_prefix(keep, lds4, tot);
+      if (keep) {
+        if (pos < lim)
+          L.ev[nb + pos] = v;
+        else
+          atomicOr(err, 4);
+      }
+      kept += tot;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (nn > room) atomicOr(err, 4);
+      cur[y] = kept;
+      L.off[y] = nb;
+      L.cnt[y] = lim;
+      if (move) L.cap[y] = room;
+      if (work) {
+        atomicAdd(work, (unsigned long long)(oe - ob));
+        atomicAdd(work + 1, (unsigned long long)kept);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Points whose known==2 time may change: the rewritten points and their face
 // neighbours.
 __global__ void k_mark_kt(BParams P, uint32_t nd, const uint32_t *__restrict__ dl, uint8_t *__restrict__ qflag) {
@@ -624,19 +706,39 @@ __global__ void k_mark_kt(BParams P, uint32_t nd, const uint32_t *__restrict__ d
 __global__ void __launch_bounds__(256) k_mark_decide(BParams P, uint32_t nq, const uint32_t *__restrict__ ql,
                                                      Lists L, const uint8_t *__restrict__ dflag,
                                                      const uint8_t *__restrict__ ktchg, uint8_t *__restrict__ sflag,
+                                                     uint32_t *__restrict__ longq, uint32_t *nlong,
                                                      unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t scanned = 0, marked = 0;
   const uint32_t x = t < nq ? ql[t] : 0u;
-  if (t < nq && (dflag[x] || ktchg[x])) {
+  const bool act = t < nq && (dflag[x] || ktchg[x]);
+  const bool lng = act && L.cnt[x] > LIST_LONG;
+  wave_append(lng, x, longq, nlong);  // a workgroup each (k_mark_decide_long)
+  if (act) {
     sflag[x] = 1;
-    const uint64_t b = L.b(x), e = L.e(x);
-    for (uint64_t q = b; q < e; q++) sflag[event_start(P, L.ev[q])] = 1;
-    scanned = e - b;
     marked = 1;
+    if (!lng) {
+      const uint64_t b = L.b(x), e = L.e(x);
+      for (uint64_t q = b; q < e; q++) sflag[event_start(P, L.ev[q])] = 1;
+      scanned = e - b;
+    }
   }
   wave_add(work, scanned);
   wave_add(work + 1, marked);
+}
+
+__global__ void __launch_bounds__(256) k_mark_decide_long(BParams P, const uint32_t *__restrict__ longq,
+                                                          const uint32_t *nlong, Lists L,
+                                                          uint8_t *__restrict__ sflag, unsigned long long *work) {
+  const uint32_t nl = *nlong;
+  uint64_t scanned = 0;
+  for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    const uint32_t x = longq[q];
+    const uint64_t b = L.b(x), e = L.e(x);
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) sflag[event_start(P, L.ev[i])] = 1;
+    scanned += (e - b + blockDim.x - 1 - threadIdx.x) / blockDim.x;
+  }
+  wave_add(work, scanned);
 }
 
 __global__ void k_apply(uint32_t nc, const uint32_t *__restrict__ chlist, int32_t *__restrict__ ell,
@@ -740,7 +842,9 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   uint32_t *ncnt = buf<uint32_t>(ctx, "sweep_ncnt", N);
   uint32_t *alloc = buf<uint32_t>(ctx, "sweep_alloc", (size_t)N + 1);
   uint64_t *aoff = buf<uint64_t>(ctx, "sweep_aoff", (size_t)N + 1);
-  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 4);  // nch, ndl, nql, nsl
+  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 6);  // nch, ndl, nql, nsl, long lists x2
+  uint32_t *longq = buf<uint32_t>(ctx, "sweep_longq", N);
+  const unsigned gLong = (unsigned)persistent_threads(ctx->device, 256) / 256 / 2;  // 4 workgroups per CU
   int *d_err = buf<int>(ctx, "sweep_err", 1);
   unsigned long long *d_stats = buf<unsigned long long>(ctx, "sweep_stats", 5);
   unsigned long long *d_work = buf<unsigned long long>(ctx, "sweep_work", 16);  // algorithmic work counters
@@ -830,8 +934,11 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       continue;
     }
     // 2. rewrite those lists; add the changed starts' events
-    C2_LAUNCH(ctx, k_patch_keep, dim3(gD), dim3(B), 0, P, nd, dl, ncnt, alloc, aoff, pool_top, L, chg, cur, d_err,
-              d_work + 0);
+    C2_HIP(hipMemsetAsync(d_n + 4, 0, 2 * sizeof(uint32_t), st));
+    C2_LAUNCH(ctx, k_patch_keep, dim3(gD), dim3(B), 0, P, nd, dl, ncnt, alloc, aoff, pool_top, L, chg, cur, longq,
+              d_n + 4, d_err, d_work + 0);
+    C2_LAUNCH(ctx, k_patch_keep_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 4, dl, ncnt, alloc, aoff, pool_top,
+              L, chg, cur, d_err, d_work + 0);
     C2_LAUNCH(ctx, k_ev_fill, dim3(gC), dim3(B), 0, P, nc, chlist, toff, tp, ell2, lam2, L, cur, d_err, d_work + 4);
     s_nd += nd;
     C2_LAUNCH(ctx, k_apply, dim3(gC), dim3(B), 0, nc, chlist, ell, lam, ell2, lam2, chg);
@@ -843,7 +950,9 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     const unsigned gQ = grid_for(nq, B);
     C2_LAUNCH(ctx, k_kentry, dim3(gQ), dim3(B), 0, P, nq, ql, L, uni, Kt, ktchg);
     // 4. starts whose inputs changed decide again
-    C2_LAUNCH(ctx, k_mark_decide, dim3(gQ), dim3(B), 0, P, nq, ql, L, dflag, ktchg, sflag, d_work + 2);
+    C2_LAUNCH(ctx, k_mark_decide, dim3(gQ), dim3(B), 0, P, nq, ql, L, dflag, ktchg, sflag, longq, d_n + 5,
+              d_work + 2);
+    C2_LAUNCH(ctx, k_mark_decide_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 5, L, sflag, d_work + 2);
     s_nq += nq;
     const uint32_t ns = collect(sflag, 1, sl, d_n + 3);
     C2_HIP(hipMemsetAsync(d_n, 0, sizeof(uint32_t), st));
