@@ -54,6 +54,7 @@ static constexpr int SORT_LDS = 4096;   // longer still: segmented radix sort
 
 __device__ __forceinline__ uint64_t ev_time(uint64_t e) { return e >> LAB_BITS; }
 __device__ __forceinline__ int32_t ev_lab(uint64_t e) { return (int32_t)(e & ((1u << LAB_BITS) - 1)); }
+__device__ __forceinline__ uint32_t ev_pos(uint64_t e) { return (uint32_t)(ev_time(e) & ((1u << POS_BITS) - 1)); }
 __device__ __forceinline__ uint64_t make_event(uint64_t t0, int l, int32_t lab) {
   return ((t0 | (uint64_t)l) << LAB_BITS) | (uint64_t)(uint32_t)lab;
 }
@@ -215,16 +216,17 @@ __global__ void k_reserve(uint32_t N, const uint32_t *__restrict__ cnt, uint32_t
 __global__ void __launch_bounds__(256) k_ev_fill(BParams P, uint32_t n, const uint32_t *__restrict__ list,
                                                  const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
                                                  const int32_t *__restrict__ ell, const int32_t *__restrict__ lam,
-                                                 Lists L, uint32_t *__restrict__ cur, int *err,
-                                                 unsigned long long *work) {
+                                                 const uint16_t *__restrict__ from, Lists L,
+                                                 uint32_t *__restrict__ cur, int *err, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = t < n ? (list ? list[t] : t) : 0u;
   const int e = t < n ? ell[s] : 0;
-  if (e > 0) {
+  const int f = (t < n && from) ? (int)from[s] : 0;  // events before f are already in place
+  if (e > f) {
     const uint64_t t0 = sweep_index(P, s) << POS_BITS;
     const int32_t lab = lam[s];
     const uint64_t o = toff[s];
-    for (int l0 = 0; l0 < e; l0 += 4) {  // four slot reservations in flight at once
+    for (int l0 = f; l0 < e; l0 += 4) {  // four slot reservations in flight at once
       const int m = (e - l0) < 4 ? e - l0 : 4;
       uint32_t y[4], slot[4];
       for (int u = 0; u < 4; u++) y[u] = u < m ? tp[o + l0 + u] : 0u;
@@ -237,7 +239,7 @@ __global__ void __launch_bounds__(256) k_ev_fill(BParams P, uint32_t n, const ui
       }
     }
   }
-  wave_add(work, (uint64_t)e);
+  wave_add(work, (uint64_t)(e > f ? e - f : 0));
 }
 
 __device__ __forceinline__ bool list_uniform(const uint64_t *__restrict__ ev, uint64_t b, uint64_t e) {
@@ -505,14 +507,18 @@ __global__ void __launch_bounds__(256) k_decide(BParams P, uint32_t n, const uin
 
 // ---------------------------------------------------------------- incremental step
 
-// Count changes per point from the changed starts' old and new paths; every
-// point on either path gets its list rewritten (dflag).
+// Count changes per point from the changed starts' old and new paths.  Both
+// paths are prefixes of the start's trajectory, so when the label stays the
+// same the events of the common prefix are identical and only the positions
+// from min(old, new) length on change (from[s]); every point holding such a
+// position gets its list rewritten (dflag).
 __global__ void __launch_bounds__(256) k_delta(uint32_t nc, const uint32_t *__restrict__ chlist,
                                                const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
                                                const int32_t *__restrict__ ell, const int32_t *__restrict__ ell2,
                                                const int32_t *__restrict__ lam, const int32_t *__restrict__ lam2,
                                                int32_t *__restrict__ delta, uint8_t *__restrict__ dflag,
-                                               unsigned long long *stats, unsigned long long *work) {
+                                               uint16_t *__restrict__ from, unsigned long long *stats,
+                                               unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t st[4] = {0, 0, 0, 0};
   uint64_t upd = 0;
@@ -522,7 +528,9 @@ __global__ void __launch_bounds__(256) k_delta(uint32_t nc, const uint32_t *__re
     const int eo = ell[s], en = ell2[s];
     const int m = eo > en ? eo : en, mn = eo < en ? eo : en;
     const bool same = lam[s] == lam2[s];
-    for (int l = 0; l < m; l++) {
+    const int f = same ? mn : 0;
+    from[s] = (uint16_t)f;
+    for (int l = f; l < m; l++) {
       const uint32_t y = tp[o + l];
       const int d = (l < en ? 1 : 0) - (l < eo ? 1 : 0);
       if (d) atomicAdd(&delta[y], d);
@@ -530,7 +538,7 @@ __global__ void __launch_bounds__(256) k_delta(uint32_t nc, const uint32_t *__re
     }
     st[0] = same && mn > 0;           // changes that keep the label
     st[1] = same ? (uint64_t)mn : 0;  // unchanged prefix events
-    st[2] = (uint64_t)m;              // positions touched
+    st[2] = (uint64_t)(m - f);        // positions touched
     st[3] = same ? 0 : (uint64_t)mn;  // prefix events relabelled
     upd = (uint64_t)(m - mn);
   }
@@ -567,7 +575,8 @@ __global__ void __launch_bounds__(256) k_patch_keep(BParams P, uint32_t nd, cons
                                                     const uint32_t *__restrict__ ncnt,
                                                     const uint32_t *__restrict__ alloc,
                                                     const uint64_t *__restrict__ aoff, uint64_t base, Lists L,
-                                                    const uint8_t *__restrict__ chg, uint32_t *__restrict__ cur,
+                                                    const uint8_t *__restrict__ chg,
+                                                    const uint16_t *__restrict__ from, uint32_t *__restrict__ cur,
                                                     uint32_t *__restrict__ longq, uint32_t *nlong, int *err,
                                                     unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -586,7 +595,8 @@ __global__ void __launch_bounds__(256) k_patch_keep(BParams P, uint32_t nd, cons
     uint32_t kept = 0;
     for (uint64_t x = ob; x < oe; x++) {
       const uint64_t v = L.ev[x];
-      if (!chg[event_start(P, v)]) {
+      const uint32_t st = event_start(P, v);
+      if (!chg[st] || ev_pos(v) < from[st]) {
         if (kept < lim)
           L.ev[nb + kept] = v;
         else
@@ -633,6 +643,7 @@ __global__ void __launch_bounds__(256) k_patch_keep_long(BParams P, const uint32
                                                          const uint32_t *__restrict__ alloc,
                                                          const uint64_t *__restrict__ aoff, uint64_t base, Lists L,
                                                          const uint8_t *__restrict__ chg,
+                                                         const uint16_t *__restrict__ from,
                                                          uint32_t *__restrict__ cur, int *err,
                                                          unsigned long long *work) {
   __shared__ uint32_t lds4[4];
@@ -653,7 +664,8 @@ __global__ void __launch_bounds__(256) k_patch_keep_long(BParams P, const uint32
       bool keep = false;
       if (x < oe) {
         v = L.ev[x];
-        keep = !chg[event_start(P, v)];
+        const uint32_t st = event_start(P, v);
+        keep = !chg[st] || ev_pos(v) < from[st];
       }
       uint32_t tot;
       const uint32_t pos = kept + block_prefix(keep, lds4, tot);
@@ -828,6 +840,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   uint64_t *offN = buf<uint64_t>(ctx, "sweep_off", (size_t)N + 1);
   uint32_t *cur = buf<uint32_t>(ctx, "sweep_cur", N);
   int32_t *delta = buf<int32_t>(ctx, "sweep_delta", N);
+  uint16_t *cfrom = buf<uint16_t>(ctx, "sweep_from", N);  // first changed position of each changed start
   uint64_t *Kt = buf<uint64_t>(ctx, "sweep_kt", N);
   uint8_t *uni = buf<uint8_t>(ctx, "sweep_uni", N);
   uint8_t *chg = buf<uint8_t>(ctx, "sweep_chg", NF);
@@ -887,8 +900,8 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     }
     pool_top = reserved;
     C2_HIP(hipMemsetAsync(cur, 0, sizeof(uint32_t) * N, st));
-    C2_LAUNCH(ctx, k_ev_fill, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, toff, tp, ell, lam, L, cur,
-              d_err, d_work + 4);
+    C2_LAUNCH(ctx, k_ev_fill, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, toff, tp, ell, lam,
+              (const uint16_t *)nullptr, L, cur, d_err, d_work + 4);
     sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8);
     C2_LAUNCH(ctx, k_kentry, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, L, uni, Kt,
               (uint8_t *)nullptr);
@@ -915,7 +928,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     const unsigned gC = grid_for(nc, B);
     // 1. points whose lists change, and their new sizes
     if (trace) C2_HIP(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), st));
-    C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, lam, lam2, delta, dflag,
+    C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, lam, lam2, delta, dflag, cfrom,
               trace ? d_stats : nullptr, d_work + 5);
     s_nc += nc;
     const uint32_t nd = collect(dflag, 0, dl, d_n + 1);
@@ -935,11 +948,12 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     }
     // 2. rewrite those lists; add the changed starts' events
     C2_HIP(hipMemsetAsync(d_n + 4, 0, 2 * sizeof(uint32_t), st));
-    C2_LAUNCH(ctx, k_patch_keep, dim3(gD), dim3(B), 0, P, nd, dl, ncnt, alloc, aoff, pool_top, L, chg, cur, longq,
-              d_n + 4, d_err, d_work + 0);
+    C2_LAUNCH(ctx, k_patch_keep, dim3(gD), dim3(B), 0, P, nd, dl, ncnt, alloc, aoff, pool_top, L, chg, cfrom, cur,
+              longq, d_n + 4, d_err, d_work + 0);
     C2_LAUNCH(ctx, k_patch_keep_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 4, dl, ncnt, alloc, aoff, pool_top,
-              L, chg, cur, d_err, d_work + 0);
-    C2_LAUNCH(ctx, k_ev_fill, dim3(gC), dim3(B), 0, P, nc, chlist, toff, tp, ell2, lam2, L, cur, d_err, d_work + 4);
+              L, chg, cfrom, cur, d_err, d_work + 0);
+    C2_LAUNCH(ctx, k_ev_fill, dim3(gC), dim3(B), 0, P, nc, chlist, toff, tp, ell2, lam2, cfrom, L, cur, d_err,
+              d_work + 4);
     s_nd += nd;
     C2_LAUNCH(ctx, k_apply, dim3(gC), dim3(B), 0, nc, chlist, ell, lam, ell2, lam2, chg);
     pool_top += added;
