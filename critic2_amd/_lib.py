@@ -34,7 +34,8 @@ EXPORTS = (
     "c2gpu_bader_metrics", "c2gpu_bader_step_pointers", "c2gpu_bader", "c2gpu_relabel",
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
-    "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff",
+    "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff", "c2gpu_ws_facets",
+    "c2gpu_yt",
 )
 
 
@@ -63,6 +64,14 @@ class BaderStats(C.Structure):
 class CubeHeader(C.Structure):
     _fields_ = [("nat", C.c_int), ("ismo", C.c_int), ("n", C.c_int * 3), ("x0", C.c_double * 3),
                 ("step", C.c_double * 9), ("data_offset", C.c_uint64)]
+
+
+class YtStats(C.Structure):
+    _fields_ = [("nattr", C.c_int), ("nmaxima", C.c_int), ("nias", C.c_int64), ("label_passes", C.c_int),
+                ("weight_passes", C.c_int), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class NciCut(C.Structure):
@@ -123,6 +132,9 @@ def load():
         "c2gpu_cube_write": [vp, C.c_char_p, C.POINTER(CubeHeader), vp, vp, vp, i, i],
         "c2gpu_nci_cutoff": [vp, vp, vp, vp, vp, i, C.POINTER(NciCut), i, vp, C.c_uint64, C.POINTER(C.c_uint64),
                              vp],
+        "c2gpu_ws_facets": [vp, C.POINTER(i), vp, vp],
+        "c2gpu_yt": [vp, vp, vp, vp, C.POINTER(AttractorOpts), i, vp, i, vp, C.POINTER(i), vp, i, vp, vp,
+                     C.POINTER(YtStats)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -263,6 +275,20 @@ class DeviceArray:
                 self.free()
         except Exception:
             pass
+
+
+def ws_facets(m):
+    """Voronoi-relevant lattice vectors (lattice coordinates) and facet areas
+    of the lattice with vectors m[:, i] (host code, no GPU needed)."""
+    lib = load()
+    mm = mat_f(m)
+    nv = C.c_int()
+    vec = np.zeros(42, dtype=np.int32)
+    area = np.zeros(14)
+    rc = lib.c2gpu_ws_facets(mm.ctypes.data, C.byref(nv), vec.ctypes.data, area.ctypes.data)
+    if rc != OK:
+        raise C2GpuError(rc, "c2gpu_ws_facets failed")
+    return vec[:3 * nv.value].reshape(-1, 3).copy(), area[:nv.value].copy()
 
 
 def bader_metrics(x2c, n):

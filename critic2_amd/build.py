@@ -25,6 +25,8 @@ SOURCES = [
     ("ncigrid.hip", ["-ffp-contract=off"]),
     ("cube.hip", ["-ffp-contract=off"]),
     ("ncicut.hip", ["-ffp-contract=off"]),
+    ("yt.hip", ["-ffp-contract=off"]),
+    ("wscell.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
 ]

@@ -30,6 +30,9 @@ void cube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *
 void bincube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
 void nci_cutoff_run(c2gpu_ctx *, const void *, void *, const void *, const int *, int, const c2gpu_nci_cut *, int,
                     char *, uint64_t, uint64_t *, double *);
+int ws_facets(const double m[9], int vec[3 * 14], double area[14]);
+void yt_run(c2gpu_ctx *, const double *, const int *, const double *, const c2gpu_attractor_opts *, int,
+            const double *const *, int, int32_t *, int *, double *, int, double *, double *, c2gpu_yt_stats *);
 void cube_format_run(c2gpu_ctx *, const double *, const int *, int, int, int, char *, uint64_t, uint64_t *);
 void cube_write_run(c2gpu_ctx *, const char *, const c2gpu_cube_header *, const int *, const double *,
                     const double *, int, int);
@@ -491,6 +494,26 @@ int c2gpu_nci_cutoff(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *
                      int precision, const c2gpu_nci_cut *opts, int flags, char *dat, uint64_t cap,
                      uint64_t *len, double sums[3]) {
   return guarded(ctx, [&] { nci_cutoff_run(ctx, crho, cgrad, rhoat, n, precision, opts, flags, dat, cap, len, sums); });
+}
+
+int c2gpu_ws_facets(const double m[9], int *nvec, int vec[42], double area[14]) {
+  if (!m || !nvec || !vec || !area) return C2GPU_ERR_ARG;
+  try {
+    *nvec = ws_facets(m, vec, area);
+    return C2GPU_OK;
+  } catch (const Error &e) {
+    return e.code;
+  }
+}
+
+int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9],
+             const c2gpu_attractor_opts *opts, int nfields, const double *const *fields, int flags,
+             int32_t *label_out, int *nattr_out, double *xattr_out, int xattr_cap, double *vol_out,
+             double *sums_out, c2gpu_yt_stats *stats) {
+  return guarded(ctx, [&] {
+    yt_run(ctx, rho, n, x2c, opts, nfields, fields, flags, label_out, nattr_out, xattr_out, xattr_cap, vol_out,
+           sums_out, stats);
+  });
 }
 
 }  // extern "C"

@@ -1,0 +1,465 @@
+// Yu-Trinkle basins and weights on the GPU (yt_integrate, yt@proc.f90:38-224;
+// yt_weights, :399-531; the imtype_yt sums of intgrid_fields,
+// integration@proc.f90:1208-1218, :1289-1300).
+//
+// The reference ranks the points by rho and runs down the ranking: a point
+// with no higher neighbour (over the Voronoi-relevant neighbours of the grid
+// lattice) is a maximum; a point whose higher neighbours share one basin
+// joins it; the others lie on an interatomic surface (basin 0) and pass on
+// weight to the basins of their higher neighbours, fractions chi_k / sum chi
+// with chi_k = area_k * (rho_j - rho_i).  None of this needs the ranking
+// itself, only the comparisons between neighbours (equal neighbours are
+// refused: their order would be the reference sort's):
+//   k_yt_class    higher-neighbour mask per point, maxima listed;
+//   host          maxima in decreasing rho, attractor rules (as for Bader);
+//   k_yt_prop     per point the least and greatest basin reachable upwards,
+//                 propagated to a fixpoint; equal means the point's basin
+//                 (a point joins a basin exactly when every upward path ends
+//                 in it), different means a surface point;
+//   k_yt_weights  the surface points' weights, a sparse vector of up to 8
+//                 basins each, recomputed from the higher neighbours' until
+//                 no pass changes them (double-buffered); each basin's weight
+//                 sums the contributions in decreasing rho of the higher
+//                 neighbour, the order in which yt_weights adds them, so the
+//                 weights are the reference's bit for bit;
+//   sums          basin interiors through the per-basin sums kernel, surface
+//                 points through k_yt_surface_sums.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "devutil.hpp"
+
+namespace c2 {
+
+int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
+                   int32_t *lab, std::vector<double> &xattr);
+int ws_facets(const double m[9], int vec[3 * 14], double area[14]);
+void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const double *const *, int, double *,
+                    int64_t *);
+
+namespace {
+
+constexpr int YB = 256;
+constexpr int KW = 8;  // basins per surface point
+constexpr uint32_t LO_NONE = 0xffffffffu;
+constexpr double VSMALL = 1e-40;  // param.F90 vsmall
+
+struct YtGeo {
+  int n1, n2, n3, nvec;
+  int di[14], dj[14], dk[14];
+  double al[14];
+};
+
+__device__ __forceinline__ int wrapn(int x, int n) {
+  x %= n;
+  return x < 0 ? x + n : x;
+}
+
+__device__ __forceinline__ uint32_t nbr(const YtGeo &G, int i, int j, int k, int q) {
+  return (uint32_t)wrapn(i + G.di[q], G.n1) +
+         (uint32_t)G.n1 * ((uint32_t)wrapn(j + G.dj[q], G.n2) + (uint32_t)G.n2 * (uint32_t)wrapn(k + G.dk[q], G.n3));
+}
+
+__device__ __forceinline__ void coords3(const YtGeo &G, uint32_t p, int &i, int &j, int &k) {
+  i = (int)(p % (uint32_t)G.n1);
+  const uint32_t r = p / (uint32_t)G.n1;
+  j = (int)(r % (uint32_t)G.n2);
+  k = (int)(r / (uint32_t)G.n2);
+}
+
+__device__ __forceinline__ void yt_append(bool pred, uint32_t val, uint32_t *list, uint32_t *n) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = val;
+}
+
+__global__ void __launch_bounds__(YB) k_yt_class(YtGeo G, uint32_t N, const double *__restrict__ g,
+                                                  uint16_t *__restrict__ hmask, uint32_t *__restrict__ maxl,
+                                                  uint32_t *nmax, int *err) {
+  const uint32_t p = blockIdx.x * YB + threadIdx.x;
+  uint32_t mask = 0;
+  if (p < N) {
+    int i, j, k;
+    coords3(G, p, i, j, k);
+    const double gp = g[p];
+    bool tie = false;
+    for (int q = 0; q < G.nvec; q++) {
+      const double gj = g[nbr(G, i, j, k, q)];
+      if (gj > gp) mask |= 1u << q;
+      else if (gj == gp) tie = true;
+    }
+    hmask[p] = (uint16_t)mask;
+    if (tie) atomicOr(err, 1);
+  }
+  yt_append(p < N && mask == 0, p, maxl, nmax);
+}
+
+__global__ void k_yt_gather(uint32_t n, const uint32_t *__restrict__ idx, const double *__restrict__ g,
+                            double *__restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = g[idx[t]];
+}
+
+__global__ void k_yt_init(uint32_t N, uint64_t *__restrict__ lohi) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < N) lohi[p] = (uint64_t)LO_NONE << 32;
+}
+
+__global__ void k_yt_seed(uint32_t n, const uint32_t *__restrict__ maxl, const int32_t *__restrict__ lab,
+                          uint64_t *__restrict__ lohi) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) lohi[maxl[t]] = ((uint64_t)(uint32_t)lab[t] << 32) | (uint32_t)lab[t];
+}
+
+// One pass of lo = min, hi = max over the higher neighbours, in place (the
+// values only move towards the fixpoint, so racing reads are harmless).
+__global__ void __launch_bounds__(YB) k_yt_prop(YtGeo G, uint32_t N, const uint16_t *__restrict__ hmask,
+                                                 uint64_t *lohi, int *changed) {
+  const uint32_t p = blockIdx.x * YB + threadIdx.x;
+  if (p >= N) return;
+  uint32_t mask = hmask[p];
+  if (!mask) return;
+  int i, j, k;
+  coords3(G, p, i, j, k);
+  uint32_t lo = LO_NONE, hi = 0;
+  while (mask) {
+    const int q = __ffs(mask) - 1;
+    mask &= mask - 1;
+    const uint64_t v = __hip_atomic_load(&lohi[nbr(G, i, j, k, q)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t l = (uint32_t)(v >> 32), h = (uint32_t)v;
+    lo = l < lo ? l : lo;
+    hi = h > hi ? h : hi;
+  }
+  const uint64_t nv = ((uint64_t)lo << 32) | hi;
+  if (nv != lohi[p]) {
+    __hip_atomic_store(&lohi[p], nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *changed = 1;
+  }
+}
+
+__global__ void k_yt_label(uint32_t N, const uint64_t *__restrict__ lohi, int32_t *__restrict__ lab,
+                           uint8_t *__restrict__ ias) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint64_t v = lohi[p];
+  const uint32_t lo = (uint32_t)(v >> 32), hi = (uint32_t)v;
+  const bool one = lo == hi;
+  lab[p] = one ? (int32_t)lo : 0;
+  ias[p] = one ? 0 : 1;
+}
+
+__global__ void k_yt_slots(uint32_t n, const uint32_t *__restrict__ ias, int32_t *__restrict__ slot) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) slot[ias[t]] = (int32_t)t;
+}
+
+struct WBuf {
+  uint32_t *b;  // KW basins per surface point
+  double *w;    // KW weights
+  uint8_t *c;   // entries in use
+};
+
+// Weights of each surface point from its higher neighbours' (buffer in),
+// written to buffer out; entries sorted by basin.
+__global__ void __launch_bounds__(YB) k_yt_weights(YtGeo G, uint32_t nias, const uint32_t *__restrict__ ias,
+                                                    const double *__restrict__ g, const uint16_t *__restrict__ hmask,
+                                                    const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
+                                                    WBuf in, WBuf out, int *changed, int *err) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  if (t >= nias) return;
+  const uint32_t p = ias[t];
+  int i, j, k;
+  coords3(G, p, i, j, k);
+  const double gp = g[p];
+  uint32_t hq[14];
+  double hg[14], chi[14];
+  int nh = 0;
+  double csum = 0.0;
+  const uint32_t mask = hmask[p];
+  for (int q = 0; q < G.nvec; q++) {
+    if (!(mask & (1u << q))) continue;
+    const uint32_t jn = nbr(G, i, j, k, q);
+    const double gj = g[jn];
+    const double c = fmax(G.al[q] * (gj - gp), VSMALL);
+    hq[nh] = jn;
+    hg[nh] = gj;
+    chi[nh] = c;
+    csum = csum + c;
+    nh++;
+  }
+  // contributions in decreasing rho of the higher neighbour
+  int ord[14];
+  for (int a = 0; a < nh; a++) {
+    int b = a;
+    while (b > 0 && hg[ord[b - 1]] < hg[a]) {
+      ord[b] = ord[b - 1];
+      b--;
+    }
+    ord[b] = a;
+  }
+  uint32_t bb[KW];
+  double ww[KW];
+  int na = 0;
+  bool over = false;
+  auto add = [&](uint32_t basin, double x) {
+    for (int e = 0; e < na; e++)
+      if (bb[e] == basin) {
+        ww[e] = ww[e] + x;
+        return;
+      }
+    if (na < KW) {
+      bb[na] = basin;
+      ww[na] = 0.0 + x;
+      na++;
+    } else {
+      over = true;
+    }
+  };
+  const double den = fmax(csum, VSMALL);
+  for (int a = 0; a < nh; a++) {
+    const int o = ord[a];
+    const double fr = chi[o] / den;
+    const uint32_t jn = hq[o];
+    const int32_t lj = lab[jn];
+    if (lj != 0) {
+      add((uint32_t)lj, fr * 1.0);
+    } else {
+      const int32_t s = slot[jn];
+      const int ce = in.c[s];
+      for (int e = 0; e < ce; e++) add(in.b[(size_t)KW * s + e], fr * in.w[(size_t)KW * s + e]);
+    }
+  }
+  if (over) atomicOr(err, 2);
+  // entries by basin
+  for (int a = 1; a < na; a++) {
+    const uint32_t kb = bb[a];
+    const double kw = ww[a];
+    int b = a;
+    while (b > 0 && bb[b - 1] > kb) {
+      bb[b] = bb[b - 1];
+      ww[b] = ww[b - 1];
+      b--;
+    }
+    bb[b] = kb;
+    ww[b] = kw;
+  }
+  bool same = in.c[t] == na;
+  for (int e = 0; e < na && same; e++)
+    same = in.b[(size_t)KW * t + e] == bb[e] &&
+           __double_as_longlong(in.w[(size_t)KW * t + e]) == __double_as_longlong(ww[e]);
+  out.c[t] = (uint8_t)na;
+  for (int e = 0; e < na; e++) {
+    out.b[(size_t)KW * t + e] = bb[e];
+    out.w[(size_t)KW * t + e] = ww[e];
+  }
+  if (!same) *changed = 1;
+}
+
+__global__ void k_yt_clear(uint32_t n, uint8_t *__restrict__ c) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) c[t] = 0;
+}
+
+struct YtFields {
+  const double *p[4];
+};
+
+// Surface points' weighted sums: acc[q*nattr + a], q = 0 the weights.
+__global__ void __launch_bounds__(YB) k_yt_surface_sums(uint32_t nias, const uint32_t *__restrict__ ias, WBuf W,
+                                                         int nattr, int nf, YtFields F, double *__restrict__ acc) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  if (t >= nias) return;
+  const uint32_t p = ias[t];
+  const int ce = W.c[t];
+  for (int e = 0; e < ce; e++) {
+    const int a = (int)W.b[(size_t)KW * t + e] - 1;
+    const double w = W.w[(size_t)KW * t + e];
+    if (a < 0 || a >= nattr) continue;
+    atomicAdd(&acc[a], w);
+    for (int q = 0; q < nf; q++) atomicAdd(&acc[(size_t)(q + 1) * nattr + a], w * F.p[q][p]);
+  }
+}
+
+}  // namespace
+
+void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], const c2gpu_attractor_opts *opts,
+            int nfields, const double *const *fields, int flags, int32_t *label_out, int *nattr_out,
+            double *xattr_out, int xattr_cap, double *vol_out, double *sums_out, c2gpu_yt_stats *stats) {
+  if (!rho || !n || !x2c || !opts || !nattr_out || !xattr_out || !vol_out) fail(C2GPU_ERR_ARG, "c2gpu_yt: NULL argument");
+  if (nfields < 0 || nfields > 4 || (nfields > 0 && (!fields || !sums_out)))
+    fail(C2GPU_ERR_ARG, "c2gpu_yt: 0 <= nfields <= 4, with fields and sums_out");
+  for (int a = 0; a < 3; a++)
+    if (n[a] < 3) fail(C2GPU_ERR_ARG, "c2gpu_yt: grids need n >= 3 per axis");
+  const uint64_t N64 = (uint64_t)n[0] * n[1] * n[2];
+  if (N64 >= (1ull << 32)) fail(C2GPU_ERR_ARG, "c2gpu_yt: grids need fewer than 2^32 points");
+  const uint32_t N = (uint32_t)N64;
+  hipStream_t st = ctx->stream;
+  Timer ttot(st);
+  c2gpu_yt_stats S{};
+  const bool dev = flags & C2GPU_PTR_DEVICE;
+  // geometry: the Voronoi-relevant neighbours of the grid lattice
+  YtGeo G{};
+  G.n1 = n[0];
+  G.n2 = n[1];
+  G.n3 = n[2];
+  double m[9];
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) m[r + 3 * c] = x2c[r + 3 * c] / (double)n[c];
+  int vec[42];
+  G.nvec = ws_facets(m, vec, G.al);
+  for (int q = 0; q < G.nvec; q++) {
+    G.di[q] = vec[3 * q];
+    G.dj[q] = vec[3 * q + 1];
+    G.dk[q] = vec[3 * q + 2];
+  }
+  // inputs on the device
+  const double *g = rho;
+  YtFields F{};
+  if (dev) {
+    require_device_ptr(rho, "rho");
+    for (int q = 0; q < nfields; q++) {
+      require_device_ptr(fields[q], "fields[q]");
+      F.p[q] = fields[q];
+    }
+    if (label_out) require_device_ptr(label_out, "label_out");
+  } else {
+    double *d = buf<double>(ctx, "yt_rho", N);
+    C2_HIP(hipMemcpyAsync(d, rho, 8 * (size_t)N, hipMemcpyHostToDevice, st));
+    g = d;
+    for (int q = 0; q < nfields; q++) {
+      double *df = buf<double>(ctx, "yt_f" + std::to_string(q), N);
+      C2_HIP(hipMemcpyAsync(df, fields[q], 8 * (size_t)N, hipMemcpyHostToDevice, st));
+      F.p[q] = df;
+    }
+  }
+  uint16_t *hmask = buf<uint16_t>(ctx, "yt_hmask", N);
+  uint32_t *maxl = buf<uint32_t>(ctx, "yt_maxl", N);
+  uint32_t *d_cnt = buf<uint32_t>(ctx, "yt_cnt", 2);
+  int *d_err = buf<int>(ctx, "yt_err", 2);  // err, changed
+  C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
+  C2_HIP(hipMemsetAsync(d_err, 0, 8, st));
+  const unsigned gN = grid_for(N, YB);
+  C2_LAUNCH(ctx, k_yt_class, dim3(gN), dim3(YB), 0, G, N, g, hmask, maxl, d_cnt, d_err);
+  add_kernel_bytes(ctx, "k_yt_class", (double)N * (8.0 * (G.nvec + 1) + 2.0));
+  const uint32_t nmax = read_scalar(ctx, d_cnt);
+  if (read_scalar(ctx, d_err) & 1)
+    fail(C2GPU_ERR_DEGENERATE, "c2gpu_yt: neighbouring grid points with equal values (their order is the "
+                               "reference sort's)");
+  // maxima in decreasing rho; the attractor rules
+  std::vector<uint32_t> h_max(nmax);
+  std::vector<double> h_gmax(nmax);
+  if (nmax) {
+    double *d_gm = buf<double>(ctx, "yt_gmax", nmax);
+    C2_LAUNCH(ctx, k_yt_gather, dim3(grid_for(nmax, YB)), dim3(YB), 0, nmax, maxl, g, d_gm);
+    C2_HIP(hipMemcpyAsync(h_max.data(), maxl, 4 * (size_t)nmax, hipMemcpyDeviceToHost, st));
+    C2_HIP(hipMemcpyAsync(h_gmax.data(), d_gm, 8 * (size_t)nmax, hipMemcpyDeviceToHost, st));
+    C2_HIP(hipStreamSynchronize(st));
+  }
+  std::vector<uint32_t> order(nmax);
+  for (uint32_t q = 0; q < nmax; q++) order[q] = q;
+  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return h_gmax[a] != h_gmax[b] ? h_gmax[a] > h_gmax[b] : h_max[a] < h_max[b];
+  });
+  for (uint32_t q = 1; q < nmax; q++)
+    if (h_gmax[order[q]] == h_gmax[order[q - 1]])
+      fail(C2GPU_ERR_DEGENERATE, "c2gpu_yt: maxima with equal values (their numbering is the reference sort's)");
+  std::vector<double> dv(3 * (size_t)nmax);
+  for (uint32_t q = 0; q < nmax; q++) {
+    const uint32_t l = h_max[q];
+    dv[3 * q] = (double)(l % (uint32_t)n[0]) / n[0];
+    dv[3 * q + 1] = (double)((l / (uint32_t)n[0]) % (uint32_t)n[1]) / n[1];
+    dv[3 * q + 2] = (double)(l / ((uint32_t)n[0] * (uint32_t)n[1])) / n[2];
+  }
+  std::vector<int32_t> mlab(nmax);
+  std::vector<double> xattr;
+  const int nattr = attractor_rule(opts, nmax, order.data(), dv.data(), mlab.data(), xattr);
+  if (nattr > xattr_cap) fail(C2GPU_ERR_ARG, "c2gpu_yt: xattr_cap too small for " + std::to_string(nattr));
+  if (nattr < 1) fail(C2GPU_ERR_ARG, "c2gpu_yt: no attractors");
+  S.nmaxima = (int)nmax;
+  S.nattr = nattr;
+  // labels: least and greatest basin reachable upwards
+  uint64_t *lohi = buf<uint64_t>(ctx, "yt_lohi", N);
+  int32_t *d_mlab = buf<int32_t>(ctx, "yt_mlab", nmax > 0 ? nmax : 1);
+  C2_HIP(hipMemcpyAsync(d_mlab, mlab.data(), 4 * (size_t)nmax, hipMemcpyHostToDevice, st));
+  C2_LAUNCH(ctx, k_yt_init, dim3(gN), dim3(YB), 0, N, lohi);
+  if (nmax) C2_LAUNCH(ctx, k_yt_seed, dim3(grid_for(nmax, YB)), dim3(YB), 0, nmax, maxl, d_mlab, lohi);
+  int passes = 0;
+  for (;;) {
+    C2_HIP(hipMemsetAsync(d_err + 1, 0, 4, st));
+    for (int r = 0; r < 8; r++) {
+      C2_LAUNCH(ctx, k_yt_prop, dim3(gN), dim3(YB), 0, G, N, hmask, lohi, d_err + 1);
+      passes++;
+    }
+    add_kernel_bytes(ctx, "k_yt_prop", 8.0 * (double)N * (2.0 + 8.0 * G.nvec / 2.0));
+    if (!read_scalar(ctx, d_err + 1)) break;
+    if (passes > 4 * (n[0] + n[1] + n[2]) + 64) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_yt: label propagation");
+  }
+  S.label_passes = passes;
+  int32_t *lab = dev && label_out ? label_out : buf<int32_t>(ctx, "yt_lab", N);
+  uint8_t *iasf = buf<uint8_t>(ctx, "yt_iasf", N);
+  C2_LAUNCH(ctx, k_yt_label, dim3(gN), dim3(YB), 0, N, lohi, lab, iasf);
+  uint32_t *ias = buf<uint32_t>(ctx, "yt_ias", N);
+  const uint32_t nias = compact_flags(ctx, "yt", iasf, N, ias);
+  S.nias = nias;
+  // surface weights
+  WBuf W[2];
+  int wpasses = 0;
+  int cur = 0;
+  if (nias) {
+    int32_t *slot = buf<int32_t>(ctx, "yt_slot", N);
+    C2_LAUNCH(ctx, k_yt_slots, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, slot);
+    for (int b = 0; b < 2; b++) {
+      W[b].b = buf<uint32_t>(ctx, "yt_wb" + std::to_string(b), (size_t)KW * nias);
+      W[b].w = buf<double>(ctx, "yt_ww" + std::to_string(b), (size_t)KW * nias);
+      W[b].c = buf<uint8_t>(ctx, "yt_wc" + std::to_string(b), nias);
+      C2_LAUNCH(ctx, k_yt_clear, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, W[b].c);
+    }
+    for (;;) {
+      C2_HIP(hipMemsetAsync(d_err + 1, 0, 4, st));
+      C2_LAUNCH(ctx, k_yt_weights, dim3(grid_for(nias, YB)), dim3(YB), 0, G, nias, ias, g, hmask, lab, slot, W[cur],
+                W[cur ^ 1], d_err + 1, d_err);
+      cur ^= 1;
+      wpasses++;
+      if (!read_scalar(ctx, d_err + 1)) break;
+      if (wpasses > 4 * (n[0] + n[1] + n[2]) + 64) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_yt: surface weights");
+    }
+    if (read_scalar(ctx, d_err) & 2)
+      fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: more than 8 basins meet at a surface point");
+  }
+  S.weight_passes = wpasses;
+  // sums: interiors, then surface points
+  std::vector<int64_t> counts(nattr);
+  std::vector<double> isums((size_t)nfields * nattr);
+  basin_sums_run(ctx, lab, N, nattr, nfields, F.p, C2GPU_PTR_DEVICE, nfields ? isums.data() : nullptr,
+                 counts.data());
+  const size_t nb = (size_t)(nfields + 1) * nattr;
+  double *acc = buf<double>(ctx, "yt_acc", nb);
+  C2_HIP(hipMemsetAsync(acc, 0, 8 * nb, st));
+  if (nias)
+    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, W[cur], nattr, nfields, F,
+              acc);
+  std::vector<double> h(nb);
+  C2_HIP(hipMemcpyAsync(h.data(), acc, 8 * nb, hipMemcpyDeviceToHost, st));
+  if (label_out && !dev) C2_HIP(hipMemcpyAsync(label_out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
+  C2_HIP(hipStreamSynchronize(st));
+  for (int a = 0; a < nattr; a++) vol_out[a] = (double)counts[a] + h[a];
+  for (int q = 0; q < nfields; q++)
+    for (int a = 0; a < nattr; a++)
+      sums_out[(size_t)q * nattr + a] = isums[(size_t)q * nattr + a] + h[(size_t)(q + 1) * nattr + a];
+  for (size_t c = 0; c < xattr.size(); c++) xattr_out[c] = xattr[c];
+  *nattr_out = nattr;
+  S.ms_total = ttot.stop();
+  if (stats) *stats = S;
+}
+
+}  // namespace c2

@@ -310,6 +310,45 @@ int c2gpu_batch_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int attr
                     c2gpu_batch_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Yu-Trinkle integration                                               */
+/* ------------------------------------------------------------------ */
+
+/* Facets of the Wigner-Seitz cell of the lattice with vectors m (columns):
+ * the Voronoi-relevant lattice vectors (3 integers each, lattice
+ * coordinates) and facet areas; for the grid lattice x2c(:,i)/n(i) this is
+ * grid3%vec / grid3%area (wscell, tools@proc.f90:679-800).  *nvec <= 14.
+ * Host only. */
+int c2gpu_ws_facets(const double m[9], int *nvec, int vec[42], double area[14]);
+
+typedef struct {
+  int nattr;          /* attractors                                        */
+  int nmaxima;        /* grid maxima                                       */
+  int64_t nias;       /* points on the interatomic surfaces (basin 0)      */
+  int label_passes;   /* propagation passes for the labels                 */
+  int weight_passes;  /* propagation passes for the surface weights        */
+  double ms_total;
+} c2gpu_yt_stats;
+
+/* yt_integrate (yt@proc.f90:38-224) with the weights of yt_weights
+ * (:399-531) summed as intgrid_fields does for imtype_yt
+ * (integration@proc.f90:1208-1218, :1289-1300).  Points are ranked by rho;
+ * a point with no higher neighbour among the grid's Voronoi-relevant
+ * neighbours is a maximum (attractor rules as in c2gpu_bader); a point whose
+ * higher neighbours all lie in one basin joins it; the others (the
+ * interatomic surfaces, label 0) split their weight over the basins of
+ * their higher neighbours in proportion to facet area times rho difference.
+ * label_out: basin per point in grid order (x fastest; critic2 keeps it in
+ * rank order, yt@proc.f90:218).  vol_out[a] = sum of basin a+1's weights,
+ * sums_out[q*nattr + a] = sum of weights times fields[q] (both unscaled:
+ * critic2 multiplies by Omega/N).  Equal values at neighbouring points give
+ * C2GPU_ERR_DEGENERATE (their order is the reference sort's).  rho,
+ * fields and label_out are device memory with C2GPU_PTR_DEVICE. */
+int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9],
+             const c2gpu_attractor_opts *opts, int nfields, const double *const *fields, int flags,
+             int32_t *label_out, int *nattr_out, double *xattr_out, int xattr_cap, double *vol_out,
+             double *sums_out, c2gpu_yt_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Gaussian cube input and output                                       */
 /* ------------------------------------------------------------------ */
 

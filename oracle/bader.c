@@ -656,3 +656,152 @@ void oracle_step_pointers(const double *f, const int n[3], const double car2lat[
   free(s.known);
   free(s.volnum);
 }
+
+/* ==================================================================== */
+/* Yu-Trinkle integration: yt_integrate (yt@proc.f90:38-224) and        */
+/* yt_weights (yt@proc.f90:399-531) with the sums of intgrid_fields     */
+/* (integration@proc.f90:1208-1218, :1289-1300) for imtype_yt.          */
+/* ==================================================================== */
+
+static const double *yt_sort_g;
+static int yt_cmp(const void *a, const void *b) {
+  const int64_t ia = *(const int64_t *)a, ib = *(const int64_t *)b;
+  const double fa = yt_sort_g[ia], fb = yt_sort_g[ib];
+  if (fa < fb) return -1;
+  if (fa > fb) return 1;
+  return ia < ib ? -1 : (ia > ib ? 1 : 0);
+}
+
+/* f: the grid (x fastest).  vec (3 per vector, grid steps) and area: the
+ * Voronoi-relevant vectors of the grid lattice and their facet areas
+ * (grid3%vec, grid3%area).  Returns 0, or 1 if two neighbouring points hold
+ * equal values (the order qcksort gives such a pair decides the result; the
+ * restatement sorts ties by index and refuses them instead), 2 if the
+ * attractor list overflows xcap.  idg: the basin of each point in grid
+ * order (0 on the interatomic surface; critic2 stores ibasin in sorted
+ * order, yt@proc.f90:218).  vol[a] = sum of the weights of basin a+1,
+ * sums[q*nattr + a] = sum of w*fields[q]; both unscaled (critic2
+ * multiplies by Omega/N). */
+int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, const int *vec, const double *area,
+              int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
+              double *vol, double *sums) {
+  const int64_t nn = (int64_t)n[0] * n[1] * n[2];
+  int64_t *io = malloc(sizeof(int64_t) * nn), *iio = malloc(sizeof(int64_t) * nn);
+  int32_t *ibasin = calloc(nn, sizeof(int32_t)); /* by sorted position, as in yt_integrate */
+  int *nlo = calloc(nn, sizeof(int));
+  int64_t *inear = calloc(nn * (size_t)nvec, sizeof(int64_t));
+  double *fnear = calloc(nn * (size_t)nvec, sizeof(double));
+  int rc = 0;
+  for (int64_t i = 0; i < nn; i++) io[i] = i;
+  yt_sort_g = f;
+  qsort(io, nn, sizeof(int64_t), yt_cmp); /* from smaller to larger (qcksort) */
+  for (int64_t i = 0; i < nn; i++) iio[io[i]] = i;
+#define YT_NB(i, k)                                                                           \
+  (wrap((int)((i) % n[0]) + vec[3 * (k)], n[0]) +                                             \
+   n[0] * (wrap((int)(((i) / n[0]) % n[1]) + vec[3 * (k) + 1], n[1]) +                        \
+           (int64_t)n[1] * wrap((int)((i) / ((int64_t)n[0] * n[1])) + vec[3 * (k) + 2], n[2])))
+  /* equal neighbours: the order between them would be qcksort's */
+  for (int64_t i = 0; i < nn && !rc; i++)
+    for (int k = 0; k < nvec; k++)
+      if (f[YT_NB(i, k)] == f[i]) rc = 1;
+  int nattr = g->mode != ORC_ATTR_NOATOMS ? g->nat : 0;
+  if (nattr > xcap) rc = 2;
+  for (int a = 0; a < nattr && !rc; a++)
+    for (int c = 0; c < 3; c++) xattr[3 * a + c] = g->xat[3 * a + c];
+  const double ratom = g->mode == ORC_ATTR_NEAREST_ATOM ? 1e40 : g->ratom;
+  const double vsmall = 1e-40;
+  int64_t ihi[64];
+  double chi[64];
+  /* run over the points in order of decreasing density (yt@proc.f90:100-181) */
+  for (int64_t ii = nn - 1; ii >= 0 && !rc; ii--) {
+    const int64_t i = io[ii];
+    int nhi = 0;
+    double csum = 0.0;
+    for (int k = 0; k < nvec; k++) {
+      const int64_t j = YT_NB(i, k);
+      const int64_t jj = iio[j];
+      if (jj > ii) {
+        ihi[nhi] = jj;
+        chi[nhi] = fmax(area[k] * (f[j] - f[i]), vsmall);
+        csum = csum + chi[nhi];
+        nhi++;
+      }
+    }
+    nlo[ii] = 0;
+    if (nhi == 0) { /* a local maximum */
+      const double dv[3] = {(double)(i % n[0]) / n[0], (double)((i / n[0]) % n[1]) / n[1],
+                            (double)(i / ((int64_t)n[0] * n[1])) / n[2]};
+      int assigned = 0;
+      if (g->mode != ORC_ATTR_NOATOMS) {
+        const int nid = identify_atom(g, dv, ratom);
+        if (nid > 0) {
+          ibasin[ii] = nid;
+          assigned = 1;
+        }
+      }
+      if (!assigned && ratom > vsmall) {
+        for (int k = 0; k < nattr; k++)
+          if (are_lclose(g, dv, xattr + 3 * k, ratom)) {
+            ibasin[ii] = k + 1;
+            assigned = 1;
+            break;
+          }
+      }
+      if (!assigned) {
+        if (nattr + 1 > xcap) {
+          rc = 2;
+          break;
+        }
+        for (int c = 0; c < 3; c++) xattr[3 * nattr + c] = dv[c];
+        nattr++;
+        ibasin[ii] = nattr;
+      }
+    } else {
+      int isias = ibasin[ihi[0]] == 0;
+      for (int k = 0; k < nhi; k++) isias = isias || (ibasin[ihi[k]] != ibasin[ihi[0]]);
+      if (!isias) {
+        ibasin[ii] = ibasin[ihi[0]];
+      } else {
+        ibasin[ii] = 0;
+        for (int k = 0; k < nhi; k++) {
+          const int64_t kk = ihi[k];
+          inear[(size_t)nvec * kk + nlo[kk]] = ii;
+          fnear[(size_t)nvec * kk + nlo[kk]] = chi[k] / fmax(csum, vsmall);
+          nlo[kk]++;
+        }
+      }
+    }
+  }
+#undef YT_NB
+  if (!rc) {
+    for (int64_t i = 0; i < nn; i++) idg[i] = ibasin[iio[i]];
+    /* weights of each basin (yt_weights) and their sums (intgrid_fields) */
+    double *waux = malloc(sizeof(double) * nn);
+    for (int m = 1; m <= nattr; m++) {
+      for (int64_t j = 0; j < nn; j++) waux[j] = ibasin[j] == m ? 1.0 : 0.0;
+      for (int64_t j = nn - 1; j >= 0; j--)
+        if (fabs(waux[j]) > 0.0)
+          for (int k = 0; k < nlo[j]; k++) {
+            const int64_t t = inear[(size_t)nvec * j + k];
+            waux[t] = waux[t] + fnear[(size_t)nvec * j + k] * waux[j];
+          }
+      double sv = 0.0;
+      for (int64_t p = 0; p < nn; p++) sv += waux[iio[p]];
+      vol[m - 1] = sv;
+      for (int q = 0; q < nfields; q++) {
+        double s = 0.0;
+        for (int64_t p = 0; p < nn; p++) s += waux[iio[p]] * fields[q][p];
+        sums[(size_t)q * nattr + (m - 1)] = s;
+      }
+    }
+    free(waux);
+  }
+  *nattr_out = nattr;
+  free(io);
+  free(iio);
+  free(ibasin);
+  free(nlo);
+  free(inear);
+  free(fnear);
+  return rc;
+}

@@ -135,6 +135,15 @@ void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
                         const double *xat, const int *zat, double rimg,
                         double *out);
 
+/* Yu-Trinkle integration (yt@proc.f90:38-224, :399-531; the imtype_yt sums
+ * of intgrid_fields).  vec: 3*nvec grid steps, area: nvec facet areas of the
+ * grid lattice's Wigner-Seitz cell.  idg in grid order; vol and sums
+ * unscaled; sums[q*nattr + a].  Returns 0, 1 (equal neighbouring values:
+ * refused), 2 (more than xcap attractors). */
+int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, const int *vec, const double *area,
+              int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
+              double *vol, double *sums);
+
 /* Cube readers (grid3mod@proc.f90:512-635); f == NULL: header only.
  * step: column i = the file's axis-i vector.  Return 0, or 1 on a read
  * error. */

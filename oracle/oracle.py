@@ -367,3 +367,41 @@ def nci_cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e
     if onlyneg:
         cg[crho > 0] = 100.0
     return dat, cg, sums
+
+
+def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40):
+    """yt_integrate + yt_weights + the imtype_yt sums (oracle_yt).  Returns
+    (labels in grid order, nattr, xattr, vol, sums[q, a]); raises on equal
+    neighbouring values."""
+    lib = load()
+    lib.oracle_yt.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcGeom), C.c_int, C.c_void_p, C.c_void_p,
+                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                              C.c_void_p]
+    lib.oracle_yt.restype = C.c_int
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    nn = np.asarray(n, dtype=np.int32)
+    xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1)
+    g = OrcGeom()
+    for q, v in enumerate(_matf(x2c)):
+        g.x2c[q] = v
+    g.nat = xat.size // 3
+    g.xat = xat.ctypes.data
+    g.mode = mode
+    g.ratom = ratom
+    vec = np.ascontiguousarray(vec, dtype=np.int32).reshape(-1)
+    area = np.ascontiguousarray(area, dtype=np.float64)
+    fl = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in fields]
+    fp = (C.c_void_p * max(len(fl), 1))(*[x.ctypes.data for x in fl])
+    idg = np.zeros(f.size, dtype=np.int32)
+    cap = 4 * g.nat + 4096
+    xattr = np.zeros(3 * cap)
+    vol = np.zeros(cap)
+    sums = np.zeros(max(len(fl), 1) * cap)
+    na = C.c_int(0)
+    rc = lib.oracle_yt(f.ctypes.data, nn.ctypes.data, C.byref(g), len(area), vec.ctypes.data, area.ctypes.data,
+                       len(fl), fp, idg.ctypes.data, C.byref(na), xattr.ctypes.data, cap, vol.ctypes.data,
+                       sums.ctypes.data)
+    if rc:
+        raise ValueError(f"oracle_yt: status {rc}")
+    k = na.value
+    return idg, k, xattr[:3 * k].reshape(-1, 3), vol[:k], sums[:len(fl) * k].reshape(len(fl), k)

@@ -1,0 +1,59 @@
+"""GPU Yu-Trinkle integration (c2gpu_yt) against the oracle's restatement of
+yt_integrate / yt_weights and the imtype_yt sums: labels bit-exact, the same
+attractors, weight sums within 1e-10 relative."""
+import numpy as np
+import pytest
+
+from critic2_amd import _lib
+from critic2_amd import bader as gb
+from critic2_amd import yt as gy
+from critic2_amd.synth import random_cell
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(ctx, cell, f, mode_kw, device=False):
+    vec, area = _lib.ws_facets(np.asarray(cell.x2c) / np.array(cell.n)[None, :])
+    mode, ra = gb._mode(mode_kw.get("nnm", False), mode_kw.get("ratom"), mode_kw.get("noatoms", False))
+    ref = oracle.yt(f, cell.n, cell.x2c, cell.xat, vec, area, fields=[f, f * f], mode=mode, ratom=ra)
+    if device:
+        df = ctx.to_device(f)
+        df2 = ctx.to_device(f * f)
+        got = gy.yt_integrate(ctx, df, cell.n, cell.x2c, cell.xat, fields=[df, df2], **mode_kw)
+        got["labels"] = got["labels"].numpy()
+    else:
+        got = gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat, fields=[f, f * f], **mode_kw)
+    return ref, got
+
+
+@pytest.mark.parametrize("seed,edge,ang,nat,n,mode_kw,device", [
+    (61, (8.0, 8.5, 9.0), (90, 90, 90), 4, (36, 40, 44), {}, False),
+    (62, (8.5, 8.0, 9.5), (80, 100, 105), 5, (40, 36, 44), {}, True),
+    (63, (9.0, 9.0, 9.0), (75, 95, 110), 6, (44, 44, 40), {"nnm": True, "ratom": 1.0}, False),
+    (64, (8.0, 9.0, 8.5), (88, 92, 97), 4, (36, 40, 40), {"noatoms": True, "ratom": 1.0}, False),
+])
+def test_yt_matches_oracle(ctx, seed, edge, ang, nat, n, mode_kw, device):
+    cell = random_cell(seed, edge, ang, nat, (1, 6, 7, 8), n)
+    print(f"[case] yt seed={seed} n={cell.n} mode={mode_kw} device={device}", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    ref, got = _run_both(ctx, cell, f, mode_kw, device)
+    idg, nattr, xattr, vol, sums = ref
+    assert got["nattr"] == nattr
+    np.testing.assert_array_equal(got["xattr"], xattr)
+    assert int(np.count_nonzero(got["labels"] != idg)) == 0
+    np.testing.assert_allclose(got["vol"], vol, rtol=1e-10)
+    np.testing.assert_allclose(got["sums"], sums, rtol=1e-10)
+    st = got["stats"]
+    assert st["nias"] == int(np.count_nonzero(idg == 0))
+    print(f"  nattr {nattr} surface points {st['nias']} label passes {st['label_passes']} "
+          f"weight passes {st['weight_passes']}", flush=True)
+
+
+def test_yt_refuses_equal_neighbours(ctx):
+    cell = random_cell(65, (8.0, 8.0, 8.0), (90, 90, 90), 2, (1, 6), (16, 16, 16))
+    f = np.ones(16 ** 3)
+    f[100] = 2.0
+    with pytest.raises(_lib.C2GpuError) as e:
+        gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat)
+    assert e.value.code == 8

@@ -1,0 +1,78 @@
+"""CPU checks of the Yu-Trinkle pieces: the Wigner-Seitz facets of the grid
+lattice (closed forms and the volume identity) and the oracle's restatement
+of yt_integrate / yt_weights (partition of unity, one-basin limit)."""
+import numpy as np
+import pytest
+
+from critic2_amd import _lib
+
+
+def _vol_from_facets(m, vec, area):
+    # the cell is the union of pyramids facet x (|v|/2): V = sum A |v| / 6
+    return float(np.sum(area * np.linalg.norm(vec @ np.asarray(m).T, axis=1)) / 6.0)
+
+
+def test_ws_orthorhombic_closed_form():
+    vec, area = _lib.ws_facets(np.diag([1.0, 2.0, 3.0]))
+    got = {tuple(int(x) for x in v): a for v, a in zip(vec, area)}
+    want = {(1, 0, 0): 6.0, (-1, 0, 0): 6.0, (0, 1, 0): 3.0, (0, -1, 0): 3.0, (0, 0, 1): 2.0, (0, 0, -1): 2.0}
+    assert got.keys() == want.keys()
+    for k in want:
+        assert got[k] == pytest.approx(want[k], rel=1e-13)
+
+
+def test_ws_fcc_bcc_hexagonal_closed_forms():
+    fcc = 0.5 * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], float).T
+    vec, area = _lib.ws_facets(fcc)  # rhombic dodecahedron
+    assert len(vec) == 12
+    np.testing.assert_allclose(area, np.sqrt(2) / 8, rtol=1e-12)
+    bcc = 0.5 * np.array([[-1, 1, 1], [1, -1, 1], [1, 1, -1]], float).T
+    vec, area = _lib.ws_facets(bcc)  # truncated octahedron: 6 squares, 8 hexagons
+    assert len(vec) == 14
+    a = np.sort(area)
+    np.testing.assert_allclose(a[:6], 1 / 8, rtol=1e-12)
+    np.testing.assert_allclose(a[6:], 3 * np.sqrt(3) / 16, rtol=1e-12)
+    hexl = np.array([[1, 0, 0], [-0.5, np.sqrt(3) / 2, 0], [0, 0, 1.6]]).T
+    vec, area = _lib.ws_facets(hexl)  # hexagonal prism
+    assert len(vec) == 8
+    a = np.sort(area)
+    np.testing.assert_allclose(a[:2], 3 * np.sqrt(3) / 2 * (1 / np.sqrt(3)) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(a[2:], 1.6 / np.sqrt(3), rtol=1e-12)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_ws_triclinic_volume_identity(seed):
+    rng = np.random.default_rng(seed)
+    from critic2_amd.synth import lattice_from_params
+    m = lattice_from_params(*rng.uniform(3, 6, 3), *rng.uniform(70, 110, 3)) / np.array([40, 44, 48])[None, :]
+    vec, area = _lib.ws_facets(m)
+    assert len(vec) in (6, 8, 10, 12, 14) and len(vec) % 2 == 0
+    assert _vol_from_facets(m, vec, area) == pytest.approx(abs(np.linalg.det(m)), rel=1e-10)
+    # facets come in +-v pairs of equal area
+    d = {tuple(int(x) for x in v): a for v, a in zip(vec, area)}
+    for v, a in d.items():
+        assert d[tuple(-x for x in v)] == pytest.approx(a, rel=1e-12)
+
+
+def test_oracle_yt_partition_of_unity_and_single_basin():
+    from oracle import oracle
+    from tests.helpers import small_cell
+    from critic2_amd.synth import BOHR_PER_ANG
+    cell = small_cell(5, (16, 18, 20), nat=3, ang=(80, 95, 105))
+    f = oracle.slater_grid(cell.x2c, cell.n, cell.xat, cell.zat, 8.0 * BOHR_PER_ANG)
+    vec, area = _lib.ws_facets(np.asarray(cell.x2c) / np.array(cell.n)[None, :])
+    idg, nattr, xattr, vol, sums = oracle.yt(f, cell.n, cell.x2c, cell.xat, vec, area, fields=[f])
+    N = f.size
+    assert vol.sum() == pytest.approx(N, rel=1e-12)  # every point's weights sum to one
+    assert sums[0].sum() == pytest.approx(f.sum(), rel=1e-12)
+    assert np.all(vol >= np.bincount(idg, minlength=nattr + 1)[1:] - 1e-9)
+    # one attractor: everything in it
+    single = np.ones(N)
+    x = np.indices(cell.n).reshape(3, -1).T / np.array(cell.n)
+    d = x - np.array([0.5, 0.5, 0.5])
+    single = np.exp(-np.sum((d @ np.asarray(cell.x2c).T) ** 2, axis=1) + 1e-9 * np.arange(N) / N)
+    # x fastest: the indices above are (i, j, k) with i slowest -> reorder
+    single = single.reshape(cell.n).transpose(2, 1, 0).reshape(-1)
+    idg1, n1, _, vol1, s1 = oracle.yt(single, cell.n, cell.x2c, [[0.5, 0.5, 0.5]], vec, area, fields=[single])
+    assert n1 == 1 and np.all(idg1 == 1)
+    assert vol1[0] == pytest.approx(N, rel=1e-14)
