@@ -12,16 +12,19 @@
 // refused: their order would be the reference sort's):
 //   k_yt_class    higher-neighbour mask per point, maxima listed;
 //   host          maxima in decreasing rho, attractor rules (as for Bader);
-//   k_yt_prop     per point the least and greatest basin reachable upwards,
-//                 propagated to a fixpoint; equal means the point's basin
-//                 (a point joins a basin exactly when every upward path ends
-//                 in it), different means a surface point;
-//   k_yt_weights  the surface points' weights, a sparse vector of up to 8
-//                 basins each, recomputed from the higher neighbours' until
-//                 no pass changes them (double-buffered); each basin's weight
-//                 sums the contributions in decreasing rho of the higher
-//                 neighbour, the order in which yt_weights adds them, so the
-//                 weights are the reference's bit for bit;
+//   k_yt_lab_step per point the least and greatest basin reachable upwards,
+//                 computed once, in topological order: a point is done when
+//                 its last higher neighbour is (a count of pending higher
+//                 neighbours per point, frontier after frontier from the
+//                 maxima); equal means the point's basin (a point joins a
+//                 basin exactly when every upward path ends in it),
+//                 different means a surface point;
+//   k_yt_w_step   the surface points' weights, a sparse vector of up to 8
+//                 basins each, in the same topological order over the
+//                 surface points; each basin's weight sums the contributions
+//                 in decreasing rho of the higher neighbour, the order in
+//                 which yt_weights adds them, so the weights are the
+//                 reference's bit for bit;
 //   sums          basin interiors through the per-basin sums kernel, surface
 //                 points through k_yt_surface_sums.
 #include <hip/hip_runtime.h>
@@ -84,8 +87,8 @@ __device__ __forceinline__ void yt_append(bool pred, uint32_t val, uint32_t *lis
 }
 
 __global__ void __launch_bounds__(YB) k_yt_class(YtGeo G, uint32_t N, const double *__restrict__ g,
-                                                  uint16_t *__restrict__ hmask, uint32_t *__restrict__ maxl,
-                                                  uint32_t *nmax, int *err) {
+                                                  uint16_t *__restrict__ hmask, uint32_t *__restrict__ pend,
+                                                  uint32_t *__restrict__ maxl, uint32_t *nmax, int *err) {
   const uint32_t p = blockIdx.x * YB + threadIdx.x;
   uint32_t mask = 0;
   if (p < N) {
@@ -99,6 +102,7 @@ __global__ void __launch_bounds__(YB) k_yt_class(YtGeo G, uint32_t N, const doub
       else if (gj == gp) tie = true;
     }
     hmask[p] = (uint16_t)mask;
+    pend[p] = (uint32_t)__popc(mask);
     if (tie) atomicOr(err, 1);
   }
   yt_append(p < N && mask == 0, p, maxl, nmax);
@@ -121,30 +125,47 @@ __global__ void k_yt_seed(uint32_t n, const uint32_t *__restrict__ maxl, const i
   if (t < n) lohi[maxl[t]] = ((uint64_t)(uint32_t)lab[t] << 32) | (uint32_t)lab[t];
 }
 
-// One pass of lo = min, hi = max over the higher neighbours, in place (the
-// values only move towards the fixpoint, so racing reads are harmless).
-__global__ void __launch_bounds__(YB) k_yt_prop(YtGeo G, uint32_t N, const uint16_t *__restrict__ hmask,
-                                                 uint64_t *lohi, int *changed) {
-  const uint32_t p = blockIdx.x * YB + threadIdx.x;
-  if (p >= N) return;
-  uint32_t mask = hmask[p];
-  if (!mask) return;
+__device__ __forceinline__ uint64_t yt_lohi_of(const YtGeo &G, uint32_t p, const uint16_t *__restrict__ hmask,
+                                                const uint64_t *lohi) {
   int i, j, k;
   coords3(G, p, i, j, k);
+  uint32_t mask = hmask[p];
   uint32_t lo = LO_NONE, hi = 0;
   while (mask) {
     const int q = __ffs(mask) - 1;
     mask &= mask - 1;
-    const uint64_t v = __hip_atomic_load(&lohi[nbr(G, i, j, k, q)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t v = lohi[nbr(G, i, j, k, q)];
     const uint32_t l = (uint32_t)(v >> 32), h = (uint32_t)v;
     lo = l < lo ? l : lo;
     hi = h > hi ? h : hi;
   }
-  const uint64_t nv = ((uint64_t)lo << 32) | hi;
-  if (nv != lohi[p]) {
-    __hip_atomic_store(&lohi[p], nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *changed = 1;
+  return ((uint64_t)lo << 32) | hi;
+}
+
+// One frontier of the labels: every point of `front` is final; each lower
+// neighbour loses one pending higher neighbour, and the thread that takes
+// the last computes it and puts it on the next frontier.
+__global__ void __launch_bounds__(YB) k_yt_lab_step(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
+                                                     const double *__restrict__ g,
+                                                     const uint16_t *__restrict__ hmask, uint32_t *__restrict__ pend,
+                                                     uint64_t *lohi, uint32_t *__restrict__ next, uint32_t *nnext) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  uint32_t done[14];
+  int nd = 0;
+  if (t < nf) {
+    const uint32_t p = front[t];
+    int i, j, k;
+    coords3(G, p, i, j, k);
+    const double gp = g[p];
+    for (int q = 0; q < G.nvec; q++) {
+      const uint32_t jn = nbr(G, i, j, k, q);
+      if (g[jn] < gp && atomicSub(&pend[jn], 1u) == 1u) {
+        lohi[jn] = yt_lohi_of(G, jn, hmask, lohi);
+        done[nd++] = jn;
+      }
+    }
   }
+  for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
 }
 
 __global__ void k_yt_label(uint32_t N, const uint64_t *__restrict__ lohi, int32_t *__restrict__ lab,
@@ -169,15 +190,11 @@ struct WBuf {
   uint8_t *c;   // entries in use
 };
 
-// Weights of each surface point from its higher neighbours' (buffer in),
-// written to buffer out; entries sorted by basin.
-__global__ void __launch_bounds__(YB) k_yt_weights(YtGeo G, uint32_t nias, const uint32_t *__restrict__ ias,
-                                                    const double *__restrict__ g, const uint16_t *__restrict__ hmask,
-                                                    const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
-                                                    WBuf in, WBuf out, int *changed, int *err) {
-  const uint32_t t = blockIdx.x * YB + threadIdx.x;
-  if (t >= nias) return;
-  const uint32_t p = ias[t];
+// Weights of surface point p (slot t) from its higher neighbours', all
+// final; entries sorted by basin.
+__device__ void yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const double *__restrict__ g,
+                                 const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
+                                 const int32_t *__restrict__ slot, WBuf W, int *err) {
   int i, j, k;
   coords3(G, p, i, j, k);
   const double gp = g[p];
@@ -235,12 +252,11 @@ __global__ void __launch_bounds__(YB) k_yt_weights(YtGeo G, uint32_t nias, const
       add((uint32_t)lj, fr * 1.0);
     } else {
       const int32_t s = slot[jn];
-      const int ce = in.c[s];
-      for (int e = 0; e < ce; e++) add(in.b[(size_t)KW * s + e], fr * in.w[(size_t)KW * s + e]);
+      const int ce = W.c[s];
+      for (int e = 0; e < ce; e++) add(W.b[(size_t)KW * s + e], fr * W.w[(size_t)KW * s + e]);
     }
   }
   if (over) atomicOr(err, 2);
-  // entries by basin
   for (int a = 1; a < na; a++) {
     const uint32_t kb = bb[a];
     const double kw = ww[a];
@@ -253,21 +269,64 @@ __global__ void __launch_bounds__(YB) k_yt_weights(YtGeo G, uint32_t nias, const
     bb[b] = kb;
     ww[b] = kw;
   }
-  bool same = in.c[t] == na;
-  for (int e = 0; e < na && same; e++)
-    same = in.b[(size_t)KW * t + e] == bb[e] &&
-           __double_as_longlong(in.w[(size_t)KW * t + e]) == __double_as_longlong(ww[e]);
-  out.c[t] = (uint8_t)na;
+  W.c[t] = (uint8_t)na;
   for (int e = 0; e < na; e++) {
-    out.b[(size_t)KW * t + e] = bb[e];
-    out.w[(size_t)KW * t + e] = ww[e];
+    W.b[(size_t)KW * t + e] = bb[e];
+    W.w[(size_t)KW * t + e] = ww[e];
   }
-  if (!same) *changed = 1;
 }
 
-__global__ void k_yt_clear(uint32_t n, uint8_t *__restrict__ c) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) c[t] = 0;
+// Pending surface neighbours above each surface point; the first frontier
+// (none pending) gets its weights here.
+__global__ void __launch_bounds__(YB) k_yt_w_init(YtGeo G, uint32_t nias, const uint32_t *__restrict__ ias,
+                                                   const double *__restrict__ g, const uint16_t *__restrict__ hmask,
+                                                   const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
+                                                   uint32_t *__restrict__ pend, WBuf W, uint32_t *__restrict__ front,
+                                                   uint32_t *nfront, int *err) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  bool ready = false;
+  uint32_t p = 0;
+  if (t < nias) {
+    p = ias[t];
+    int i, j, k;
+    coords3(G, p, i, j, k);
+    uint32_t mask = hmask[p], c = 0;
+    while (mask) {
+      const int q = __ffs(mask) - 1;
+      mask &= mask - 1;
+      if (lab[nbr(G, i, j, k, q)] == 0) c++;
+    }
+    pend[p] = c;
+    if (c == 0) {
+      yt_point_weights(G, p, t, g, hmask, lab, slot, W, err);
+      ready = true;
+    }
+  }
+  yt_append(ready, p, front, nfront);
+}
+
+__global__ void __launch_bounds__(YB) k_yt_w_step(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
+                                                   const double *__restrict__ g, const uint16_t *__restrict__ hmask,
+                                                   const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
+                                                   uint32_t *__restrict__ pend, WBuf W, uint32_t *__restrict__ next,
+                                                   uint32_t *nnext, int *err) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  uint32_t done[14];
+  int nd = 0;
+  if (t < nf) {
+    const uint32_t p = front[t];
+    int i, j, k;
+    coords3(G, p, i, j, k);
+    const double gp = g[p];
+    for (int q = 0; q < G.nvec; q++) {
+      const uint32_t jn = nbr(G, i, j, k, q);
+      if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) {
+        yt_point_weights(G, jn, (uint32_t)slot[jn], g, hmask, lab, slot, W, err);
+        done[nd++] = jn;
+      }
+    }
+  }
+  for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
 }
 
 struct YtFields {
@@ -349,7 +408,8 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
   C2_HIP(hipMemsetAsync(d_err, 0, 8, st));
   const unsigned gN = grid_for(N, YB);
-  C2_LAUNCH(ctx, k_yt_class, dim3(gN), dim3(YB), 0, G, N, g, hmask, maxl, d_cnt, d_err);
+  uint32_t *pend = buf<uint32_t>(ctx, "yt_pend", N);
+  C2_LAUNCH(ctx, k_yt_class, dim3(gN), dim3(YB), 0, G, N, g, hmask, pend, maxl, d_cnt, d_err);
   add_kernel_bytes(ctx, "k_yt_class", (double)N * (8.0 * (G.nvec + 1) + 2.0));
   const uint32_t nmax = read_scalar(ctx, d_cnt);
   if (read_scalar(ctx, d_err) & 1)
@@ -393,17 +453,22 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   C2_HIP(hipMemcpyAsync(d_mlab, mlab.data(), 4 * (size_t)nmax, hipMemcpyHostToDevice, st));
   C2_LAUNCH(ctx, k_yt_init, dim3(gN), dim3(YB), 0, N, lohi);
   if (nmax) C2_LAUNCH(ctx, k_yt_seed, dim3(grid_for(nmax, YB)), dim3(YB), 0, nmax, maxl, d_mlab, lohi);
+  // frontier after frontier from the maxima
+  uint32_t *fr[2] = {buf<uint32_t>(ctx, "yt_front0", N), buf<uint32_t>(ctx, "yt_front1", N)};
+  C2_HIP(hipMemcpyAsync(fr[0], maxl, 4 * (size_t)nmax, hipMemcpyDeviceToDevice, st));
+  uint32_t nf = nmax;
+  uint64_t ndone = nmax;
   int passes = 0;
-  for (;;) {
-    C2_HIP(hipMemsetAsync(d_err + 1, 0, 4, st));
-    for (int r = 0; r < 8; r++) {
-      C2_LAUNCH(ctx, k_yt_prop, dim3(gN), dim3(YB), 0, G, N, hmask, lohi, d_err + 1);
-      passes++;
-    }
-    add_kernel_bytes(ctx, "k_yt_prop", 8.0 * (double)N * (2.0 + 8.0 * G.nvec / 2.0));
-    if (!read_scalar(ctx, d_err + 1)) break;
-    if (passes > 4 * (n[0] + n[1] + n[2]) + 64) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_yt: label propagation");
+  while (nf) {
+    C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
+    C2_LAUNCH(ctx, k_yt_lab_step, dim3(grid_for(nf, YB)), dim3(YB), 0, G, nf, fr[passes & 1], g, hmask, pend, lohi,
+              fr[(passes & 1) ^ 1], d_cnt + 1);
+    add_kernel_bytes(ctx, "k_yt_lab_step", (double)nf * (4.0 + 8.0 * G.nvec * 2.0));
+    passes++;
+    nf = read_scalar(ctx, d_cnt + 1);
+    ndone += nf;
   }
+  if (ndone != N) fail(C2GPU_ERR_ARG, "c2gpu_yt: internal inconsistency in the label order");
   S.label_passes = passes;
   int32_t *lab = dev && label_out ? label_out : buf<int32_t>(ctx, "yt_lab", N);
   uint8_t *iasf = buf<uint8_t>(ctx, "yt_iasf", N);
@@ -411,28 +476,29 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   uint32_t *ias = buf<uint32_t>(ctx, "yt_ias", N);
   const uint32_t nias = compact_flags(ctx, "yt", iasf, N, ias);
   S.nias = nias;
-  // surface weights
-  WBuf W[2];
+  // surface weights, frontier after frontier over the surface points
+  WBuf W{};
   int wpasses = 0;
-  int cur = 0;
   if (nias) {
     int32_t *slot = buf<int32_t>(ctx, "yt_slot", N);
     C2_LAUNCH(ctx, k_yt_slots, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, slot);
-    for (int b = 0; b < 2; b++) {
-      W[b].b = buf<uint32_t>(ctx, "yt_wb" + std::to_string(b), (size_t)KW * nias);
-      W[b].w = buf<double>(ctx, "yt_ww" + std::to_string(b), (size_t)KW * nias);
-      W[b].c = buf<uint8_t>(ctx, "yt_wc" + std::to_string(b), nias);
-      C2_LAUNCH(ctx, k_yt_clear, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, W[b].c);
-    }
-    for (;;) {
-      C2_HIP(hipMemsetAsync(d_err + 1, 0, 4, st));
-      C2_LAUNCH(ctx, k_yt_weights, dim3(grid_for(nias, YB)), dim3(YB), 0, G, nias, ias, g, hmask, lab, slot, W[cur],
-                W[cur ^ 1], d_err + 1, d_err);
-      cur ^= 1;
+    W.b = buf<uint32_t>(ctx, "yt_wb", (size_t)KW * nias);
+    W.w = buf<double>(ctx, "yt_ww", (size_t)KW * nias);
+    W.c = buf<uint8_t>(ctx, "yt_wc", nias);
+    C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
+    C2_LAUNCH(ctx, k_yt_w_init, dim3(grid_for(nias, YB)), dim3(YB), 0, G, nias, ias, g, hmask, lab, slot, pend, W,
+              fr[0], d_cnt + 1, d_err);
+    uint32_t wf = read_scalar(ctx, d_cnt + 1);
+    uint64_t wdone = wf;
+    while (wf) {
+      C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
+      C2_LAUNCH(ctx, k_yt_w_step, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, fr[wpasses & 1], g, hmask, lab, slot,
+                pend, W, fr[(wpasses & 1) ^ 1], d_cnt + 1, d_err);
       wpasses++;
-      if (!read_scalar(ctx, d_err + 1)) break;
-      if (wpasses > 4 * (n[0] + n[1] + n[2]) + 64) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_yt: surface weights");
+      wf = read_scalar(ctx, d_cnt + 1);
+      wdone += wf;
     }
+    if (wdone != nias) fail(C2GPU_ERR_ARG, "c2gpu_yt: internal inconsistency in the weight order");
     if (read_scalar(ctx, d_err) & 2)
       fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: more than 8 basins meet at a surface point");
   }
@@ -446,8 +512,7 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   double *acc = buf<double>(ctx, "yt_acc", nb);
   C2_HIP(hipMemsetAsync(acc, 0, 8 * nb, st));
   if (nias)
-    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, W[cur], nattr, nfields, F,
-              acc);
+    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, W, nattr, nfields, F, acc);
   std::vector<double> h(nb);
   C2_HIP(hipMemcpyAsync(h.data(), acc, 8 * nb, hipMemcpyDeviceToHost, st));
   if (label_out && !dev) C2_HIP(hipMemcpyAsync(label_out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
