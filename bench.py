@@ -301,6 +301,73 @@ def run_cube(ctx, args):
     print(json.dumps(out), flush=True)
 
 
+def run_yt(ctx, args):
+    """Yu-Trinkle integration of the 512^3 workload's FP64 grid (resident in
+    HBM): basins, surface weights, and Volume/Pop weight sums through c2gpu_yt.
+    cpu_baseline: the oracle's restatement of yt_integrate/yt_weights on the
+    same cell at 96^3."""
+    from critic2_amd import yt as gy
+    cell = config(3, n_override=(args.n,) * 3)
+    N = cell.npts
+    rho = ctx.empty((N,), np.float64)
+    gb.synth_slater(ctx, cell, device_out=rho)
+    print(f"[bench] grid {cell.n} resident, {N} points", file=sys.stderr, flush=True)
+    res = None
+    for w in range(args.warmup):
+        res = gy.yt_integrate(ctx, rho, cell.n, cell.x2c, cell.xat, fields=[rho])
+        res["labels"].free()
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        res = gy.yt_integrate(ctx, rho, cell.n, cell.x2c, cell.xat, fields=[rho])
+        res["labels"].free()
+        print(f"[bench] step {s_ + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    kstats = ctx.kernel_stats()
+    ctx.timing(False)
+    reporting = {k: v for k, v in kstats.items() if v[2] > 0}
+    name = max(reporting.items(), key=lambda kv: kv[1][0])[0]
+    ms_tot, launches, nbytes = kstats[name]
+    avg_ms = ms_tot / launches
+    ach = nbytes / launches / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
+            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps,
+            "algorithmic_bytes_per_launch": int(nbytes / launches)}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        small = config(3, n_override=(96, 96, 96))
+        fs = gb.synth_slater(ctx, small)
+        vec, area = _lib.ws_facets(np.asarray(small.x2c) / np.array(small.n)[None, :])
+        t1 = time.perf_counter()
+        oracle.yt(fs, small.n, small.x2c, small.xat, vec, area, fields=[fs])
+        tc = time.perf_counter() - t1
+        cpu = {"value": small.npts / tc / 1e9, "unit": "Gpoints/s", "cores": 1, "kind": "port",
+               "sample": f"oracle yt_integrate + yt_weights + Volume/Pop sums on the same 30 A / 216-atom "
+                         f"cell at 96^3 ({small.npts} points), 1 thread"}
+    value = N / dt / 1e9
+    st = res["stats"]
+    out = {
+        "metric": f"Gpoints/s YT integration @{cell.n[0]}^3 (labels and weights exact)",
+        "value": round(value, 6), "unit": "Gpoints/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"cubic cell a=30 A, 216 seeded atoms, Slater-sum FP64 grid "
+                               f"{cell.n[0]}x{cell.n[1]}x{cell.n[2]}; YT basins + weights + Volume/Pop",
+                   "grid": list(cell.n), "atoms": int(len(cell.zat)), "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "yt": {k: st[k] for k in ("nattr", "nmaxima", "nias", "label_passes", "weight_passes")},
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in
+                                sorted(kstats.items(), key=lambda kv: -kv[1][0])[:8]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +380,8 @@ def main():
                     help="instead of the 512^3 step, time the batched path on this many dimer cells "
                          "(512 = the batching workload: edges from {48,64,96,128,160}, 6-40 atoms)")
     ap.add_argument("--nstreams", type=int, default=0, help="batched path: concurrent Bader workers (0: 8)")
+    ap.add_argument("--yt", action="store_true",
+                    help="instead of the Bader+NCI step, time Yu-Trinkle integration on the --n^3 grid")
     ap.add_argument("--cube", action="store_true",
                     help="instead of the Bader+NCI step, time the text cube parser on the --n^3 grid's "
                          "precisecube text")
@@ -329,6 +398,9 @@ def main():
         return
     if args.cube:
         run_cube(ctx, args)
+        return
+    if args.yt:
+        run_yt(ctx, args)
         return
     if args.copy_probe:
         probe = config(2)

@@ -48,7 +48,7 @@ void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const doubl
 namespace {
 
 constexpr int YB = 256;
-constexpr int KW = 8;  // basins per surface point
+constexpr int KW = 32;  // basins a surface point keeps in registers; wider points use a workgroup
 constexpr uint32_t LO_NONE = 0xffffffffu;
 constexpr double VSMALL = 1e-40;  // param.F90 vsmall
 
@@ -184,21 +184,26 @@ __global__ void k_yt_slots(uint32_t n, const uint32_t *__restrict__ ias, int32_t
   if (t < n) slot[ias[t]] = (int32_t)t;
 }
 
+// Sparse weight vectors of the surface points: slot t holds c[t] entries
+// (basin b, weight w) from off[t] in a pool filled by a bump allocator.
 struct WBuf {
-  uint32_t *b;  // KW basins per surface point
-  double *w;    // KW weights
-  uint8_t *c;   // entries in use
+  uint32_t *b;
+  double *w;
+  uint64_t *off;
+  uint16_t *c;
+  unsigned long long *top;  // entries handed out
+  uint64_t cap;             // pool entries
 };
 
-// Weights of surface point p (slot t) from its higher neighbours', all
-// final; entries sorted by basin.
-__device__ void yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const double *__restrict__ g,
-                                 const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
-                                 const int32_t *__restrict__ slot, WBuf W, int *err) {
+// The higher neighbours of p in decreasing rho (hq) with their fractions
+// chi_k / max(sum chi, vsmall), chi_k = max(area_k (rho_j - rho_p), vsmall)
+// summed in facet order (yt@proc.f90:108-119, :175).
+__device__ __forceinline__ int yt_fracs(const YtGeo &G, uint32_t p, const double *__restrict__ g,
+                                        const uint16_t *__restrict__ hmask, uint32_t hq[14], double fr[14]) {
   int i, j, k;
   coords3(G, p, i, j, k);
   const double gp = g[p];
-  uint32_t hq[14];
+  uint32_t q0[14];
   double hg[14], chi[14];
   int nh = 0;
   double csum = 0.0;
@@ -208,13 +213,13 @@ __device__ void yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const d
     const uint32_t jn = nbr(G, i, j, k, q);
     const double gj = g[jn];
     const double c = fmax(G.al[q] * (gj - gp), VSMALL);
-    hq[nh] = jn;
+    q0[nh] = jn;
     hg[nh] = gj;
     chi[nh] = c;
     csum = csum + c;
     nh++;
   }
-  // contributions in decreasing rho of the higher neighbour
+  const double den = fmax(csum, VSMALL);
   int ord[14];
   for (int a = 0; a < nh; a++) {
     int b = a;
@@ -224,6 +229,22 @@ __device__ void yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const d
     }
     ord[b] = a;
   }
+  for (int a = 0; a < nh; a++) {
+    hq[a] = q0[ord[a]];
+    fr[a] = chi[ord[a]] / den;
+  }
+  return nh;
+}
+
+// Weights of surface point p (slot t) from its higher neighbours', all
+// final, accumulated in registers; entries sorted by basin.  Returns false
+// (nothing written) when more than KW basins meet at p.
+__device__ bool yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const double *__restrict__ g,
+                                 const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
+                                 const int32_t *__restrict__ slot, WBuf W, int *err) {
+  uint32_t hq[14];
+  double fr[14];
+  const int nh = yt_fracs(G, p, g, hmask, hq, fr);
   uint32_t bb[KW];
   double ww[KW];
   int na = 0;
@@ -242,21 +263,19 @@ __device__ void yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const d
       over = true;
     }
   };
-  const double den = fmax(csum, VSMALL);
-  for (int a = 0; a < nh; a++) {
-    const int o = ord[a];
-    const double fr = chi[o] / den;
-    const uint32_t jn = hq[o];
+  for (int a = 0; a < nh && !over; a++) {
+    const uint32_t jn = hq[a];
     const int32_t lj = lab[jn];
     if (lj != 0) {
-      add((uint32_t)lj, fr * 1.0);
+      add((uint32_t)lj, fr[a] * 1.0);
     } else {
       const int32_t s = slot[jn];
       const int ce = W.c[s];
-      for (int e = 0; e < ce; e++) add(W.b[(size_t)KW * s + e], fr * W.w[(size_t)KW * s + e]);
+      const uint64_t so = W.off[s];
+      for (int e = 0; e < ce; e++) add(W.b[so + e], fr[a] * W.w[so + e]);
     }
   }
-  if (over) atomicOr(err, 2);
+  if (over) return false;
   for (int a = 1; a < na; a++) {
     const uint32_t kb = bb[a];
     const double kw = ww[a];
@@ -269,20 +288,119 @@ __device__ void yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const d
     bb[b] = kb;
     ww[b] = kw;
   }
-  W.c[t] = (uint8_t)na;
+  const uint64_t o = atomicAdd(W.top, (unsigned long long)na);
+  if (o + (uint64_t)na > W.cap) {  // pool full: the host enlarges it and runs the weights again
+    atomicOr(err, 4);
+    W.c[t] = 0;
+    W.off[t] = 0;
+    return true;
+  }
+  W.c[t] = (uint16_t)na;
+  W.off[t] = o;
   for (int e = 0; e < na; e++) {
-    W.b[(size_t)KW * t + e] = bb[e];
-    W.w[(size_t)KW * t + e] = ww[e];
+    W.b[o + e] = bb[e];
+    W.w[o + e] = ww[e];
+  }
+  return true;
+}
+
+// The frontier's weights, one thread per point; points where more than KW
+// basins meet go to the wide list.
+__global__ void __launch_bounds__(YB) k_yt_w_narrow(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
+                                                     const double *__restrict__ g, const uint16_t *__restrict__ hmask,
+                                                     const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
+                                                     WBuf W, uint32_t *__restrict__ wide, uint32_t *nwide, int *err) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  bool w = false;
+  uint32_t p = 0;
+  if (t < nf) {
+    p = front[t];
+    w = !yt_point_weights(G, p, (uint32_t)slot[p], g, hmask, lab, slot, W, err);
+  }
+  yt_append(w, p, wide, nwide);
+}
+
+// Wide points, one workgroup each: a dense accumulator per basin in LDS; the
+// neighbours are added one after the other in decreasing rho (a neighbour's
+// entries have distinct basins, so its additions do not collide), then the
+// basins present are written in increasing order.
+__global__ void __launch_bounds__(YB) k_yt_w_wide(YtGeo G, const uint32_t *__restrict__ wide, int nattr,
+                                                   const double *__restrict__ g, const uint16_t *__restrict__ hmask,
+                                                   const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
+                                                   WBuf W, int *err) {
+  extern __shared__ double acc[];  // nattr doubles, then nattr flags
+  uint8_t *pres = reinterpret_cast<uint8_t *>(acc + nattr);
+  __shared__ uint32_t hq[14];
+  __shared__ double fr[14];
+  __shared__ int nh;
+  __shared__ uint32_t wsum[YB / 64];
+  __shared__ unsigned long long base;
+  const uint32_t p = wide[blockIdx.x];
+  for (int a = threadIdx.x; a < nattr; a += YB) {
+    acc[a] = 0.0;
+    pres[a] = 0;
+  }
+  if (threadIdx.x == 0) nh = yt_fracs(G, p, g, hmask, hq, fr);
+  __syncthreads();
+  for (int a = 0; a < nh; a++) {
+    const uint32_t jn = hq[a];
+    const int32_t lj = lab[jn];
+    if (lj != 0) {
+      if (threadIdx.x == 0) {
+        acc[lj - 1] = acc[lj - 1] + fr[a] * 1.0;
+        pres[lj - 1] = 1;
+      }
+    } else {
+      const int32_t s = slot[jn];
+      const int ce = W.c[s];
+      const uint64_t so = W.off[s];
+      for (int e = threadIdx.x; e < ce; e += YB) {
+        const int b = (int)W.b[so + e] - 1;
+        acc[b] = acc[b] + fr[a] * W.w[so + e];
+        pres[b] = 1;
+      }
+    }
+    __syncthreads();
+  }
+  // the basins present, in order: block-wide count, then positions
+  uint32_t mine = 0;
+  for (int a = threadIdx.x; a < nattr; a += YB) mine += pres[a];
+  uint32_t v = mine;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const uint32_t na = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  const uint32_t t = (uint32_t)slot[p];
+  if (threadIdx.x == 0) {
+    base = atomicAdd(W.top, (unsigned long long)na);
+    if (base + na > W.cap) {
+      atomicOr(err, 4);
+      W.c[t] = 0;
+      W.off[t] = 0;
+    } else {
+      W.c[t] = (uint16_t)na;
+      W.off[t] = base;
+    }
+  }
+  __syncthreads();
+  if (base + na > W.cap) return;
+  if (threadIdx.x == 0) {  // serial compaction keeps increasing basin order
+    uint64_t o = base;
+    for (int a = 0; a < nattr; a++)
+      if (pres[a]) {
+        W.b[o] = (uint32_t)(a + 1);
+        W.w[o] = acc[a];
+        o++;
+      }
   }
 }
 
-// Pending surface neighbours above each surface point; the first frontier
-// (none pending) gets its weights here.
+// Pending surface neighbours above each surface point; those with none
+// form the first frontier.
 __global__ void __launch_bounds__(YB) k_yt_w_init(YtGeo G, uint32_t nias, const uint32_t *__restrict__ ias,
-                                                   const double *__restrict__ g, const uint16_t *__restrict__ hmask,
-                                                   const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
-                                                   uint32_t *__restrict__ pend, WBuf W, uint32_t *__restrict__ front,
-                                                   uint32_t *nfront, int *err) {
+                                                   const uint16_t *__restrict__ hmask,
+                                                   const int32_t *__restrict__ lab, uint32_t *__restrict__ pend,
+                                                   uint32_t *__restrict__ front, uint32_t *nfront) {
   const uint32_t t = blockIdx.x * YB + threadIdx.x;
   bool ready = false;
   uint32_t p = 0;
@@ -297,19 +415,17 @@ __global__ void __launch_bounds__(YB) k_yt_w_init(YtGeo G, uint32_t nias, const 
       if (lab[nbr(G, i, j, k, q)] == 0) c++;
     }
     pend[p] = c;
-    if (c == 0) {
-      yt_point_weights(G, p, t, g, hmask, lab, slot, W, err);
-      ready = true;
-    }
+    ready = c == 0;
   }
   yt_append(ready, p, front, nfront);
 }
 
+// The next frontier: each surface point below a frontier point loses one
+// pending neighbour; the thread that takes the last lists it.
 __global__ void __launch_bounds__(YB) k_yt_w_step(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
-                                                   const double *__restrict__ g, const uint16_t *__restrict__ hmask,
-                                                   const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
-                                                   uint32_t *__restrict__ pend, WBuf W, uint32_t *__restrict__ next,
-                                                   uint32_t *nnext, int *err) {
+                                                   const double *__restrict__ g, const int32_t *__restrict__ lab,
+                                                   uint32_t *__restrict__ pend, uint32_t *__restrict__ next,
+                                                   uint32_t *nnext) {
   const uint32_t t = blockIdx.x * YB + threadIdx.x;
   uint32_t done[14];
   int nd = 0;
@@ -320,10 +436,7 @@ __global__ void __launch_bounds__(YB) k_yt_w_step(YtGeo G, uint32_t nf, const ui
     const double gp = g[p];
     for (int q = 0; q < G.nvec; q++) {
       const uint32_t jn = nbr(G, i, j, k, q);
-      if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) {
-        yt_point_weights(G, jn, (uint32_t)slot[jn], g, hmask, lab, slot, W, err);
-        done[nd++] = jn;
-      }
+      if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
     }
   }
   for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
@@ -333,20 +446,31 @@ struct YtFields {
   const double *p[4];
 };
 
-// Surface points' weighted sums: acc[q*nattr + a], q = 0 the weights.
+// Surface points' weighted sums: acc[q*nattr + a], q = 0 the weights;
+// binned per workgroup in LDS, one global addition per bin and workgroup.
 __global__ void __launch_bounds__(YB) k_yt_surface_sums(uint32_t nias, const uint32_t *__restrict__ ias, WBuf W,
                                                          int nattr, int nf, YtFields F, double *__restrict__ acc) {
-  const uint32_t t = blockIdx.x * YB + threadIdx.x;
-  if (t >= nias) return;
-  const uint32_t p = ias[t];
-  const int ce = W.c[t];
-  for (int e = 0; e < ce; e++) {
-    const int a = (int)W.b[(size_t)KW * t + e] - 1;
-    const double w = W.w[(size_t)KW * t + e];
-    if (a < 0 || a >= nattr) continue;
-    atomicAdd(&acc[a], w);
-    for (int q = 0; q < nf; q++) atomicAdd(&acc[(size_t)(q + 1) * nattr + a], w * F.p[q][p]);
+  extern __shared__ double bins[];  // (nf + 1) * nattr
+  const int nb = (nf + 1) * nattr;
+  for (int q = threadIdx.x; q < nb; q += YB) bins[q] = 0.0;
+  __syncthreads();
+  for (uint32_t t = blockIdx.x * YB + threadIdx.x; t < nias; t += gridDim.x * YB) {
+    const uint32_t p = ias[t];
+    const int ce = W.c[t];
+    const uint64_t so = W.off[t];
+    double fv[4];
+    for (int q = 0; q < nf; q++) fv[q] = F.p[q][p];
+    for (int e = 0; e < ce; e++) {
+      const int a = (int)W.b[so + e] - 1;
+      const double w = W.w[so + e];
+      if (a < 0 || a >= nattr) continue;
+      atomicAdd(&bins[a], w);
+      for (int q = 0; q < nf; q++) atomicAdd(&bins[(q + 1) * nattr + a], w * fv[q]);
+    }
   }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nb; q += YB)
+    if (bins[q] != 0.0) atomicAdd(&acc[q], bins[q]);
 }
 
 }  // namespace
@@ -476,31 +600,60 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   uint32_t *ias = buf<uint32_t>(ctx, "yt_ias", N);
   const uint32_t nias = compact_flags(ctx, "yt", iasf, N, ias);
   S.nias = nias;
-  // surface weights, frontier after frontier over the surface points
+  // surface weights, frontier after frontier over the surface points; the
+  // pool of weight entries starts at 8 per point and grows if it fills
   WBuf W{};
   int wpasses = 0;
   if (nias) {
     int32_t *slot = buf<int32_t>(ctx, "yt_slot", N);
     C2_LAUNCH(ctx, k_yt_slots, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, slot);
-    W.b = buf<uint32_t>(ctx, "yt_wb", (size_t)KW * nias);
-    W.w = buf<double>(ctx, "yt_ww", (size_t)KW * nias);
-    W.c = buf<uint8_t>(ctx, "yt_wc", nias);
-    C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
-    C2_LAUNCH(ctx, k_yt_w_init, dim3(grid_for(nias, YB)), dim3(YB), 0, G, nias, ias, g, hmask, lab, slot, pend, W,
-              fr[0], d_cnt + 1, d_err);
-    uint32_t wf = read_scalar(ctx, d_cnt + 1);
-    uint64_t wdone = wf;
-    while (wf) {
+    W.off = buf<uint64_t>(ctx, "yt_woff", nias);
+    W.c = buf<uint16_t>(ctx, "yt_wc", nias);
+    W.top = buf<unsigned long long>(ctx, "yt_wtop", 1);
+    uint32_t *wide = buf<uint32_t>(ctx, "yt_wide", nias);
+    const size_t wide_lds = 9 * (size_t)nattr;
+    if (wide_lds > 120 * 1024) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: too many attractors for the LDS accumulator");
+    uint64_t cap = 8ull * nias;
+    for (;;) {
+      S.wide = 0;
+      W.cap = cap;
+      W.b = buf<uint32_t>(ctx, "yt_wb", cap);
+      W.w = buf<double>(ctx, "yt_ww", cap);
+      C2_HIP(hipMemsetAsync(W.top, 0, 8, st));
+      C2_HIP(hipMemsetAsync(d_err, 0, 4, st));
       C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
-      C2_LAUNCH(ctx, k_yt_w_step, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, fr[wpasses & 1], g, hmask, lab, slot,
-                pend, W, fr[(wpasses & 1) ^ 1], d_cnt + 1, d_err);
-      wpasses++;
-      wf = read_scalar(ctx, d_cnt + 1);
-      wdone += wf;
+      C2_LAUNCH(ctx, k_yt_w_init, dim3(grid_for(nias, YB)), dim3(YB), 0, G, nias, ias, hmask, lab, pend, fr[0],
+                d_cnt + 1);
+      uint32_t wf = read_scalar(ctx, d_cnt + 1);
+      uint64_t wdone = wf;
+      wpasses = 0;
+      while (wf) {
+        uint32_t *cf = fr[wpasses & 1], *nx = fr[(wpasses & 1) ^ 1];
+        C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
+        C2_LAUNCH(ctx, k_yt_w_narrow, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, cf, g, hmask, lab, slot, W, wide,
+                  d_cnt, d_err);
+        // each point: its and its higher neighbours' rho, masks, labels and slots
+        add_kernel_bytes(ctx, "k_yt_w_narrow", (double)wf * (4.0 + 2.0 + 8.0 * (G.nvec + 1) + 8.0 * G.nvec));
+        C2_LAUNCH(ctx, k_yt_w_step, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, cf, g, lab, pend, nx, d_cnt + 1);
+        uint32_t cnt2[2];
+        C2_HIP(hipMemcpyAsync(cnt2, d_cnt, 8, hipMemcpyDeviceToHost, st));
+        C2_HIP(hipStreamSynchronize(st));
+        if (cnt2[0]) {
+          C2_LAUNCH(ctx, k_yt_w_wide, dim3(cnt2[0]), dim3(YB), wide_lds, G, wide, nattr, g, hmask, lab, slot, W,
+                    d_err);
+          S.wide += cnt2[0];
+        }
+        wpasses++;
+        wf = cnt2[1];
+        wdone += wf;
+      }
+      if (wdone != nias) fail(C2GPU_ERR_ARG, "c2gpu_yt: internal inconsistency in the weight order");
+      // weight entries: each written once and read back by the points below
+      add_kernel_bytes(ctx, "k_yt_w_narrow", 24.0 * (double)read_scalar(ctx, W.top));
+      const int e = read_scalar(ctx, d_err);
+      if (!(e & 4)) break;
+      cap = read_scalar(ctx, W.top) + (cap >> 1);  // every entry asked for, and room to spare
     }
-    if (wdone != nias) fail(C2GPU_ERR_ARG, "c2gpu_yt: internal inconsistency in the weight order");
-    if (read_scalar(ctx, d_err) & 2)
-      fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: more than 8 basins meet at a surface point");
   }
   S.weight_passes = wpasses;
   // sums: interiors, then surface points
@@ -509,10 +662,15 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   basin_sums_run(ctx, lab, N, nattr, nfields, F.p, C2GPU_PTR_DEVICE, nfields ? isums.data() : nullptr,
                  counts.data());
   const size_t nb = (size_t)(nfields + 1) * nattr;
+  if (8 * nb > 60000) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: too many attractors x fields for the LDS sums");
   double *acc = buf<double>(ctx, "yt_acc", nb);
   C2_HIP(hipMemsetAsync(acc, 0, 8 * nb, st));
-  if (nias)
-    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(grid_for(nias, YB)), dim3(YB), 0, nias, ias, W, nattr, nfields, F, acc);
+  if (nias) {
+    const unsigned gs = std::min<unsigned>(grid_for(nias, YB), (unsigned)persistent_threads(ctx->device, YB) / YB);
+    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(gs), dim3(YB), 8 * nb, nias, ias, W, nattr, nfields, F, acc);
+    add_kernel_bytes(ctx, "k_yt_surface_sums", (double)nias * (14.0 + 8.0 * nfields) +
+                                                   12.0 * (double)read_scalar(ctx, W.top));
+  }
   std::vector<double> h(nb);
   C2_HIP(hipMemcpyAsync(h.data(), acc, 8 * nb, hipMemcpyDeviceToHost, st));
   if (label_out && !dev) C2_HIP(hipMemcpyAsync(label_out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));

@@ -326,6 +326,7 @@ typedef struct {
   int64_t nias;       /* points on the interatomic surfaces (basin 0)      */
   int label_passes;   /* propagation passes for the labels                 */
   int weight_passes;  /* propagation passes for the surface weights        */
+  int64_t wide;       /* surface points where more than 32 basins meet     */
   double ms_total;
 } c2gpu_yt_stats;
 

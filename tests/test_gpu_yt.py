@@ -50,6 +50,22 @@ def test_yt_matches_oracle(ctx, seed, edge, ang, nat, n, mode_kw, device):
           f"weight passes {st['weight_passes']}", flush=True)
 
 
+def test_yt_many_basins_meet_wide_path(ctx):
+    """40 atoms in a small cell: surface points where more than 32 basins
+    meet take the workgroup accumulator; still the oracle's weights."""
+    cell = random_cell(66, (9.0, 9.5, 10.0), (85, 95, 100), 40, (1, 6, 7, 8), (40, 44, 48))
+    print(f"[case] yt wide n={cell.n} atoms=40", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    ref, got = _run_both(ctx, cell, f, {})
+    idg, nattr, xattr, vol, sums = ref
+    assert got["nattr"] == nattr
+    assert int(np.count_nonzero(got["labels"] != idg)) == 0
+    np.testing.assert_allclose(got["vol"], vol, rtol=1e-10)
+    np.testing.assert_allclose(got["sums"], sums, rtol=1e-10)
+    print(f"  wide points {got['stats']['wide']}", flush=True)
+    assert got["stats"]["wide"] > 0
+
+
 def test_yt_refuses_equal_neighbours(ctx):
     cell = random_cell(65, (8.0, 8.0, 8.0), (90, 90, 90), 2, (1, 6), (16, 16, 16))
     f = np.ones(16 ** 3)
