@@ -35,7 +35,7 @@ EXPORTS = (
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
     "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff", "c2gpu_ws_facets",
-    "c2gpu_yt",
+    "c2gpu_yt", "c2gpu_multipoles",
 )
 
 
@@ -133,6 +133,7 @@ def load():
         "c2gpu_nci_cutoff": [vp, vp, vp, vp, vp, i, C.POINTER(NciCut), i, vp, C.c_uint64, C.POINTER(C.c_uint64),
                              vp],
         "c2gpu_ws_facets": [vp, C.POINTER(i), vp, vp],
+        "c2gpu_multipoles": [vp, vp, vp, vp, vp, i, vp, i, i, vp],
         "c2gpu_yt": [vp, vp, vp, vp, C.POINTER(AttractorOpts), i, vp, i, vp, C.POINTER(i), vp, i, vp, vp,
                      C.POINTER(YtStats)],
     }

@@ -178,3 +178,25 @@ def synth_slater(ctx: Context, cell, rimg_ang=8.0, device_out=None):
 
 __all__ = ["BasinData", "bader_integrate", "basin_sums", "intgrid_fields", "bader", "fft_deriv",
            "synth_slater", "Context", "C2GpuError"]
+
+
+def multipoles(ctx: Context, labels, f, n, x2c, xattr, lmax: int = 5):
+    """Basin multipoles of the field f (INTEGRABLE f MULTIPOLE lmax,
+    integration@proc.f90:1340-1362): array (nattr, (lmax+1)^2) of sums of
+    the real solid harmonics (genrlm_real order) times f over each basin,
+    unscaled (critic2 multiplies by Omega/N).  labels and f: host arrays or
+    DeviceArrays; xattr fractional (nattr, 3)."""
+    n = tuple(int(v) for v in n)
+    xattr = np.ascontiguousarray(xattr, dtype=np.float64).reshape(-1, 3)
+    nattr = xattr.shape[0]
+    nlm = (lmax + 1) ** 2
+    dev = hasattr(labels, "data_ptr")
+    if not dev:
+        labels = np.ascontiguousarray(labels, dtype=np.int32).reshape(-1)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    out = np.zeros(nattr * nlm)
+    nn = np.asarray(n, dtype=np.int32)
+    ctx.check(ctx.lib.c2gpu_multipoles(ctx.h, _lib.ptr(labels), _lib.ptr(f), nn.ctypes.data,
+                                       _lib.mat_f(x2c).ctypes.data, nattr, xattr.ctypes.data, lmax,
+                                       _lib.PTR_DEVICE if dev else 0, out.ctypes.data), "c2gpu_multipoles")
+    return out.reshape(nattr, nlm)

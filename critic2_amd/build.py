@@ -26,6 +26,7 @@ SOURCES = [
     ("cube.hip", ["-ffp-contract=off"]),
     ("ncicut.hip", ["-ffp-contract=off"]),
     ("yt.hip", ["-ffp-contract=off"]),
+    ("mpole.hip", ["-ffp-contract=off"]),
     ("wscell.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),

@@ -309,6 +309,18 @@ int c2gpu_batch_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int attr
                     int nspecies, const c2gpu_radial_table *tables, int nstreams, int flags,
                     c2gpu_batch_stats *stats);
 
+/* Basin multipoles (INTEGRABLE f MULTIPOLE lmax, integration@proc.f90:
+ * 1302-1362, the Bader branch): mpole_out[a*(lmax+1)^2 + q] = sum over the
+ * points of basin a+1 of rrlm(q) f, rrlm the real regular solid harmonics
+ * of genrlm_real (tools_math@proc.f90:273-306; order C00, C11, C10, S11,
+ * C22, C21, C20, S21, S22, ...) at the shortest lattice translate of
+ * p/n - xattr(a) (Cartesian).  Unscaled (critic2 multiplies by Omega/N);
+ * the layout is critic2's mpole((lmax+1)**2, nattr).  xattr: fractional,
+ * 3 per attractor, host memory.  0 <= lmax <= 10.  With C2GPU_PTR_DEVICE
+ * label and f are device memory. */
+int c2gpu_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
+                     int nattr, const double *xattr, int lmax, int flags, double *mpole_out);
+
 /* ------------------------------------------------------------------ */
 /* Yu-Trinkle integration                                               */
 /* ------------------------------------------------------------------ */

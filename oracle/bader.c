@@ -805,3 +805,128 @@ int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, cons
   free(fnear);
   return rc;
 }
+
+/* ==================================================================== */
+/* Basin multipoles (integration@proc.f90:1340-1362, Bader branch):     */
+/* shortest (crystalmod@proc.f90:1054-1085), tosphere and genylm /      */
+/* genrlm_real (tools_math@proc.f90:273-406), in the reference's route  */
+/* through spherical angles.                                            */
+/* ==================================================================== */
+
+#include <complex.h>
+
+/* genylm: spherical harmonics with the Condon-Shortley phase, packed
+ * j = l(l+1)+m (0-based), Masters and Richards-Dinger recursion. */
+static void genylm(int lmax, const double tp[2], double complex *ylm) {
+  const double fourpi = 12.566370614359172954;
+  double x[64];
+  double complex z[64];
+  ylm[0] = 0.28209479177387814347;
+  if (lmax == 0) return;
+  const double sn = sin(tp[0]), cs = cos(tp[0]);
+  for (int m = 1; m <= lmax; m++) {
+    const double t1 = (double)m * tp[1];
+    z[m] = cos(t1) + I * sin(t1);
+  }
+  for (int l = 1; l <= lmax; l++) {
+    x[l] = (l % 2 == 0) ? 1.0 : -1.0;
+    double dx = 0.0;
+    for (int m = l; m >= 1; m--) {
+      const double t1 = sqrt((double)((l + m) * (l - m + 1)));
+      x[m - 1] = -(sn * dx + (double)(2 * m) * cs * x[m]) / t1;
+      dx = sn * x[m] * t1;
+    }
+    double t1 = sn, sum = 0.0;
+    for (int m = 1; m <= l; m++) {
+      x[m] = t1 * x[m];
+      sum = sum + x[m] * x[m];
+      t1 = t1 * sn;
+    }
+    sum = 2.0 * sum + x[0] * x[0];
+    t1 = sqrt((double)(2 * l + 1) / (fourpi * sum));
+    int lm1 = l * (l + 1), lm2 = lm1;
+    ylm[lm1] = t1 * x[0];
+    for (int m = 1; m <= l; m++) {
+      lm1++;
+      lm2--;
+      ylm[lm1] = t1 * x[m] * z[m];
+      ylm[lm2] = conj(ylm[lm1]);
+      if (m % 2 != 0) ylm[lm2] = -ylm[lm2];
+    }
+  }
+}
+
+/* genrlm_real: real regular solid harmonics from (r, theta, phi). */
+void oracle_genrlm_real(int lmax, double r, const double tp[2], double *rrlm) {
+  double complex rlm[(16 + 1) * (16 + 1)];
+  const double pi = 3.14159265358979323846, sh = 1.0 / sqrt(2.0);
+  genylm(lmax, tp, rlm);
+  for (int l = 0; l <= lmax; l++) {
+    const int imin = l * l, imax = (l + 1) * (l + 1);
+    const double sc = sqrt(4.0 * pi / (2 * l + 1)) * pow(r, l);
+    for (int q = imin; q < imax; q++) rlm[q] = rlm[q] * sc;
+    rrlm[imin + l] = creal(rlm[imin + l]);
+    for (int m = 1; m <= l; m++) {
+      const int ip = imin + l + m, im = imin + l - m;
+      const double iphas = (m % 2 == 0) ? 1.0 : -1.0;
+      const double complex a = iphas * rlm[ip] + rlm[im];
+      const double complex b = -iphas * I * rlm[ip] + I * rlm[im];
+      rrlm[im] = sh * creal(a);
+      rrlm[ip] = sh * creal(b);
+    }
+  }
+}
+
+/* tosphere */
+void oracle_tosphere(const double v[3], double *r, double tp[2]) {
+  const double eps = 1e-14, pi = 3.14159265358979323846;
+  *r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  tp[0] = tp[1] = 0.0;
+  if (*r > eps) {
+    const double t1 = v[2] / *r;
+    if (t1 >= 1.0) tp[0] = 0.0;
+    else if (t1 <= -1.0) tp[0] = pi;
+    else tp[0] = acos(t1);
+    if (fabs(v[0]) > eps || fabs(v[1]) > eps) tp[1] = atan2(v[1], v[0]);
+  }
+}
+
+/* mpole[a * nlm + q], a the basin (0-based), unscaled (critic2 multiplies
+ * by Omega/N).  shortest: nint fold, then the shortest translate over
+ * -2..2 (first strictly shorter). */
+void oracle_multipoles(const int32_t *idg, const double *f, const int n[3], const double x2c[9], int nattr,
+                       const double *xattr, int lmax, double *mpole) {
+  const int nlm = (lmax + 1) * (lmax + 1);
+  double rrlm[17 * 17];
+  for (int q = 0; q < nattr * nlm; q++) mpole[q] = 0.0;
+  for (int64_t k = 0; k < n[2]; k++)
+    for (int64_t j = 0; j < n[1]; j++)
+      for (int64_t i = 0; i < n[0]; i++) {
+        const int64_t p = i + n[0] * (j + n[1] * k);
+        const int ix = idg[p];
+        if (ix < 1 || ix > nattr) continue;
+        double dv[3] = {(double)i / n[0] - xattr[3 * (ix - 1)], (double)j / n[1] - xattr[3 * (ix - 1) + 1],
+                        (double)k / n[2] - xattr[3 * (ix - 1) + 2]};
+        for (int c = 0; c < 3; c++) dv[c] = dv[c] - (double)lround(dv[c]);
+        double v[3];
+        x2c_apply(x2c, dv, v);
+        double best = v[0] * v[0] + v[1] * v[1] + v[2] * v[2], bv[3] = {v[0], v[1], v[2]};
+        for (int a = -2; a <= 2; a++)
+          for (int b = -2; b <= 2; b++)
+            for (int c = -2; c <= 2; c++) {
+              double xt[3] = {dv[0] + a, dv[1] + b, dv[2] + c}, t[3];
+              x2c_apply(x2c, xt, t);
+              const double d = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+              if (d < best) {
+                best = d;
+                bv[0] = t[0];
+                bv[1] = t[1];
+                bv[2] = t[2];
+              }
+            }
+        double r, tp[2];
+        oracle_tosphere(bv, &r, tp);
+        oracle_genrlm_real(lmax, r, tp, rrlm);
+        for (int q = 0; q < nlm; q++) mpole[(ix - 1) * nlm + q] += rrlm[q] * f[p];
+      }
+}

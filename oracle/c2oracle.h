@@ -144,6 +144,13 @@ int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, cons
               int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
               double *vol, double *sums);
 
+/* Basin multipoles (integration@proc.f90:1340-1362): mpole[a*nlm + q],
+ * nlm = (lmax+1)^2, unscaled.  genrlm_real / tosphere restated for tests. */
+void oracle_multipoles(const int32_t *idg, const double *f, const int n[3], const double x2c[9], int nattr,
+                       const double *xattr, int lmax, double *mpole);
+void oracle_genrlm_real(int lmax, double r, const double tp[2], double *rrlm);
+void oracle_tosphere(const double v[3], double *r, double tp[2]);
+
 /* Cube readers (grid3mod@proc.f90:512-635); f == NULL: header only.
  * step: column i = the file's axis-i vector.  Return 0, or 1 on a read
  * error. */

@@ -405,3 +405,33 @@ def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40):
         raise ValueError(f"oracle_yt: status {rc}")
     k = na.value
     return idg, k, xattr[:3 * k].reshape(-1, 3), vol[:k], sums[:len(fl) * k].reshape(len(fl), k)
+
+
+def rrlm(v, lmax):
+    """genrlm_real at the Cartesian vector v through tosphere (the
+    reference's route through spherical angles)."""
+    lib = load()
+    lib.oracle_tosphere.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_genrlm_real.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    r = np.zeros(1)
+    tp = np.zeros(2)
+    lib.oracle_tosphere(v.ctypes.data, r.ctypes.data, tp.ctypes.data)
+    out = np.zeros((lmax + 1) ** 2)
+    lib.oracle_genrlm_real(lmax, float(r[0]), tp.ctypes.data, out.ctypes.data)
+    return out
+
+
+def multipoles(idg, f, n, x2c, xattr, lmax=5):
+    """Basin multipoles (oracle_multipoles): (nattr, (lmax+1)^2), unscaled."""
+    lib = load()
+    lib.oracle_multipoles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                      C.c_void_p]
+    idg = np.ascontiguousarray(idg, dtype=np.int32).reshape(-1)
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    nn = np.asarray(n, dtype=np.int32)
+    xattr = np.ascontiguousarray(xattr, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(xattr.shape[0] * (lmax + 1) ** 2)
+    lib.oracle_multipoles(idg.ctypes.data, f.ctypes.data, nn.ctypes.data, _matf(x2c).ctypes.data, xattr.shape[0],
+                          xattr.ctypes.data, lmax, out.ctypes.data)
+    return out.reshape(xattr.shape[0], -1)
