@@ -376,6 +376,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--n", type=int, default=512, help="grid edge (default: the 512^3 workload)")
+    ap.add_argument("--config", type=int, default=3,
+                    help="workload: 3 = the 512^3 cubic cell (default), 5 = the 768^3 triclinic cell with 500 "
+                         "atoms (Z up to 26)")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 512^3 step, time the batched path on this many dimer cells "
                          "(512 = the batching workload: edges from {48,64,96,128,160}, 6-40 atoms)")
@@ -409,7 +412,7 @@ def main():
         print(json.dumps({"copy_probe": "k_sqrt", "points": probe.npts,
                           "bytes_read": 8 * probe.npts, "bytes_written": 8 * probe.npts}), flush=True)
         return
-    cell = config(3, n_override=(args.n,) * 3)
+    cell = config(args.config, n_override=(args.n,) * 3 if args.config == 3 else None)
     N = cell.npts
     tables, species = species_of(cell)
     rho = ctx.empty((N,), np.float64)
@@ -472,7 +475,8 @@ def main():
     value = N / dt / 1e9
     top = sorted(ktimes.items(), key=lambda kv: -kv[1][0])[:8]
     out = {
-        "metric": "Gpoints/s Bader+NCI @512^3, mixed vs FP64 GPU (labels exact); %HBM peak",
+        "metric": "Gpoints/s Bader+NCI @512^3, mixed vs FP64 GPU (labels exact); %HBM peak" if args.config == 3
+                  else f"Gpoints/s Bader+NCI @{cell.n[0]}^3 triclinic, mixed (labels exact); %HBM peak",
         "value": round(value, 6),
         "unit": "Gpoints/s",
         "n_gpus": 1,
@@ -484,9 +488,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "cubic cell a=30 A, 216 seeded atoms (Z in 1,6,7,8), Slater-sum FP64 grid "
-                               f"{cell.n[0]}x{cell.n[1]}x{cell.n[2]}; Bader labels + Volume/Pop + FP32 "
-                               "promolecular NCI",
+        "config": {"workload": (("cubic cell a=30 A, 216 seeded atoms (Z in 1,6,7,8)" if args.config == 3 else
+                                  "triclinic cell a=b=c=40 A, 500 seeded atoms (Z in 1,6,7,8,14,26)") +
+                                 f", Slater-sum FP64 grid {cell.n[0]}x{cell.n[1]}x{cell.n[2]}; Bader labels + "
+                                 "Volume/Pop + FP32 promolecular NCI"),
                    "grid": list(cell.n), "atoms": int(len(cell.zat)), "parallelism": "single GPU"},
         "hbm_frac_compulsory": round(20.0 * N / dt / 1e9 / HBM_PEAK_GBS, 6),
         "roofline": roof,

@@ -33,7 +33,7 @@ struct RefineResult {
 RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin, int32_t *&V, int32_t *&V2,
                         int8_t *&known, int8_t *&known2, uint32_t *scratch, int nth, int maxlen);
 
-static constexpr int MAX_PATH = 1024;  // positions are stamped in 10 bits
+static constexpr int MAX_PATH = 2048;  // positions are stamped in 11 bits
 
 struct Never {
   __device__ bool operator()(uint32_t) const { return false; }
@@ -221,12 +221,14 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     if (ntraj <= cap) break;
     cap = ntraj;  // exact size now known: walk again into a pool that fits
   }
-  if (read_scalar(ctx, d_err) & 1) fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: ascent path longer than 1024 points");
+  if (read_scalar(ctx, d_err) & 1) fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: ascent path longer than 2048 points");
   exclusive_scan(ctx, "len", d_len, d_toff, (int64_t)N + 1);
   uint32_t *d_tp = buf<uint32_t>(ctx, "bader_tp", ntraj);
   C2_LAUNCH(ctx, k_traj_reorder, dim3(gN), dim3(B), 0, N, d_pos, d_toff,
             (const uint32_t *)ctx->bufs["bader_pool"].ptr, d_tp);
   S.traj_points = (int64_t)ntraj;
+  // a large walk pool is not kept for the next call: the sweep's lists need the room
+  if (ctx->bufs["bader_pool"].bytes > (16ull << 30)) release(ctx, "bader_pool");
 
   // endpoints (grid maxima and no-move points) -> attractors, on the host
   uint8_t *d_isend = buf<uint8_t>(ctx, "bader_isend", N);
@@ -305,6 +307,10 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     h_lab.swap(lab);
     upload_labels();
     SR = sweep_run(ctx, P, N, d_toff, d_tp, d_maxlab, V, known);
+  }
+  {  // a large event pool is not kept either: refinement needs the room
+    auto it = ctx->bufs.find("sweep_pool");
+    if (it != ctx->bufs.end() && it->second.bytes > (16ull << 30)) release(ctx, "sweep_pool");
   }
   S.sweep_iters = SR.iters;
   S.events_total = SR.events_total;

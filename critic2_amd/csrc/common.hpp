@@ -86,6 +86,35 @@ inline T *buf(c2gpu_ctx *ctx, const std::string &name, size_t count) {
   return static_cast<T *>(buffer(ctx, name, count * sizeof(T)));
 }
 
+// Frees a named buffer (after the stream's work), so the next call of
+// buffer() for it allocates afresh.
+inline void release(c2gpu_ctx *ctx, const std::string &name) {
+  auto it = ctx->bufs.find(name);
+  if (it == ctx->bufs.end()) return;
+  if (it->second.ptr) {
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    C2_HIP(hipFree(it->second.ptr));
+  }
+  ctx->bufs.erase(it);
+}
+
+// A buffer of at least min_bytes and at most want_bytes: as much of
+// want_bytes as the device's free memory allows, keeping `spare` bytes free
+// for later allocations.  The old contents are not kept.
+inline void *buffer_within(c2gpu_ctx *ctx, const std::string &name, size_t min_bytes, size_t want_bytes,
+                           size_t spare) {
+  auto &b = ctx->bufs[name];
+  if (b.ptr && b.bytes >= want_bytes) return b.ptr;
+  release(ctx, name);
+  size_t fr = 0, tot = 0;
+  C2_HIP(hipMemGetInfo(&fr, &tot));
+  size_t bytes = want_bytes;
+  if (fr < spare + bytes) bytes = fr > spare + min_bytes ? fr - spare : min_bytes;
+  bytes &= ~(size_t)255;
+  if (bytes < min_bytes) bytes = min_bytes;
+  return buffer(ctx, name, bytes);
+}
+
 inline hipEvent_t event_get(c2gpu_ctx *ctx) {
   if (!ctx->evpool.empty()) {
     hipEvent_t e = ctx->evpool.back();

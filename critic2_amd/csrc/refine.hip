@@ -8,6 +8,8 @@
 // per-point rules (k_refn_mark).
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "bader_dev.hpp"
 #include "common.hpp"
 #include "devutil.hpp"
@@ -231,7 +233,8 @@ RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin
     std::swap(V, V2);
     const unsigned long long nre = read_scalar(ctx, d_cnt);
     const int e = read_scalar(ctx, d_err);
-    if (e & 1) fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: refinement path longer than 1024 points");
+    if (e & 1)
+      fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: refinement path longer than " + std::to_string(maxlen) + " points");
     if (e & 4) fail(C2GPU_ERR_NEWMAX, "refine_edge: should be no new maxima in edge refinement");
     if (e & 8) fail(C2GPU_ERR_DEGENERATE, "c2gpu_bader: refinement path ends on a point being reassigned");
     if (nre == 0) break;

@@ -12,7 +12,7 @@
 //
 // Here that history is a "story": for every point s, ell[s] (0 if s is not a
 // start, else the length of its committed path) and lam[s] (its label).
-// Every commit is an event stamped time = (sweep index << 10 | position on
+// Every commit is an event stamped time = (sweep index << 11 | position on
 // the path); per point, the events form a time-sorted list from which the
 // state of the serial sweep at any moment is read back: volnum(y) at time t is
 // the label of y's last event before t, and the time y enters known == 2 is
@@ -47,7 +47,7 @@
 namespace c2 {
 
 static constexpr uint64_t KT_INF = ~0ull;
-static constexpr int POS_BITS = 10;     // position on a path: < 1024
+static constexpr int POS_BITS = 11;     // position on a path: < 2048
 static constexpr int LAB_BITS = 20;     // label: < 2^20
 static constexpr int SORT_INLINE = 48;  // longer event lists are sorted by workgroups
 static constexpr int SORT_LDS = 4096;   // longer still: segmented radix sort
@@ -779,6 +779,8 @@ __global__ void k_final_state(uint32_t N, Lists L, const uint64_t *__restrict__ 
 
 // ---------------------------------------------------------------- host driver
 
+constexpr uint64_t HUGE_BATCH = 1ull << 27;
+
 // Sorts the lists of the points in `list` (all points when null).
 static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const Lists &L, uint8_t *uni,
                        uint32_t *longlist, uint32_t *hugelist, unsigned long long *work) {
@@ -793,22 +795,41 @@ static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const L
   C2_HIP(hipStreamSynchronize(st));
   if (nlh[0]) C2_LAUNCH(ctx, k_ev_sort_long, dim3(nlh[0]), dim3(256), 0, longlist, L, uni);
   if (nlh[1]) {
+    // very long lists: gathered in batches of about HUGE_BATCH events (a
+    // single longer list is a batch of its own), each batch sorted by one
+    // segmented radix sort in a compact buffer and copied back
     const uint32_t nh = nlh[1];
     uint32_t *hlen = buf<uint32_t>(ctx, "sweep_hugelen", (size_t)nh + 1);
     uint64_t *hoff = buf<uint64_t>(ctx, "sweep_hugeoff", (size_t)nh + 1);
     C2_LAUNCH(ctx, k_huge_len, dim3(grid_for((int64_t)nh + 1, 64)), dim3(64), 0, nh, hugelist, L, hlen);
     exclusive_scan(ctx, "huge", hlen, hoff, (int64_t)nh + 1);
-    const uint64_t tot = read_scalar(ctx, hoff + nh);
-    if (tot >= (1ull << 31)) fail(C2GPU_ERR_ARG, "c2gpu_bader: sweep event lists too long to sort");
-    uint64_t *keys = buf<uint64_t>(ctx, "sweep_hugekeys", tot), *alt = buf<uint64_t>(ctx, "sweep_hugealt", tot);
-    C2_LAUNCH(ctx, k_huge_gather, dim3(nh), dim3(256), 0, hugelist, L, hoff, keys);
-    size_t tmp = 0;
-    C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, keys, alt, (int)tot, (int)nh, hoff, hoff + 1, 0,
-                                                      64, st));
-    void *t = buffer(ctx, "sweep_segsort_tmp", tmp);
-    C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(t, tmp, keys, alt, (int)tot, (int)nh, hoff, hoff + 1, 0, 64,
-                                                      st));
-    C2_LAUNCH(ctx, k_huge_copyback, dim3(nh), dim3(256), 0, hugelist, L, hoff, alt, uni);
+    std::vector<uint64_t> ho((size_t)nh + 1);
+    C2_HIP(hipMemcpyAsync(ho.data(), hoff, 8 * ((size_t)nh + 1), hipMemcpyDeviceToHost, st));
+    C2_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> bo;  // batch-relative segment offsets
+    for (uint32_t q0 = 0; q0 < nh;) {
+      uint32_t q1 = q0 + 1;
+      while (q1 < nh && ho[q1 + 1] - ho[q0] <= HUGE_BATCH) q1++;
+      const uint64_t base = ho[q0], tot = ho[q1] - base;
+      if (tot >= (1ull << 31)) fail(C2GPU_ERR_ARG, "c2gpu_bader: sweep event list too long to sort");
+      const uint32_t nb = q1 - q0;
+      bo.resize((size_t)nb + 1);
+      for (uint32_t q = 0; q <= nb; q++) bo[q] = ho[q0 + q] - base;
+      uint64_t *boff = buf<uint64_t>(ctx, "sweep_hugeboff", (size_t)nb + 1);
+      C2_HIP(hipMemcpyAsync(boff, bo.data(), 8 * ((size_t)nb + 1), hipMemcpyHostToDevice, st));
+      uint64_t *keys = buf<uint64_t>(ctx, "sweep_hugekeys", tot), *alt = buf<uint64_t>(ctx, "sweep_hugealt", tot);
+      C2_LAUNCH(ctx, k_huge_gather, dim3(nb), dim3(256), 0, hugelist + q0, L, boff, keys);
+      size_t tmp = 0;
+      C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, keys, alt, (int)tot, (int)nb, boff, boff + 1,
+                                                        0, 64, st));
+      void *t = buffer(ctx, "sweep_segsort_tmp", tmp);
+      C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(t, tmp, keys, alt, (int)tot, (int)nb, boff, boff + 1, 0, 64,
+                                                        st));
+      C2_LAUNCH(ctx, k_huge_copyback, dim3(nb), dim3(256), 0, hugelist + q0, L, boff, alt, uni);
+      // the host offsets are rewritten by the next batch: wait for this one's copy
+      C2_HIP(hipStreamSynchronize(st));
+      q0 = q1;
+    }
   }
 }
 
@@ -893,9 +914,12 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     C2_LAUNCH(ctx, k_reserve, dim3(grid_for((int64_t)N + 1, B)), dim3(B), 0, N, cntN, capN, capv);
     exclusive_scan(ctx, "ev", capv, offN, (int64_t)N + 1);
     const uint64_t reserved = read_scalar(ctx, offN + N);
-    const uint64_t want = reserved + (reserved >> 2) + (uint64_t)N;  // room for moved lists
+    // room for moved lists: a quarter more, or what the device has left
+    // (a fuller pool only rebuilds more often)
+    const uint64_t want = reserved + (reserved >> 2) + (uint64_t)N;
     if (want > pool_cap) {
-      L.ev = buf<uint64_t>(ctx, "sweep_pool", want);
+      const uint64_t least = reserved + (reserved >> 5) + (uint64_t)N / 8;
+      L.ev = static_cast<uint64_t *>(buffer_within(ctx, "sweep_pool", 8 * least, 8 * want, 6ull << 30));
       pool_cap = ctx->bufs["sweep_pool"].bytes / 8;
     }
     pool_top = reserved;
