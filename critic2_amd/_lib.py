@@ -35,7 +35,7 @@ EXPORTS = (
     "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
     "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff", "c2gpu_ws_facets",
-    "c2gpu_yt", "c2gpu_multipoles",
+    "c2gpu_yt", "c2gpu_multipoles", "c2gpu_nci_promol_nodes", "c2gpu_nci_fragments", "c2gpu_nci_cutoff_frag",
 )
 
 
@@ -77,6 +77,10 @@ class YtStats(C.Structure):
 class NciCut(C.Structure):
     _fields_ = [("rhocut", C.c_double), ("dimcut", C.c_double), ("rhoplot", C.c_double),
                 ("srhorange", C.c_double * 2), ("onlyneg", C.c_int), ("rho_void", C.c_double)]
+
+
+class NciNodes(C.Structure):
+    _fields_ = [("nstep", C.c_int * 3), ("x0", C.c_double * 3), ("xmat", C.c_double * 9)]
 
 
 class RadialTable(C.Structure):
@@ -132,6 +136,10 @@ def load():
         "c2gpu_cube_write": [vp, C.c_char_p, C.POINTER(CubeHeader), vp, vp, vp, i, i],
         "c2gpu_nci_cutoff": [vp, vp, vp, vp, vp, i, C.POINTER(NciCut), i, vp, C.c_uint64, C.POINTER(C.c_uint64),
                              vp],
+        "c2gpu_nci_cutoff_frag": [vp, vp, vp, vp, vp, i, d, d, vp, i, C.POINTER(NciCut), i, vp, C.c_uint64,
+                                  C.POINTER(C.c_uint64), vp],
+        "c2gpu_nci_promol_nodes": [vp, C.POINTER(NciNodes), vp, C.POINTER(Promol), i, i, vp, vp, vp],
+        "c2gpu_nci_fragments": [vp, C.POINTER(NciNodes), i, C.POINTER(RadialTable), i, vp, vp, vp, i, i, vp],
         "c2gpu_ws_facets": [vp, C.POINTER(i), vp, vp],
         "c2gpu_multipoles": [vp, vp, vp, vp, vp, i, vp, i, i, vp],
         "c2gpu_yt": [vp, vp, vp, vp, C.POINTER(AttractorOpts), i, vp, i, vp, C.POINTER(i), vp, i, vp, vp,

@@ -28,8 +28,12 @@ void cube_header(const char *, c2gpu_cube_header *, int);
 void cube_parse_run(c2gpu_ctx *, const char *, uint64_t, int, c2gpu_cube_header *, double *);
 void cube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
 void bincube_read_run(c2gpu_ctx *, const char *, int, c2gpu_cube_header *, double *);
-void nci_cutoff_run(c2gpu_ctx *, const void *, void *, const void *, const int *, int, const c2gpu_nci_cut *, int,
-                    char *, uint64_t, uint64_t *, double *);
+void nci_cutoff_run(c2gpu_ctx *, const void *, void *, const void *, const void *, int, double, double, const int *,
+                    int, const c2gpu_nci_cut *, int, char *, uint64_t, uint64_t *, double *);
+void nci_promol_nodes_run(c2gpu_ctx *, const c2gpu_nci_nodes *, const double *, const c2gpu_promol *, int, int,
+                          void *, void *, void *);
+void nci_fragments_run(c2gpu_ctx *, const c2gpu_nci_nodes *, int, const c2gpu_radial_table *, int, const int *,
+                       const double *, const int *, int, int, void *);
 int ws_facets(const double m[9], int vec[3 * 14], double area[14]);
 void yt_run(c2gpu_ctx *, const double *, const int *, const double *, const c2gpu_attractor_opts *, int,
             const double *const *, int, int32_t *, int *, double *, int, double *, double *, c2gpu_yt_stats *);
@@ -495,7 +499,35 @@ int c2gpu_cube_write(c2gpu_ctx *ctx, const char *path, const c2gpu_cube_header *
 int c2gpu_nci_cutoff(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *rhoat, const int n[3],
                      int precision, const c2gpu_nci_cut *opts, int flags, char *dat, uint64_t cap,
                      uint64_t *len, double sums[3]) {
-  return guarded(ctx, [&] { nci_cutoff_run(ctx, crho, cgrad, rhoat, n, precision, opts, flags, dat, cap, len, sums); });
+  return guarded(ctx, [&] {
+    nci_cutoff_run(ctx, crho, cgrad, rhoat, nullptr, 0, 0.0, 0.0, n, precision, opts, flags, dat, cap, len, sums);
+  });
+}
+
+int c2gpu_nci_cutoff_frag(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *rhoat, const void *rhofrag,
+                          int nfrag, double rhoparam, double rhoparam2, const int n[3], int precision,
+                          const c2gpu_nci_cut *opts, int flags, char *dat, uint64_t cap, uint64_t *len,
+                          double sums[3]) {
+  return guarded(ctx, [&] {
+    nci_cutoff_run(ctx, crho, cgrad, rhoat, rhofrag, nfrag, rhoparam, rhoparam2, n, precision, opts, flags, dat,
+                   cap, len, sums);
+  });
+}
+
+int c2gpu_nci_promol_nodes(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nodes, const double x2c[9],
+                           const c2gpu_promol *promol, int precision, int flags, void *crho_out, void *cgrad_out,
+                           void *rhoat_out) {
+  return guarded(ctx, [&] {
+    nci_promol_nodes_run(ctx, nodes, x2c, promol, precision, flags, crho_out, cgrad_out, rhoat_out);
+  });
+}
+
+int c2gpu_nci_fragments(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nodes, int nspecies, const c2gpu_radial_table *tables,
+                        int nfrag, const int *frag_nat, const double *xyz, const int *species, int precision,
+                        int flags, void *rhofrag_out) {
+  return guarded(ctx, [&] {
+    nci_fragments_run(ctx, nodes, nspecies, tables, nfrag, frag_nat, xyz, species, precision, flags, rhofrag_out);
+  });
 }
 
 int c2gpu_ws_facets(const double m[9], int *nvec, int vec[42], double area[14]) {

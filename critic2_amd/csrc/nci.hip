@@ -34,9 +34,7 @@ struct Images {
 __device__ __forceinline__ bool image_hits_brick(const Geo &G, const Images &I, int b, int m) {
   const int b1 = b % G.nb1, b2 = (b / G.nb1) % G.nb2, b3 = b / (G.nb1 * G.nb2);
   double c[3];
-  const double f0 = (b1 * BX + 0.5 * (BX - 1)) / G.n1, f1 = (b2 * BY + 0.5 * (BY - 1)) / G.n2,
-               f2 = (b3 * BZ + 0.5 * (BZ - 1)) / G.n3;
-  for (int r = 0; r < 3; r++) c[r] = G.x2c[r] * f0 + G.x2c[r + 3] * f1 + G.x2c[r + 6] * f2;
+  geo_cart(G, b1 * BX + 0.5 * (BX - 1), b2 * BY + 0.5 * (BY - 1), b3 * BZ + 0.5 * (BZ - 1), c);
   const double4 p = I.pos[m];
   const double dx = p.x - c[0], dy = p.y - c[1], dz = p.z - c[2];
   const double rr = I.rcut[(int)p.w] + I.brad;
@@ -216,7 +214,9 @@ __global__ void __launch_bounds__(256) k_promol_d(Geo G, PromolD T, const double
   const double l2 = eig_mid_t<double>(h00, h11, h22, h01, h02, h12);
   const double a = fabs(f) * 100.0;
   const double crho = (l2 < 0.0 || (l2 == 0.0 && signbit(l2))) ? -a : a;
-  store_out<double>(O, out_index(G, O.zfast, i, j, k), crho, s);
+  const size_t o = out_index(G, O.zfast, i, j, k);
+  store_out<double>(O, o, crho, s);
+  if (O.rhoat) ((double *)O.rhoat)[o] = f;
 }
 
 // FP32 promolecular NCI: same sums with per-interval cubic tables.
@@ -283,7 +283,9 @@ __device__ __forceinline__ void promol_f_node(const Geo &G, const PromolF &T, co
   const float l2 = eig_mid_t<float>(h00, h11, h22, h01, h02, h12);
   const float a = fabsf(f) * 100.f;
   const float crho = (l2 < 0.f || (l2 == 0.f && signbit(l2))) ? -a : a;
-  store_out<float>(O, out_index(G, O.zfast, i, j, k), crho, s);
+  const size_t o = out_index(G, O.zfast, i, j, k);
+  store_out<float>(O, o, crho, s);
+  if (O.rhoat) ((float *)O.rhoat)[o] = f;
 }
 
 __global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const float4 *__restrict__ imgf,
@@ -395,7 +397,7 @@ void nci_grid_launch(c2gpu_ctx *ctx, const Geo &G, const double x2c[9], const do
                      const double *d_rho, const NciOut &O);
 
 static Geo make_geo(const int n[3], const double x2c[9]) {
-  Geo G;
+  Geo G{};
   G.n1 = n[0];
   G.n2 = n[1];
   G.n3 = n[2];
@@ -406,11 +408,21 @@ static Geo make_geo(const int n[3], const double x2c[9]) {
   return G;
 }
 
+// The nodes x0 + i xmat(:,1) + j xmat(:,2) + k xmat(:,3) of an nciplot cube.
+static Geo make_geo_box(const c2gpu_nci_nodes &nd, const double x2c[9]) {
+  Geo G = make_geo(nd.nstep, x2c);
+  G.box = 1;
+  for (int r = 0; r < 3; r++) G.x0[r] = nd.x0[r];
+  for (int q = 0; q < 9; q++) G.xm[q] = nd.xmat[q];
+  return G;
+}
+
 static double brick_radius(const Geo &G) {
   double h[3][3];
   const double e[3] = {0.5 * (BX - 1) / G.n1, 0.5 * (BY - 1) / G.n2, 0.5 * (BZ - 1) / G.n3};
   for (int a = 0; a < 3; a++)
-    for (int r = 0; r < 3; r++) h[a][r] = G.x2c[r + 3 * a] * e[a];
+    for (int r = 0; r < 3; r++) h[a][r] = G.box ? G.xm[r + 3 * a] * (0.5 * ((a == 0 ? BX : a == 1 ? BY : BZ) - 1))
+                                                : G.x2c[r + 3 * a] * e[a];
   double best = 0;
   for (int s = 0; s < 8; s++) {
     double v[3];
@@ -424,9 +436,11 @@ static double brick_radius(const Geo &G) {
 
 // All periodic images of the atoms that can lie within rc_max of a point of
 // the cell: fractional coordinates within rc_max*|b_c| of [0,1] per axis.
+// With lo/hi: the images that can reach a point whose fractional
+// coordinates lie in [lo, hi] instead of the cell.
 static void expand_images(const double x2c[9], int nat, const double *xat, const int *spc,
                           double rc_max, std::vector<double> &pos, std::vector<int> &isp,
-                          std::vector<int> &iat) {
+                          std::vector<int> &iat, const double *lo = nullptr, const double *hi = nullptr) {
   // reciprocal vectors (rows of inverse of x2c)
   const double *a = x2c;
   double inv[9];
@@ -441,23 +455,26 @@ static void expand_images(const double x2c[9], int nat, const double *xat, const
   inv[2] = (a[1] * a[5] - a[4] * a[2]) / det;
   inv[5] = -(a[0] * a[5] - a[3] * a[2]) / det;
   inv[8] = (a[0] * a[4] - a[3] * a[1]) / det;
-  double delta[3];
-  int lim[3];
+  double delta[3], flo[3], fhi[3];
+  int tlo[3], thi[3];
   for (int c = 0; c < 3; c++) {  // row c of inv = reciprocal vector b_c
     const double bl = std::sqrt(inv[c] * inv[c] + inv[c + 3] * inv[c + 3] + inv[c + 6] * inv[c + 6]);
     delta[c] = rc_max * bl;
-    lim[c] = (int)std::ceil(delta[c]) + 1;
+    flo[c] = lo ? lo[c] : 0.0;
+    fhi[c] = hi ? hi[c] : 1.0;
+    tlo[c] = (int)std::floor(flo[c] - delta[c]) - 1;
+    thi[c] = (int)std::ceil(fhi[c] + delta[c]) + 1;
   }
   for (int at = 0; at < nat; at++)
-    for (int t0 = -lim[0]; t0 <= lim[0]; t0++)
-      for (int t1 = -lim[1]; t1 <= lim[1]; t1++)
-        for (int t2 = -lim[2]; t2 <= lim[2]; t2++) {
+    for (int t0 = tlo[0]; t0 <= thi[0]; t0++)
+      for (int t1 = tlo[1]; t1 <= thi[1]; t1++)
+        for (int t2 = tlo[2]; t2 <= thi[2]; t2++) {
           double xf[3] = {xat[3 * at] - std::floor(xat[3 * at]) + t0,
                           xat[3 * at + 1] - std::floor(xat[3 * at + 1]) + t1,
                           xat[3 * at + 2] - std::floor(xat[3 * at + 2]) + t2};
           bool ok = true;
           for (int c = 0; c < 3; c++)
-            if (xf[c] < -delta[c] || xf[c] > 1.0 + delta[c]) ok = false;
+            if (xf[c] < flo[c] - delta[c] || xf[c] > fhi[c] + delta[c]) ok = false;
           if (!ok) continue;
           for (int r = 0; r < 3; r++)
             pos.push_back(x2c[r] * xf[0] + x2c[r + 3] * xf[1] + x2c[r + 6] * xf[2]);
@@ -540,16 +557,16 @@ struct PromolDev {
   int nimg;
 };
 
-static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const double x2c[9]) {
-  if (!pm || pm->nspecies < 1 || !pm->tables || pm->nat < 1 || !pm->xat || !pm->species)
-    fail(C2GPU_ERR_ARG, "c2gpu_nci: promolecular tables required");
-  const int ns = pm->nspecies;
+// The radial tables on the device (FP64 nodes and values, FP32 cubic
+// coefficients); returns the largest cutoff min(cutrad, rmax).
+static double upload_tables(c2gpu_ctx *ctx, int ns, const c2gpu_radial_table *tables, PromolDev &P) {
+  if (ns < 1 || !tables) fail(C2GPU_ERR_ARG, "c2gpu_nci: radial tables required");
   std::vector<int> toff(ns), tng(ns);
   int tot = 0;
   double rcmax = 0;
   std::vector<double> ta(ns), tb(ns), trmax(ns), tcut(ns);
   for (int s = 0; s < ns; s++) {
-    const c2gpu_radial_table &t = pm->tables[s];
+    const c2gpu_radial_table &t = tables[s];
     if (t.ngrid < 6 || !t.r || !t.f || !t.fp || !t.fpp) fail(C2GPU_ERR_ARG, "c2gpu_nci: bad radial table");
     toff[s] = tot;
     tng[s] = t.ngrid;
@@ -561,11 +578,9 @@ static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const dou
     tcut[s] = t.cutrad < t.rmax ? t.cutrad : t.rmax;
     rcmax = tcut[s] > rcmax ? tcut[s] : rcmax;
   }
-  for (int a = 0; a < pm->nat; a++)
-    if (pm->species[a] < 0 || pm->species[a] >= ns) fail(C2GPU_ERR_ARG, "c2gpu_nci: bad species index");
   std::vector<double> r(tot), f(tot), fp(tot), fpp(tot);
   for (int s = 0; s < ns; s++) {
-    const c2gpu_radial_table &t = pm->tables[s];
+    const c2gpu_radial_table &t = tables[s];
     std::memcpy(&r[toff[s]], t.r, 8 * t.ngrid);
     std::memcpy(&f[toff[s]], t.f, 8 * t.ngrid);
     std::memcpy(&fp[toff[s]], t.fp, 8 * t.ngrid);
@@ -609,16 +624,6 @@ static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const dou
       node[o + base] = (float)u0;
     }
   }
-  std::vector<double> pos;
-  std::vector<int> isp, iat;
-  expand_images(x2c, pm->nat, pm->xat, pm->species, rcmax, pos, isp, iat);
-  const int nimg = (int)isp.size();
-  std::vector<double4> h_img(nimg);
-  std::vector<float4> h_imgf(nimg);
-  for (int m = 0; m < nimg; m++) {
-    h_img[m] = make_double4(pos[3 * m], pos[3 * m + 1], pos[3 * m + 2], (double)isp[m]);
-    h_imgf[m] = make_float4((float)pos[3 * m], (float)pos[3 * m + 1], (float)pos[3 * m + 2], (float)isp[m]);
-  }
   std::vector<float> cut2(ns), tainvb(ns), tloga(ns), tr0(ns), trmaxf(ns);
   for (int s = 0; s < ns; s++) {
     cut2[s] = (float)(tcut[s] * tcut[s]);
@@ -632,7 +637,6 @@ static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const dou
     C2_HIP(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     return d;
   };
-  PromolDev P;
   P.D.r = (const double *)up("pm_r", r.data(), 8 * tot);
   P.D.f = (const double *)up("pm_f", f.data(), 8 * tot);
   P.D.fp = (const double *)up("pm_fp", fp.data(), 8 * tot);
@@ -651,10 +655,38 @@ static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const dou
   P.F.tr0 = (const float *)up("pm_tr0", tr0.data(), 4 * ns);
   P.F.trmax = (const float *)up("pm_trmaxf", trmaxf.data(), 4 * ns);
   P.F.tnode = (const float *)up("pm_node", node.data(), 4 * node.size());
-  P.img = (double4 *)up("pm_img", h_img.data(), sizeof(double4) * nimg);
-  P.imgf = (float4 *)up("pm_imgf", h_imgf.data(), sizeof(float4) * nimg);
   P.rcut = (double *)up("pm_rcut", tcut.data(), 8 * ns);
   P.cut2f = (float *)up("pm_cut2", cut2.data(), 4 * ns);
+  C2_HIP(hipStreamSynchronize(ctx->stream));  // the host vectors end here
+  return rcmax;
+}
+
+// Tables and the atom images of the cell that can reach the nodes (the cell,
+// or the fractional range [lo, hi] of an nciplot cube).
+static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const double x2c[9],
+                               const double *lo = nullptr, const double *hi = nullptr) {
+  if (!pm || pm->nspecies < 1 || !pm->tables || pm->nat < 1 || !pm->xat || !pm->species)
+    fail(C2GPU_ERR_ARG, "c2gpu_nci: promolecular tables required");
+  for (int a = 0; a < pm->nat; a++)
+    if (pm->species[a] < 0 || pm->species[a] >= pm->nspecies) fail(C2GPU_ERR_ARG, "c2gpu_nci: bad species index");
+  PromolDev P;
+  const double rcmax = upload_tables(ctx, pm->nspecies, pm->tables, P);
+  std::vector<double> pos;
+  std::vector<int> isp, iat;
+  expand_images(x2c, pm->nat, pm->xat, pm->species, rcmax, pos, isp, iat, lo, hi);
+  const int nimg = (int)isp.size();
+  std::vector<double4> h_img(nimg > 0 ? nimg : 1);
+  std::vector<float4> h_imgf(nimg > 0 ? nimg : 1);
+  for (int m = 0; m < nimg; m++) {
+    h_img[m] = make_double4(pos[3 * m], pos[3 * m + 1], pos[3 * m + 2], (double)isp[m]);
+    h_imgf[m] = make_float4((float)pos[3 * m], (float)pos[3 * m + 1], (float)pos[3 * m + 2], (float)isp[m]);
+  }
+  P.img = buf<double4>(ctx, "pm_img", h_img.size());
+  P.imgf = buf<float4>(ctx, "pm_imgf", h_imgf.size());
+  C2_HIP(hipMemcpyAsync(P.img, h_img.data(), sizeof(double4) * h_img.size(), hipMemcpyHostToDevice, ctx->stream));
+  C2_HIP(hipMemcpyAsync(P.imgf, h_imgf.data(), sizeof(float4) * h_imgf.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  C2_HIP(hipStreamSynchronize(ctx->stream));
   P.nimg = nimg;
   return P;
 }
@@ -1029,6 +1061,191 @@ void nci_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
     C2_HIP(hipMemcpyAsync(crho_out, O.crho, osz * N, hipMemcpyDeviceToHost, st));
     C2_HIP(hipMemcpyAsync(cgrad_out, O.cgrad, osz * N, hipMemcpyDeviceToHost, st));
   }
+  C2_HIP(hipStreamSynchronize(st));
+}
+
+
+// ---------------------------------------------------------------- nciplot cubes and fragments
+
+static void inv3(const double a[9], double inv[9]) {
+  const double det = a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
+                     a[6] * (a[1] * a[5] - a[4] * a[2]);
+  if (!(det != 0.0)) fail(C2GPU_ERR_ARG, "c2gpu_nci: singular cell");
+  inv[0] = (a[4] * a[8] - a[7] * a[5]) / det;
+  inv[3] = -(a[3] * a[8] - a[6] * a[5]) / det;
+  inv[6] = (a[3] * a[7] - a[6] * a[4]) / det;
+  inv[1] = -(a[1] * a[8] - a[7] * a[2]) / det;
+  inv[4] = (a[0] * a[8] - a[6] * a[2]) / det;
+  inv[7] = -(a[0] * a[7] - a[6] * a[1]) / det;
+  inv[2] = (a[1] * a[5] - a[4] * a[2]) / det;
+  inv[5] = -(a[0] * a[5] - a[3] * a[2]) / det;
+  inv[8] = (a[0] * a[4] - a[3] * a[1]) / det;
+}
+
+// Fractional range [lo, hi] of the nodes of an nciplot cube (its corners).
+static void box_frac_range(const Geo &G, const double x2c[9], double lo[3], double hi[3]) {
+  double c2x[9];
+  inv3(x2c, c2x);
+  for (int c = 0; c < 3; c++) {
+    lo[c] = 1e300;
+    hi[c] = -1e300;
+  }
+  for (int q = 0; q < 8; q++) {
+    const double u = (q & 1) ? G.n1 - 1 : 0, v = (q & 2) ? G.n2 - 1 : 0, w = (q & 4) ? G.n3 - 1 : 0;
+    double x[3], f[3];
+    for (int r = 0; r < 3; r++) x[r] = G.x0[r] + u * G.xm[r] + v * G.xm[r + 3] + w * G.xm[r + 6];
+    for (int r = 0; r < 3; r++) f[r] = c2x[r] * x[0] + c2x[r + 3] * x[1] + c2x[r + 6] * x[2];
+    for (int c = 0; c < 3; c++) {
+      lo[c] = f[c] < lo[c] ? f[c] : lo[c];
+      hi[c] = f[c] > hi[c] ? f[c] : hi[c];
+    }
+  }
+}
+
+static Geo checked_box(const c2gpu_nci_nodes *nd, const double x2c[9], const char *who) {
+  if (!nd || !x2c) fail(C2GPU_ERR_ARG, std::string(who) + ": NULL argument");
+  for (int c = 0; c < 3; c++)
+    if (nd->nstep[c] < 1) fail(C2GPU_ERR_ARG, std::string(who) + ": nstep must be positive");
+  if ((uint64_t)nd->nstep[0] * nd->nstep[1] * nd->nstep[2] >= (1ull << 32))
+    fail(C2GPU_ERR_ARG, std::string(who) + ": fewer than 2^32 nodes");
+  return make_geo_box(*nd, x2c);
+}
+
+void nci_promol_nodes_run(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nd, const double x2c[9], const c2gpu_promol *pm,
+                          int precision, int flags, void *crho_out, void *cgrad_out, void *rhoat_out) {
+  const Geo G = checked_box(nd, x2c, "c2gpu_nci_promol_nodes");
+  if (!crho_out || !cgrad_out) fail(C2GPU_ERR_ARG, "c2gpu_nci_promol_nodes: NULL output");
+  if (precision != C2GPU_FP64 && precision != C2GPU_FP32)
+    fail(C2GPU_ERR_ARG, "c2gpu_nci_promol_nodes: precision must be C2GPU_FP64 or C2GPU_FP32");
+  const bool f32 = precision == C2GPU_FP32;
+  const size_t N = (size_t)G.n1 * G.n2 * G.n3, osz = f32 ? 4 : 8;
+  const bool dev = flags & C2GPU_PTR_DEVICE;
+  hipStream_t st = ctx->stream;
+  double lo[3], hi[3];
+  box_frac_range(G, x2c, lo, hi);
+  PromolDev P = upload_promol(ctx, pm, x2c, lo, hi);
+  BrickLists BL = build_bricks(ctx, G, P.img, P.rcut, P.nimg, "pm");
+  NciOut O;
+  O.is_f32 = f32 ? 1 : 0;
+  O.zfast = 1;
+  O.crho = dev ? crho_out : buffer(ctx, "nci_crho", osz * N);
+  O.cgrad = dev ? cgrad_out : buffer(ctx, "nci_cgrad", osz * N);
+  O.rhoat = rhoat_out ? (dev ? rhoat_out : buffer(ctx, "nci_rhoat", osz * N)) : nullptr;
+  const int nbk = G.nb1 * G.nb2 * G.nb3;
+  if (f32)
+    C2_LAUNCH(ctx, k_promol_f, dim3(nbk), dim3(256), 0, G, P.F, P.imgf, P.cut2f, BL.off, BL.lst, O);
+  else
+    C2_LAUNCH(ctx, k_promol_d, dim3(nbk), dim3(256), 0, G, P.D, P.img, BL.off, BL.lst, O);
+  if (!dev) {
+    C2_HIP(hipMemcpyAsync(crho_out, O.crho, osz * N, hipMemcpyDeviceToHost, st));
+    C2_HIP(hipMemcpyAsync(cgrad_out, O.cgrad, osz * N, hipMemcpyDeviceToHost, st));
+    if (rhoat_out) C2_HIP(hipMemcpyAsync(rhoat_out, O.rhoat, osz * N, hipMemcpyDeviceToHost, st));
+  }
+  C2_HIP(hipStreamSynchronize(st));
+}
+
+// FP32 density of one table at r (the lookup of promol_f_node); 0 at and
+// beyond rmax.
+__device__ __forceinline__ float table_rho_f(const PromolF &T, int s, float r) {
+  if (r >= T.trmax[s]) return 0.f;
+  const float r1 = T.tr0[s];
+  int ir;
+  if (r <= r1) {
+    ir = 1;
+    r = r1;
+  } else {
+    ir = 1 + (int)floorf((__logf(r) - T.tlog_a[s]) * T.ta_inv_b[s]);
+  }
+  const int ng = T.tng[s];
+  int base = ir < 2 ? 2 : ir;
+  base = base > ng - 2 ? ng - 2 : base;
+  const int ci = T.toff[s] + base;
+  const float u = r - T.tnode[ci];
+  const float4 cf = T.coef[4 * (size_t)ci + 0];
+  const float rho = ((cf.w * u + cf.z) * u + cf.y) * u + cf.x;
+  return rho > 0.f ? rho : 0.f;
+}
+
+// Per node and fragment, the density of the fragment's atoms alone
+// (nci@proc.f90:580-596): atoms beyond rmax of their table skipped, the
+// distance raised to r(1) and 1e-14, the interpolated density clipped at 0.
+// out(p + f npts), p the z-fastest node index.
+template <class T>
+__global__ void __launch_bounds__(256) k_frag(Geo G, PromolD D, PromolF F, const double4 *__restrict__ fat,
+                                              const int *__restrict__ fstart, int nfrag, uint64_t npts,
+                                              T *__restrict__ out) {
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npts) return;
+  const int k = (int)(p % (uint64_t)G.n3);
+  const uint64_t q = p / (uint64_t)G.n3;
+  const int j = (int)(q % (uint64_t)G.n2), i = (int)(q / (uint64_t)G.n2);
+  double xc[3];
+  node_cart(G, i, j, k, xc);
+  for (int f = 0; f < nfrag; f++) {
+    T acc = T(0);
+    for (int a = fstart[f]; a < fstart[f + 1]; a++) {
+      const double4 at = fat[a];
+      const int s = (int)at.w;
+      if constexpr (sizeof(T) == 8) {
+        const double x0 = xc[0] - at.x, x1 = xc[1] - at.y, x2 = xc[2] - at.z;
+        double d = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+        if (d > D.trmax[s]) continue;
+        const double r1 = D.r[D.toff[s]];
+        d = d > r1 ? d : r1;
+        d = d > 1e-14 ? d : 1e-14;
+        double rho, rhop, rhopp;
+        interp_d(D, s, d, rho, rhop, rhopp);
+        rho = rho > 0.0 ? rho : 0.0;
+        acc = acc + rho;
+      } else {
+        const float x0 = (float)xc[0] - (float)at.x, x1 = (float)xc[1] - (float)at.y,
+                    x2 = (float)xc[2] - (float)at.z;
+        acc += table_rho_f(F, s, sqrtf(x0 * x0 + x1 * x1 + x2 * x2));
+      }
+    }
+    out[(uint64_t)f * npts + p] = acc;
+  }
+}
+
+void nci_fragments_run(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nd, int nspecies, const c2gpu_radial_table *tables,
+                       int nfrag, const int *frag_nat, const double *xyz, const int *species, int precision,
+                       int flags, void *rhofrag_out) {
+  static const double ident[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const Geo G = checked_box(nd, ident, "c2gpu_nci_fragments");
+  if (nfrag < 1 || !frag_nat || !rhofrag_out) fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: no fragments");
+  if (precision != C2GPU_FP64 && precision != C2GPU_FP32)
+    fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: precision must be C2GPU_FP64 or C2GPU_FP32");
+  std::vector<int> fstart(nfrag + 1, 0);
+  for (int f = 0; f < nfrag; f++) {
+    if (frag_nat[f] < 0) fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: negative atom count");
+    fstart[f + 1] = fstart[f] + frag_nat[f];
+  }
+  const int nat = fstart[nfrag];
+  if (nat > 0 && (!xyz || !species)) fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: NULL atoms");
+  std::vector<double4> h(nat > 0 ? nat : 1);
+  for (int a = 0; a < nat; a++) {
+    if (species[a] < 0 || species[a] >= nspecies) fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: bad species index");
+    h[a] = make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], (double)species[a]);
+  }
+  PromolDev P;
+  upload_tables(ctx, nspecies, tables, P);
+  hipStream_t st = ctx->stream;
+  double4 *d_at = buf<double4>(ctx, "frag_at", h.size());
+  int *d_fs = buf<int>(ctx, "frag_start", fstart.size());
+  C2_HIP(hipMemcpyAsync(d_at, h.data(), sizeof(double4) * h.size(), hipMemcpyHostToDevice, st));
+  C2_HIP(hipMemcpyAsync(d_fs, fstart.data(), sizeof(int) * fstart.size(), hipMemcpyHostToDevice, st));
+  const uint64_t npts = (uint64_t)G.n1 * G.n2 * G.n3;
+  const size_t osz = precision == C2GPU_FP32 ? 4 : 8;
+  const bool dev = flags & C2GPU_PTR_DEVICE;
+  void *out = dev ? rhofrag_out : buffer(ctx, "frag_out", osz * npts * nfrag);
+  const unsigned g = grid_for((int64_t)npts, 256);
+  if (precision == C2GPU_FP32)
+    C2_LAUNCH(ctx, k_frag<float>, dim3(g), dim3(256), 0, G, P.D, P.F, (const double4 *)d_at, (const int *)d_fs,
+              nfrag, npts, (float *)out);
+  else
+    C2_LAUNCH(ctx, k_frag<double>, dim3(g), dim3(256), 0, G, P.D, P.F, (const double4 *)d_at, (const int *)d_fs,
+              nfrag, npts, (double *)out);
+  if (!dev) C2_HIP(hipMemcpyAsync(rhofrag_out, out, osz * npts * nfrag, hipMemcpyDeviceToHost, st));
   C2_HIP(hipStreamSynchronize(st));
 }
 

@@ -11,16 +11,31 @@ namespace c2 {
 static constexpr int BX = 8, BY = 8, BZ = 4;  // brick of 256 nodes
 static const double PI_D = 3.14159265358979323846264338327950288;
 
+// The nodes: those of a periodic grid of the cell x2c (box == 0), at
+// x2c (i/n1, j/n2, k/n3), or those of an nciplot cube (box == 1), at
+// x0 + i xm(:,1) + j xm(:,2) + k xm(:,3) (nci@proc.f90:548).
 struct Geo {
   int n1, n2, n3;
   int nb1, nb2, nb3;  // bricks per axis
   double x2c[9];
+  int box;
+  double x0[3], xm[9];
 };
 
-__device__ __forceinline__ void node_cart(const Geo &G, int i, int j, int k, double xc[3]) {
-  const double f0 = (double)i / G.n1, f1 = (double)j / G.n2, f2 = (double)k / G.n3;
+// Cartesian position at real-valued node coordinates (u, v, w).
+__device__ __forceinline__ void geo_cart(const Geo &G, double u, double v, double w, double xc[3]) {
+  if (G.box) {
 #pragma unroll
-  for (int r = 0; r < 3; r++) xc[r] = G.x2c[r] * f0 + G.x2c[r + 3] * f1 + G.x2c[r + 6] * f2;
+    for (int r = 0; r < 3; r++) xc[r] = G.x0[r] + u * G.xm[r] + v * G.xm[r + 3] + w * G.xm[r + 6];
+  } else {
+    const double f0 = u / G.n1, f1 = v / G.n2, f2 = w / G.n3;
+#pragma unroll
+    for (int r = 0; r < 3; r++) xc[r] = G.x2c[r] * f0 + G.x2c[r + 3] * f1 + G.x2c[r + 6] * f2;
+  }
+}
+
+__device__ __forceinline__ void node_cart(const Geo &G, int i, int j, int k, double xc[3]) {
+  geo_cart(G, (double)i, (double)j, (double)k, xc);
 }
 
 // Middle eigenvalue of a symmetric 3x3 matrix (trigonometric closed form of
@@ -54,6 +69,7 @@ struct NciOut {
   void *crho, *cgrad;  // double* or float*
   int is_f32;
   int zfast;
+  void *rhoat = nullptr;  // promolecular density (same type), when wanted
 };
 
 __device__ __forceinline__ size_t out_index(const Geo &G, int zfast, int i, int j, int k) {

@@ -7,6 +7,12 @@ interpolation="grid", the tricubic grid interpolant (:565-571);
 crystalmod@env.f90:622-753).  Both return crho = sign(rho, lambda2)*100 and
 cgrad = s, z-fastest as nci@proc.f90:502-505 stores them, unless
 zfast=False.  ``cutoffs`` is the cutoff pass of :615-643.
+
+nciplot's node sets (``cell_nodes``, ``box_nodes``, ``fragment_box``,
+nci@proc.f90:384-446), the promolecular block on any of them with the
+promolecular density rhoat (``promol_nodes``), and the FRAGMENT densities
+(``fragment_atoms``, ``fragments``, :580-596, read_fragment :949-1003) feed
+the cutoff pass with fragments (:622-626).
 """
 from __future__ import annotations
 
@@ -85,13 +91,16 @@ def nciplot_promol(ctx: Context, n, x2c, tables: list[Grid1], xat, species, prec
 
 
 def cutoffs(ctx: Context, crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e30, 1e30),
-            onlyneg=False, rhoat=None, rho_void=-1.0, dat=True):
-    """nciplot's cutoff pass (nci@proc.f90:612-654, no fragments) on the GPU
-    through c2gpu_nci_cutoff.  crho, cgrad (and rhoat with rho_void > 0):
-    z-fastest arrays of one type (float64 or float32), host numpy arrays or
-    DeviceArrays.  cgrad is masked in place (s <- 100).  Returns (dat_text,
-    sums): the .dat rows '(1p,E15.7,E15.7)' as bytes (None if dat is False)
-    and the unscaled void sums (charge, promolecular charge, node count)."""
+            onlyneg=False, rhoat=None, rho_void=-1.0, dat=True, rhofrag=None, rhoparam=0.95, rhoparam2=0.75):
+    """nciplot's cutoff pass (nci@proc.f90:612-654) on the GPU through
+    c2gpu_nci_cutoff, or c2gpu_nci_cutoff_frag with fragments.  crho, cgrad
+    (and rhoat with rho_void > 0 or fragments): z-fastest arrays of one type
+    (float64 or float32), host numpy arrays or DeviceArrays; rhofrag:
+    (nfrag, npts) fragment densities as ``fragments`` returns them, with
+    RHOPARAM and RHOPARAM2 (defaults of nci@proc.f90:124-125).  cgrad is
+    masked in place (s <- 100).  Returns (dat_text, sums): the .dat rows
+    '(1p,E15.7,E15.7)' as bytes (None if dat is False) and the unscaled void
+    sums (charge, promolecular charge, node count)."""
     import ctypes as C
     dev = hasattr(crho, "data_ptr")
     dt = np.dtype(crho.dtype)
@@ -102,18 +111,26 @@ def cutoffs(ctx: Context, crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srho
         crho = np.ascontiguousarray(crho)
         if rhoat is not None:
             rhoat = np.ascontiguousarray(rhoat, dtype=dt)
+        if rhofrag is not None:
+            rhofrag = np.ascontiguousarray(rhofrag, dtype=dt)
     npts = int(np.prod(crho.shape))
+    nfrag = 0 if rhofrag is None else int(rhofrag.shape[0])
     n = np.array([npts, 1, 1], dtype=np.int32)
     o = _lib.NciCut(rhocut, dimcut, rhoplot, (C.c_double * 2)(*srhorange), int(bool(onlyneg)), rho_void)
     flags = _lib.PTR_DEVICE if dev else 0
     ln = C.c_uint64()
     sums = np.zeros(3)
-    args = (ctx.h, _lib.ptr(crho), _lib.ptr(cgrad), _lib.ptr(rhoat), n.ctypes.data, prec, C.byref(o), flags)
-    ctx.check(ctx.lib.c2gpu_nci_cutoff(*args, None, 0, C.byref(ln), None), "c2gpu_nci_cutoff")
+    if nfrag:
+        fn, name = ctx.lib.c2gpu_nci_cutoff_frag, "c2gpu_nci_cutoff_frag"
+        args = (ctx.h, _lib.ptr(crho), _lib.ptr(cgrad), _lib.ptr(rhoat), _lib.ptr(rhofrag), nfrag,
+                float(rhoparam), float(rhoparam2), n.ctypes.data, prec, C.byref(o), flags)
+    else:
+        fn, name = ctx.lib.c2gpu_nci_cutoff, "c2gpu_nci_cutoff"
+        args = (ctx.h, _lib.ptr(crho), _lib.ptr(cgrad), _lib.ptr(rhoat), n.ctypes.data, prec, C.byref(o), flags)
+    ctx.check(fn(*args, None, 0, C.byref(ln), None), name)
     size = max(ln.value, 1)
     buf = ctx.empty((size,), np.uint8) if dev else np.empty(size, dtype=np.uint8)
-    ctx.check(ctx.lib.c2gpu_nci_cutoff(*args, _lib.ptr(buf), size, C.byref(ln), sums.ctypes.data),
-              "c2gpu_nci_cutoff")
+    ctx.check(fn(*args, _lib.ptr(buf), size, C.byref(ln), sums.ctypes.data), name)
     text = None
     if dat:
         text = (buf.numpy() if dev else buf)[:ln.value].tobytes()
@@ -125,3 +142,119 @@ def dat_rows(text: bytes) -> np.ndarray:
     if not text:
         return np.zeros((0, 2))
     return np.array(text.split(), dtype=np.float64).reshape(-1, 2)
+
+
+# ------------------------------------------------------------------ node sets and fragments
+
+BOHRTOA = 0.52917720859  # param.F90:42
+ATOMEPS = 1e-2           # global.f90:33, identify_atom's default distance (bohr)
+
+
+def _nodes(nstep, x0, xmat) -> _lib.NciNodes:
+    nd = _lib.NciNodes()
+    nd.nstep[:] = [int(v) for v in nstep]
+    nd.x0[:] = [float(v) for v in x0]
+    nd.xmat[:] = [float(v) for v in np.asarray(xmat, dtype=np.float64).reshape(3, 3).T.reshape(-1)]
+    return nd
+
+
+def cell_nodes(x2c, nstep) -> _lib.NciNodes:
+    """The periodic cell's nodes (nci@proc.f90:413-426): x0 = 0,
+    xmat(:,c) = a_c / nstep(c).  x2c: lattice vectors as columns."""
+    a = np.asarray(x2c, dtype=np.float64)
+    xm = np.stack([a[:, c] / float(nstep[c]) for c in range(3)], axis=1)
+    return _nodes(nstep, (0.0, 0.0, 0.0), xm)
+
+
+def _box(x0, x1, nstep, xinc):
+    x0, x1 = np.asarray(x0, dtype=np.float64), np.asarray(x1, dtype=np.float64)
+    lo, hi = np.minimum(x0, x1), np.maximum(x0, x1)
+    if nstep is None:  # nci@proc.f90:436-437
+        xinc = np.broadcast_to(np.asarray(xinc, dtype=np.float64), (3,))
+        nstep = np.ceil(np.abs(hi - lo) / xinc).astype(int)
+    else:              # :438-439
+        nstep = np.asarray(nstep, dtype=int)
+        xinc = (hi - lo) / nstep.astype(np.float64)
+    return _nodes(nstep, lo, np.diag(xinc))
+
+
+def box_nodes(x2c, x0_cryst, x1_cryst, nstep=None, xinc=0.1) -> _lib.NciNodes:
+    """The nodes of NCIPLOT's CUBE x0 x1 box (crystallographic corners,
+    nci@proc.f90:427-443): lower Cartesian corner, step xinc per axis
+    (bohr), nstep = ceiling(|x1 - x0| / xinc), or xinc = (x1 - x0)/nstep
+    with NSTEP."""
+    a = np.asarray(x2c, dtype=np.float64)
+    return _box(a @ np.asarray(x0_cryst, dtype=np.float64), a @ np.asarray(x1_cryst, dtype=np.float64), nstep, xinc)
+
+
+def fragment_box(frags, rthres=2.0, nstep=None, xinc=0.1) -> _lib.NciNodes:
+    """The box NCIPLOT builds around its fragments when no CUBE is given
+    (findlimits, nci@proc.f90:385-403): the atoms' bounding box widened by
+    rthres (bohr, the default 2) on every side.  frags: list of (xyz, species)."""
+    xyz = np.concatenate([np.asarray(f[0], dtype=np.float64).reshape(-1, 3) for f in frags])
+    return _box(xyz.min(0) - rthres, xyz.max(0) + rthres, nstep, xinc)
+
+
+def fragment_atoms(x2c, xat, species, coords_ang):
+    """A FRAGMENT block as read_fragment reads it (nci@proc.f90:949-1003):
+    Cartesian angstrom -> bohr, and only the positions that coincide with an
+    atom of the crystal (some lattice translate within ATOMEPS, identify_atom)
+    are kept, with that atom's species.  Returns (xyz_bohr (m, 3), species (m,))."""
+    a = np.asarray(x2c, dtype=np.float64)
+    c2x = np.linalg.inv(a)
+    xa = np.asarray(xat, dtype=np.float64).reshape(-1, 3)
+    sp = np.asarray(species).reshape(-1)
+    shifts = np.indices((3, 3, 3)).reshape(3, -1).T - 1  # the 27 neighbouring translates
+    keep, spc = [], []
+    for x in np.asarray(coords_ang, dtype=np.float64).reshape(-1, 3) / BOHRTOA:
+        d = c2x @ x - xa
+        d -= np.round(d)
+        dist = np.sqrt((((d[:, None, :] + shifts[None, :, :]) @ a.T) ** 2).sum(-1)).min(1)
+        i = int(np.argmin(dist))
+        if dist[i] < ATOMEPS:
+            keep.append(x)
+            spc.append(int(sp[i]))
+    return np.array(keep).reshape(-1, 3), np.array(spc, dtype=np.int32)
+
+
+def promol_nodes(ctx: Context, nodes, x2c, tables: list[Grid1], xat, species, precision="fp64", outs=None):
+    """The promolecular NCI block on an nciplot node set (c2gpu_nci_promol_nodes):
+    (crho, cgrad, rhoat), z-fastest, of the precision's type."""
+    prec = _lib.FP32 if precision == "fp32" else _lib.FP64
+    npts = int(np.prod(list(nodes.nstep)))
+    pm, keep = _promol_struct(tables, xat, species)
+    x = _lib.mat_f(x2c)
+    if outs is None:
+        dt = np.float32 if prec == _lib.FP32 else np.float64
+        outs = tuple(np.zeros(npts, dtype=dt) for _ in range(3))
+        flags = 0
+    else:
+        flags = _lib.PTR_DEVICE
+    rc = ctx.lib.c2gpu_nci_promol_nodes(ctx.h, _lib.C.byref(nodes), x.ctypes.data, _lib.C.byref(pm), prec, flags,
+                                        *(_lib.ptr(o) for o in outs))
+    ctx.check(rc, "c2gpu_nci_promol_nodes")
+    del keep
+    return outs
+
+
+def fragments(ctx: Context, nodes, tables: list[Grid1], frags, precision="fp64", out=None):
+    """Fragment densities on an nciplot node set (c2gpu_nci_fragments):
+    frags is a list of (xyz_bohr (m, 3), species (m,)); returns (nfrag, npts)."""
+    prec = _lib.FP32 if precision == "fp32" else _lib.FP64
+    npts = int(np.prod(list(nodes.nstep)))
+    pm, keep = _promol_struct(tables, np.zeros((1, 3)), np.zeros(1, dtype=np.int32))
+    nat = np.array([len(np.asarray(f[1]).reshape(-1)) for f in frags], dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate([np.asarray(f[0], dtype=np.float64).reshape(-1, 3) for f in frags])
+                               .reshape(-1)) if nat.sum() else np.zeros(3)
+    spc = np.ascontiguousarray(np.concatenate([np.asarray(f[1], dtype=np.int32).reshape(-1) for f in frags])) \
+        if nat.sum() else np.zeros(1, dtype=np.int32)
+    if out is None:
+        out = np.zeros((len(frags), npts), dtype=np.float32 if prec == _lib.FP32 else np.float64)
+        flags = 0
+    else:
+        flags = _lib.PTR_DEVICE
+    rc = ctx.lib.c2gpu_nci_fragments(ctx.h, _lib.C.byref(nodes), pm.nspecies, pm.tables, len(frags),
+                                     nat.ctypes.data, xyz.ctypes.data, spc.ctypes.data, prec, flags, _lib.ptr(out))
+    ctx.check(rc, "c2gpu_nci_fragments")
+    del keep
+    return out

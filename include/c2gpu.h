@@ -232,6 +232,43 @@ int c2gpu_nci_f32(c2gpu_ctx *ctx, const double *rho, const int n[3], const doubl
                   int scheme, int precision, const c2gpu_promol *promol, int zfast,
                   int flags, float *crho_out, float *cgrad_out);
 
+/* The nodes of an nciplot cube (nci@proc.f90:412-446, :548):
+ *   x = x0 + i*xmat(:,1) + j*xmat(:,2) + k*xmat(:,3),  0 <= i < nstep[0] ...
+ * Cartesian bohr, xmat column-major.  The periodic cell: x0 = 0 and
+ * xmat(:,c) = a_c / nstep[c] (:413-426); a CUBE or fragment box: x0 its
+ * lower corner and xmat = diag(xinc) (:427-443). */
+typedef struct {
+  int nstep[3];
+  double x0[3];
+  double xmat[9];
+} c2gpu_nci_nodes;
+
+/* The promolecular NCI block (nci@proc.f90:548-604 with the promolecular
+ * field, crystalmod@env.f90:622-753) on the nodes of an nciplot cube, in
+ * the cell or a box: crho = sign(rho, lambda2) * 100, cgrad = s, and, when
+ * rhoat_out is not NULL, rhoat = rho, the promolecular density that
+ * dopromol stores (nci@proc.f90:574-577, 600).  x2c: the cell, whose lattice
+ * repeats the atoms of promol.  precision: C2GPU_FP64 (double outputs) or
+ * C2GPU_FP32 (float outputs).  Outputs z-fastest,
+ * (0:nstep[2]-1, 0:nstep[1]-1, 0:nstep[0]-1), as nci@proc.f90:502-505. */
+int c2gpu_nci_promol_nodes(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nodes, const double x2c[9],
+                           const c2gpu_promol *promol, int precision, int flags, void *crho_out,
+                           void *cgrad_out, void *rhoat_out);
+
+/* NCIPLOT fragments (FRAGMENT ... ENDFRAGMENT, nci@proc.f90:580-596): per
+ * node and fragment, the promolecular density of the fragment's atoms alone,
+ * at the positions given (no lattice repeats): the sum over its atoms within
+ * rmax of their table of max(interp(max(d, r(1), 1e-14)), 0).
+ * frag_nat[f]: atoms of fragment f; xyz: Cartesian bohr, 3 per atom,
+ * fragments one after the other; species: radial-table index per atom.
+ * rhofrag_out(k, j, i, f): z fastest, fragment slowest, nstep product times
+ * nfrag values of the precision's type (C2GPU_FP64: double, C2GPU_FP32:
+ * float). */
+int c2gpu_nci_fragments(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nodes, int nspecies,
+                        const c2gpu_radial_table *tables, int nfrag, const int *frag_nat,
+                        const double *xyz, const int *species, int precision, int flags,
+                        void *rhofrag_out);
+
 /* Options of nciplot's cutoff pass (nci@proc.f90:612-643; keywords
  * CUTOFFS rhocut dimcut, CUTPLOT rhoplot, SRHORANGE, ONLYNEG, RHO_VOID). */
 typedef struct {
@@ -243,7 +280,7 @@ typedef struct {
                              it count; the void sums are returned           */
 } c2gpu_nci_cut;
 
-/* nciplot's cutoff pass (nci@proc.f90:612-654, without fragments) over crho
+/* nciplot's cutoff pass (nci@proc.f90:612-654) without fragments over crho
  * (sign(l2) rho * 100) and cgrad (s) as c2gpu_nci returns them, z fastest:
  * the .dat rows '(1p,E15.7,E15.7)' rho, s of the selected nodes in node
  * order (31 bytes each) into dat, then cgrad <- 100 where masked (in place).
@@ -256,6 +293,18 @@ typedef struct {
 int c2gpu_nci_cutoff(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *rhoat, const int n[3],
                      int precision, const c2gpu_nci_cut *opts, int flags, char *dat, uint64_t cap,
                      uint64_t *len, double sums[3]);
+
+/* The cutoff pass with fragments (nci@proc.f90:622-626): as
+ * c2gpu_nci_cutoff, where a node counts as intermolecular only when
+ *   S = sum_f rhofrag(f) >= rhoparam2 * rhoat  and  rhofrag(f) <= S * rhoparam
+ * for every fragment (keywords RHOPARAM2, RHOPARAM; critic2's defaults are
+ * 0.75 and 0.95), and with RHO_VOID also rhoat < rho_void.  rhofrag: nfrag
+ * arrays of the nodes, one after the other, as c2gpu_nci_fragments writes
+ * them; rhoat is required.  nfrag = 0 is c2gpu_nci_cutoff. */
+int c2gpu_nci_cutoff_frag(c2gpu_ctx *ctx, const void *crho, void *cgrad, const void *rhoat,
+                          const void *rhofrag, int nfrag, double rhoparam, double rhoparam2,
+                          const int n[3], int precision, const c2gpu_nci_cut *opts, int flags,
+                          char *dat, uint64_t cap, uint64_t *len, double sums[3]);
 
 /* ------------------------------------------------------------------ */
 /* Batches of cells                                                    */

@@ -106,6 +106,14 @@ void oracle_promol(const double xc[3], int nimg, const double *ximg,
                    const int *zidx, const orc_rgrid *tabs, const double *cut,
                    int nder, double *f, double g[3], double h[9]);
 
+/* Fragment densities at Cartesian point xc (nci@proc.f90:580-596): for
+ * each of nfrag fragments (fstart[f] .. fstart[f+1]-1 index its atoms,
+ * Cartesian positions xyz, table zidx), the sum over its atoms within rmax
+ * of max(interp(max(d, r(1), 1e-14)), 0). */
+void oracle_frag_density(const double xc[3], int nfrag, const int *fstart,
+                         const double *xyz, const int *zidx,
+                         const orc_rgrid *tabs, double *out);
+
 /* NCI per-point quantities from node values (nci@proc.f90:543-605):
  * s = gmod / (const * max(rho,1d-80)^(4/3)); crho = sign(rho, l2)*100. */
 void oracle_nci_point(double rho, double gmod, const double ehess_or_h[9],

@@ -7,6 +7,7 @@
  *   grid1 table build      grid1mod@proc.f90:206-331 (read_critic, 6-point FD)
  *   grid1%interp           grid1mod@proc.f90:86-137 (4-point Lagrange, log grid)
  *   promolecular_atom      crystalmod@env.f90:622-753
+ *   fragment densities     nci@proc.f90:580-596
  *   NCI point (s, sign)    nci@proc.f90:91, :543-605
  */
 #include "c2oracle.h"
@@ -131,6 +132,28 @@ void oracle_promol(const double xc[3], int nimg, const double *ximg,
         h[k + 3 * j] += radd;
       }
     }
+  }
+}
+
+void oracle_frag_density(const double xc[3], int nfrag, const int *fstart,
+                         const double *xyz, const int *zidx,
+                         const orc_rgrid *tabs, double *out) {
+  for (int ifr = 0; ifr < nfrag; ifr++) {
+    double acc = 0.0;
+    for (int iat = fstart[ifr]; iat < fstart[ifr + 1]; iat++) {
+      const orc_rgrid *g = &tabs[zidx[iat]];
+      double xd[3];
+      for (int c = 0; c < 3; c++) xd[c] = xc[c] - xyz[3 * iat + c];
+      double dist = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
+      if (dist > g->rmax) continue;
+      if (dist < g->r[0]) dist = g->r[0];
+      if (dist < 1e-14) dist = 1e-14;
+      double rrho, rrho1, rrho2;
+      oracle_grid1_interp(g, dist, &rrho, &rrho1, &rrho2);
+      if (rrho < 0.0) rrho = 0.0;
+      acc = acc + rrho;
+    }
+    out[ifr] = acc;
   }
 }
 

@@ -60,6 +60,7 @@ def load():
         lib.oracle_grid1_build.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         lib.oracle_grid1_interp.argtypes = [C.POINTER(OrcRgrid), C.c_double, vp, vp, vp]
         lib.oracle_promol.argtypes = [vp, C.c_int, vp, vp, C.POINTER(OrcRgrid), vp, C.c_int, vp, vp, vp]
+        lib.oracle_frag_density.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(OrcRgrid), vp]
         lib.oracle_nci_point.argtypes = [C.c_double, C.c_double, vp, C.c_int, vp, vp]
         lib.oracle_slater_grid.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
         _lib = lib
@@ -175,6 +176,41 @@ def promol_nci(points_cart, images, img_species, tables, full_hessian=True):
         lib.oracle_nci_point(f.value, gm, h.ctypes.data, 1 if full_hessian else 0, C.byref(cr), C.byref(s))
         out[q] = (f.value, gm, s.value, cr.value)
     return out
+
+
+def frag_density(points_cart, frags, tables):
+    """Fragment densities (nci@proc.f90:580-596) at Cartesian points; frags:
+    list of (xyz (m, 3) Cartesian, table index (m,)).  Returns (nfrag, npts)."""
+    lib = load()
+    tabs = (OrcRgrid * len(tables))(*[t.s for t in tables])
+    nat = [len(np.asarray(f[1]).reshape(-1)) for f in frags]
+    fstart = np.ascontiguousarray(np.concatenate([[0], np.cumsum(nat)]).astype(np.int32))
+    xyz = np.ascontiguousarray(np.concatenate([np.asarray(f[0], dtype=np.float64).reshape(-1)
+                                               for f in frags] + [np.zeros(3)]))
+    zi = np.ascontiguousarray(np.concatenate([np.asarray(f[1], dtype=np.int32).reshape(-1)
+                                              for f in frags] + [np.zeros(1, dtype=np.int32)]))
+    pts = np.asarray(points_cart, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(pts), len(frags)))
+    row = np.zeros(len(frags))
+    for q, x in enumerate(pts):
+        xc = np.ascontiguousarray(x)
+        lib.oracle_frag_density(xc.ctypes.data, len(frags), fstart.ctypes.data, xyz.ctypes.data, zi.ctypes.data,
+                                tabs, row.ctypes.data)
+        out[q] = row
+    return out.T.copy()
+
+
+def box_points(nstep, x0, xmat):
+    """Cartesian nodes x0 + i xmat(:,1) + j xmat(:,2) + k xmat(:,3) in
+    nciplot's z-fastest order (nci@proc.f90:545-548); xmat columns are the
+    steps."""
+    xm = np.asarray(xmat, dtype=np.float64).reshape(3, 3)
+    out = []
+    for i in range(nstep[0]):
+        for j in range(nstep[1]):
+            for k in range(nstep[2]):
+                out.append(np.asarray(x0, dtype=np.float64) + i * xm[:, 0] + j * xm[:, 1] + k * xm[:, 2])
+    return np.array(out)
 
 
 def nci_point(rho, gmod, hdiag):
@@ -348,18 +384,27 @@ def read_bincube(path):
 
 
 def nci_cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e30, 1e30), onlyneg=False,
-                rhoat=None, rho_void=-1.0):
-    """Cutoff pass of nci@proc.f90:612-643 (no fragments), restated with
-    numpy: the (rho, s) rows of the .dat file in node order, cgrad with 100
-    outside the plot range, and the unscaled void sums."""
+                rhoat=None, rho_void=-1.0, rhofrag=None, rhoparam=0.95, rhoparam2=0.75):
+    """Cutoff pass of nci@proc.f90:612-643, restated with numpy: the (rho, s)
+    rows of the .dat file in node order, cgrad with 100 outside the plot
+    range, and the unscaled void sums.  rhofrag (nfrag, npts): the fragment
+    test of :622-626, its sum taken fragment by fragment."""
     crho = np.asarray(crho, dtype=np.float64).reshape(-1)
     cg = np.array(cgrad, dtype=np.float64, copy=True).reshape(-1)
     inter = np.ones(crho.shape, dtype=bool)
     sums = np.zeros(3)
+    if rhofrag is not None and len(rhofrag):
+        rf = np.asarray(rhofrag, dtype=np.float64)
+        ra = np.asarray(rhoat, dtype=np.float64).reshape(-1)
+        tot = np.zeros(crho.shape)
+        for f in range(rf.shape[0]):
+            tot = tot + rf[f]
+        inter = (tot >= rhoparam2 * ra) & np.all(rf <= tot * rhoparam, axis=0)
     if rho_void > 0:
         ra = np.asarray(rhoat, dtype=np.float64).reshape(-1)
-        inter = ra < rho_void
-        sums = np.array([np.sum(np.abs(crho[inter]) / 100.0), np.sum(ra[inter]), float(np.count_nonzero(inter))])
+        void = ra < rho_void
+        inter = inter & void
+        sums = np.array([np.sum(np.abs(crho[void]) / 100.0), np.sum(ra[void]), float(np.count_nonzero(void))])
     sel = (np.abs(crho) < rhocut * 100.0) & (cg < dimcut) & inter
     dat = np.stack([crho[sel] / 100.0, cg[sel]], axis=1)
     cg[(np.abs(crho) > rhoplot * 100.0) | ~inter] = 100.0

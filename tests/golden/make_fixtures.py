@@ -8,6 +8,11 @@ from the reference's own test and data files:
                      and the expected node values of the reference test
                      tests/015_grdplot/005_nciplot_basic (its ref/*-dens.cube,
                      ref/*-grad.cube and ref/*.dat), plus the test's options.
+  urea_frag.json     tests/015_grdplot/006_nciplot_fragment: its CUBE box,
+                     options and FRAGMENT coordinates (from the .cri input)
+                     and the expected node values (ref/*-dens.cube,
+                     ref/*-grad.cube, ref/*-pdens.cube, ref/*.dat) and the
+                     cutoffs the run printed (ref/*.cro).
   urea_tables.npz    radial grids r and rr0 = sum(occ*psi^2) of the H, C, N, O
                      atomic wfc files (dat/wfc/*_pbe.wfc), truncated where
                      critic2's read_critic stops reading (rho < 1e-8).
@@ -69,6 +74,7 @@ def main():
     }
     with open(os.path.join(HERE, "urea_nci.json"), "w") as fh:
         json.dump(out, fh, indent=1)
+    fragment_case()
     zmap = {"h": 1, "c": 6, "n": 7, "o": 8}
     arrs = {}
     for sym, z in zmap.items():
@@ -78,6 +84,61 @@ def main():
         arrs[f"rr0_{sym}"] = rr0
     np.savez_compressed(os.path.join(HERE, "urea_tables.npz"), **arrs)
     tricubic_vectors()
+
+
+def fragment_case():
+    """urea_frag.json from tests/015_grdplot/006_nciplot_fragment."""
+    cri = open(os.path.join(REF, "tests/015_grdplot/006_nciplot_fragment.cri")).read().splitlines()
+    frags, cur, box, nstep, onlyneg = [], None, None, None, False
+    for ln in cri:
+        t = ln.split()
+        if not t or t[0].startswith("#"):
+            continue
+        w = t[0].lower()
+        if w == "fragment":
+            cur = []
+        elif w == "endfragment":
+            frags.append(cur)
+            cur = None
+        elif cur is not None:
+            cur.append([float(v) for v in t[:3]])
+        elif w == "cube":
+            box = [float(v) for v in t[1:7]]
+        elif w == "nstep":
+            nstep = [int(v) for v in t[1:4]]
+        elif w == "onlyneg":
+            onlyneg = True
+    base = os.path.join(REF, "tests/015_grdplot/ref/006_nciplot_fragment")
+    cro = open(base + ".cro").read()
+
+    def printed(label):
+        i = cro.index(label)
+        return float(cro[i + len(label):].split()[0])
+
+    hdr = open(base + "-pdens.cube").read().splitlines()
+    nat = int(hdr[2].split()[0])
+    out = {
+        "source": "tests/015_grdplot/006_nciplot_fragment.cri and its ref/ outputs",
+        "cube_origin_bohr": [float(v) for v in hdr[2].split()[1:4]],
+        "cube_steps_bohr": [[float(v) for v in hdr[3 + c].split()[1:4]] for c in range(3)],
+        "cube_atoms_z": [int(hdr[6 + a].split()[0]) for a in range(nat)],
+        "cube_cryst": [box[:3], box[3:]],
+        "nstep": nstep,
+        "onlyneg": onlyneg,
+        "fragments_angstrom": frags,
+        "options": {"rhocut": printed("Density cutoff (for dat file):"),
+                    "dimcut": printed("RDG cutoff (for dat file):"),
+                    "rhoplot": printed("Density plot cutoff:"),
+                    "rhoparam": 0.95, "rhoparam2": 0.75},
+        "cube_order": "z fastest: value index = k + 2*(j + 2*i)",
+        "dens_cube": cube_values(base + "-dens.cube"),
+        "grad_cube": cube_values(base + "-grad.cube"),
+        "pdens_cube": cube_values(base + "-pdens.cube"),
+        "dat": [[float(v) for v in ln.split()] for ln in open(base + ".dat") if ln.strip()],
+        "cube_atol": 1e-5,
+    }
+    with open(os.path.join(HERE, "urea_frag.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
 
 
 def tricubic_vectors():

@@ -1,0 +1,148 @@
+"""GPU parity of nciplot's node sets and fragments: the promolecular block
+on CUBE boxes (c2gpu_nci_promol_nodes), the fragment densities
+(c2gpu_nci_fragments) and the cutoff pass with fragments
+(c2gpu_nci_cutoff_frag) against the reference's 006_nciplot_fragment
+outputs and the oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from critic2_amd import nci as gn
+from critic2_amd.radial import slater_grid1
+from critic2_amd.synth import random_cell
+from oracle import oracle
+from tests.cubetext import fortran_e
+from tests.helpers import GOLDEN, fmt6, images, urea
+
+pytestmark = pytest.mark.gpu
+
+
+def _box_points(nodes):
+    xm = np.array(list(nodes.xmat)).reshape(3, 3).T
+    return oracle.box_points(list(nodes.nstep), list(nodes.x0), xm)
+
+
+def test_fragment_golden_006_on_gpu(ctx):
+    """tests/015_grdplot/006_nciplot_fragment through the GPU: density and
+    promolecular cubes to their printed digits, fragment densities against
+    the oracle, every RDG node masked, no .dat row."""
+    d, x2c, tables, xat, species = urea()
+    f = json.load(open(os.path.join(GOLDEN, "urea_frag.json")))
+    nb = gn.box_nodes(x2c, *f["cube_cryst"], nstep=f["nstep"])
+    crho, cg, rhoat = gn.promol_nodes(ctx, nb, x2c, tables, xat, species, precision="fp64")
+    assert [fmt6(v) for v in crho] == f["dens_cube"]
+    assert [fmt6(v) for v in rhoat] == f["pdens_cube"]
+    frags = [gn.fragment_atoms(x2c, xat, species, np.array(fr)) for fr in f["fragments_angstrom"]]
+    rf = gn.fragments(ctx, nb, tables, frags, precision="fp64")
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    np.testing.assert_allclose(rf, oracle.frag_density(_box_points(nb), frags, otabs), rtol=1e-13, atol=0)
+    o = f["options"]
+    text, _ = gn.cutoffs(ctx, crho, cg, rhocut=o["rhocut"], dimcut=o["dimcut"], rhoplot=o["rhoplot"],
+                         onlyneg=f["onlyneg"], rhoat=rhoat, rhofrag=rf, rhoparam=o["rhoparam"],
+                         rhoparam2=o["rhoparam2"])
+    assert text == b""
+    assert cg.tolist() == f["grad_cube"]
+
+
+def _slater_setup(cell):
+    zs = sorted(set(int(z) for z in cell.zat))
+    tables = [slater_grid1(z) for z in zs]
+    species = np.array([zs.index(int(z)) for z in cell.zat], dtype=np.int32)
+    return tables, species
+
+
+@pytest.mark.parametrize("ang", [(90, 90, 90), (80, 95, 105)])
+def test_promol_box_matches_oracle(ctx, ang):
+    """A CUBE box partly outside the cell (its atom images must reach past
+    the cell) against the oracle at the same nodes; rhoat is the
+    promolecular density."""
+    cell = random_cell(41, (6.0, 6.5, 7.0), ang, 5, (1, 6, 7, 8), (8, 8, 8))
+    tables, species = _slater_setup(cell)
+    nb = gn.box_nodes(cell.x2c, (-0.3, 0.1, 0.4), (0.9, 1.4, 1.2), nstep=(9, 7, 6))
+    crho, s, rhoat = gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision="fp64")
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    pos, idx = images(cell.x2c, cell.xat, lim=4)
+    ref = oracle.promol_nci(_box_points(nb), pos, species[idx], otabs)
+    np.testing.assert_allclose(rhoat, ref[:, 0], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(np.abs(crho), np.abs(ref[:, 3]), rtol=1e-12, atol=1e-300)
+    ok = ref[:, 0] > 1e-12
+    np.testing.assert_allclose(s[ok], ref[ok, 2], rtol=1e-9)
+    assert np.mean(np.sign(crho) == np.sign(ref[:, 3])) > 0.99
+    # FP32: float outputs within the mixed-precision error of the FP64 path
+    c32, s32, r32 = gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision="fp32")
+    assert c32.dtype == np.float32 and r32.dtype == np.float32
+    big = rhoat > 1e-6
+    assert np.max(np.abs(r32[big] - rhoat[big]) / rhoat[big]) < 1e-4
+
+
+def test_cell_nodes_equal_periodic_grid(ctx):
+    """The periodic cell as a node set (x0 = 0, xmat = a/n) gives the grid
+    path's values up to the rounding of the node positions."""
+    cell = random_cell(42, (6.0, 6.5, 7.0), (90, 90, 90), 4, (1, 6, 8), (6, 5, 7))
+    tables, species = _slater_setup(cell)
+    c0, s0 = gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp64")
+    c1, s1, r1 = gn.promol_nodes(ctx, gn.cell_nodes(cell.x2c, cell.n), cell.x2c, tables, cell.xat, species,
+                                 precision="fp64")
+    np.testing.assert_allclose(np.abs(c1), np.abs(c0), rtol=1e-12)
+    np.testing.assert_allclose(r1, np.abs(c0) / 100.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fragments_match_oracle(ctx, precision):
+    """Three fragments of image positions (one atom beyond rmax of every
+    node, one on a node) against the oracle."""
+    cell = random_cell(43, (7.0, 7.0, 7.5), (90, 90, 90), 6, (1, 6, 8), (8, 8, 8))
+    tables, species = _slater_setup(cell)
+    nb = gn.box_nodes(cell.x2c, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), nstep=(10, 9, 8))
+    pts = _box_points(nb)
+    pos, idx = images(cell.x2c, cell.xat, lim=1)
+    rng = np.random.default_rng(8)
+    part = rng.integers(0, 3, len(idx))
+    frags = [(pos[part == q], species[idx][part == q]) for q in range(3)]
+    far = np.array([[1e3, 0.0, 0.0]])
+    frags[2] = (np.concatenate([frags[2][0], far, pts[17:18]]), np.concatenate([frags[2][1], [0, 0]]))
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    ref = oracle.frag_density(pts, frags, otabs)
+    rf = gn.fragments(ctx, nb, tables, frags, precision=precision)
+    if precision == "fp64":
+        np.testing.assert_allclose(rf, ref, rtol=1e-13, atol=0)
+    else:
+        big = ref > 1e-6
+        err = np.max(np.abs(rf.astype(np.float64)[big] - ref[big]) / ref[big])
+        print(f"fragment density FP32 max relative error (rho > 1e-6): {err:.2e}")
+        assert err < 2e-4
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("device", [False, True])
+@pytest.mark.parametrize("opts", [{}, {"rho_void": 0.3, "onlyneg": True}, {"rhoparam": 0.7, "rhoparam2": 0.5}])
+def test_cutoff_frag_matches_oracle(ctx, dtype, device, opts):
+    """c2gpu_nci_cutoff_frag against the oracle: the same .dat rows (each
+    field the correctly rounded 1p,E15.7 of the value), the same masked RDG,
+    the void sums."""
+    rng = np.random.default_rng(11)
+    npts, nfrag = 20000, 3
+    crho = (rng.standard_normal(npts) * 10.0 ** rng.uniform(-5, 1.5, npts)).astype(dtype)
+    cg = (rng.uniform(0, 3, npts) * 10.0 ** rng.uniform(-3, 0.5, npts)).astype(dtype)
+    rhoat = (10.0 ** rng.uniform(-4, 0, npts)).astype(dtype)
+    share = rng.dirichlet(np.ones(nfrag) * 0.7, npts).T
+    rf = (share * rhoat.astype(np.float64) * rng.uniform(0.5, 1.1, npts)).astype(dtype)
+    kw = {"rhocut": 0.2, "dimcut": 2.0, "rhoplot": 0.1, "srhorange": (-1e30, 1e30), "onlyneg": False,
+          "rho_void": -1.0, "rhoparam": 0.95, "rhoparam2": 0.75}
+    kw.update(opts)
+    dat, cgo, sums_o = oracle.nci_cutoffs(crho.astype(np.float64), cg.astype(np.float64),
+                                          rhoat=rhoat.astype(np.float64), rhofrag=rf.astype(np.float64), **kw)
+    assert 0 < len(dat) < npts
+    want = "".join(fortran_e(float(a), 3) + fortran_e(float(b), 3) + "\n" for a, b in dat).encode()
+    g = cg.copy()
+    if device:
+        dc, dg, da, df = ctx.to_device(crho), ctx.to_device(g), ctx.to_device(rhoat), ctx.to_device(rf)
+        text, sums = gn.cutoffs(ctx, dc, dg, rhoat=da, rhofrag=df, **kw)
+        g = dg.numpy()
+    else:
+        text, sums = gn.cutoffs(ctx, crho, g, rhoat=rhoat, rhofrag=rf, **kw)
+    assert text == want
+    assert np.array_equal(g.astype(np.float64), cgo)
+    np.testing.assert_allclose(sums, sums_o, rtol=1e-12)

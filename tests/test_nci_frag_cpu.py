@@ -1,0 +1,146 @@
+"""NCIPLOT node sets and fragments on the CPU: the oracle's fragment
+densities and fragment cutoff test pinned by the reference's
+tests/015_grdplot/006_nciplot_fragment outputs, a partition identity, table
+nodes and hand-worked cases; the host helpers that mirror nciplot's box and
+FRAGMENT input."""
+import json
+import os
+
+import numpy as np
+
+from critic2_amd import nci as gn
+from critic2_amd.radial import slater_grid1
+from oracle import oracle
+from tests.helpers import GOLDEN, fmt6, images, urea
+
+SYMZ = [1, 6, 7, 8]  # the urea tables' species order (h, c, n, o)
+
+
+def _frag_case():
+    d, x2c, tables, xat, species = urea()
+    f = json.load(open(os.path.join(GOLDEN, "urea_frag.json")))
+    return d, f, x2c, tables, xat, species
+
+
+def _box_points(nodes):
+    xm = np.array(list(nodes.xmat)).reshape(3, 3).T
+    return oracle.box_points(list(nodes.nstep), list(nodes.x0), xm)
+
+
+def test_box_nodes_match_reference_cube_header():
+    """CUBE 0.2 0.2 0.2 0.8 0.8 0.8 with NSTEP 2 2 2: origin and steps as the
+    reference's cube header prints them (F12.6)."""
+    _, f, x2c, *_ = _frag_case()
+    nb = gn.box_nodes(x2c, *f["cube_cryst"], nstep=f["nstep"])
+    assert [round(v, 6) for v in nb.x0] == f["cube_origin_bohr"]
+    xm = np.array(list(nb.xmat)).reshape(3, 3).T
+    for c in range(3):
+        assert [round(v, 6) + 0.0 for v in xm[:, c]] == f["cube_steps_bohr"][c]
+
+
+def test_fragment_atoms_identify_crystal_species():
+    """read_fragment keeps positions on crystal atoms, with their species:
+    the reference lists them (Z) in its pdens cube header."""
+    _, f, x2c, _, xat, species = _frag_case()
+    z = []
+    for fr in f["fragments_angstrom"]:
+        xyz, sp = gn.fragment_atoms(x2c, xat, species, np.array(fr))
+        assert len(sp) == len(fr)
+        z += [SYMZ[s] for s in sp]
+    assert z == f["cube_atoms_z"]
+    # a position off every atom is dropped
+    xyz, sp = gn.fragment_atoms(x2c, xat, species, np.array([[0.1, 0.2, 0.3]]))
+    assert len(sp) == 0
+
+
+def test_oracle_fragment_golden_006():
+    """The whole 006_nciplot_fragment case through the oracle: density and
+    promolecular density cubes to their printed digits, every RDG node
+    masked and no .dat row.  Five nodes pass the .dat cutoffs and the two
+    negative ones every mask but the fragment test, so the empty .dat and
+    the masked RDG pin it."""
+    d, f, x2c, tables, xat, species = _frag_case()
+    nb = gn.box_nodes(x2c, *f["cube_cryst"], nstep=f["nstep"])
+    pts = _box_points(nb)
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    pos, idx = images(x2c, xat, lim=3)
+    ref = oracle.promol_nci(pts, pos, species[idx], otabs)
+    assert [fmt6(v) for v in ref[:, 3]] == f["dens_cube"]
+    assert [fmt6(v) for v in ref[:, 0]] == f["pdens_cube"]
+    frags = [gn.fragment_atoms(x2c, xat, species, np.array(fr)) for fr in f["fragments_angstrom"]]
+    rf = oracle.frag_density(pts, frags, otabs)
+    o = f["options"]
+    dat, cg, _ = oracle.nci_cutoffs(ref[:, 3], ref[:, 2], rhocut=o["rhocut"], dimcut=o["dimcut"],
+                                    rhoplot=o["rhoplot"], onlyneg=f["onlyneg"], rhoat=ref[:, 0], rhofrag=rf,
+                                    rhoparam=o["rhoparam"], rhoparam2=o["rhoparam2"])
+    assert len(dat) == 0 and f["dat"] == []
+    assert cg.tolist() == f["grad_cube"]
+    # without the fragment test five nodes would give rows, two nodes stay unmasked
+    dat0, cg0, _ = oracle.nci_cutoffs(ref[:, 3], ref[:, 2], rhocut=o["rhocut"], dimcut=o["dimcut"],
+                                      rhoplot=o["rhoplot"], onlyneg=f["onlyneg"])
+    assert len(dat0) == 5
+    assert np.count_nonzero(cg0 < 100.0) == 2
+
+
+def _slater_tables():
+    return [slater_grid1(z) for z in (1, 6, 8)]
+
+
+def test_oracle_fragments_partition_promolecular_density():
+    """Fragments that split every atom image within reach of the points
+    between them sum to the promolecular density (both cut at rmax here,
+    cutrad being larger), whatever the split."""
+    tables = _slater_tables()
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    assert all(g.cutrad >= g.rmax for g in tables)
+    rng = np.random.default_rng(5)
+    x2c = np.diag([7.0, 7.5, 8.0])
+    xat = rng.uniform(0, 1, (5, 3))
+    species = rng.integers(0, 3, 5).astype(np.int32)
+    pos, idx = images(x2c, xat, lim=3)
+    pts = rng.uniform(0, 1, (40, 3)) @ x2c.T
+    ref = oracle.promol_nci(pts, pos, species[idx], otabs)
+    part = rng.integers(0, 3, len(idx))
+    frags = [(pos[part == q], species[idx][part == q]) for q in range(3)]
+    rf = oracle.frag_density(pts, frags, otabs)
+    np.testing.assert_allclose(rf.sum(0), ref[:, 0], rtol=1e-12)
+    assert np.all(rf > 0)
+
+
+def test_oracle_fragment_single_atom_table_nodes():
+    """One atom at distance r_i of the table gives f_i (the interpolant
+    through its nodes); at the point itself f(r(1)); at rmax or beyond 0."""
+    g = slater_grid1(6)
+    otab = oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad)
+    x = np.array([[0.0, 0.0, 0.0]])
+    for i in (3, 100, 700, len(g.r) - 4):
+        d = np.array([[g.r[i], 0.0, 0.0]])
+        v = oracle.frag_density(x, [(d, np.array([0]))], [otab])[0, 0]
+        assert abs(v - max(g.f[i], 0.0)) <= 1e-12 * abs(g.f[i]) + 1e-300
+    v0 = oracle.frag_density(x, [(x.copy(), np.array([0]))], [otab])[0, 0]
+    assert abs(v0 - g.f[0]) <= 1e-12 * abs(g.f[0])
+    far = np.array([[g.rmax * 1.0000001, 0.0, 0.0]])
+    assert oracle.frag_density(x, [(far, np.array([0]))], [otab])[0, 0] == 0.0
+    # two fragments: each its own atoms only
+    both = oracle.frag_density(x, [(np.array([[g.r[100], 0, 0]]), np.array([0])), (far, np.array([0]))], [otab])
+    assert both[1, 0] == 0.0 and both[0, 0] > 0
+
+
+def test_oracle_fragment_cutoff_rule_hand_cases():
+    """inter = sum >= rhoparam2*rhoat and every fragment <= rhoparam*sum."""
+    crho = np.array([-1.0, -1.0, -1.0, -1.0])
+    cg = np.array([0.5, 0.5, 0.5, 0.5])
+    rhoat = np.array([1.0, 1.0, 1.0, 1.0])
+    rf = np.array([[0.5, 0.96, 0.3, 0.75],
+                   [0.5, 0.04, 0.3, 0.0]])
+    # node 0: 1.0 >= 0.75, 0.5 <= 0.95 -> inter
+    # node 1: 0.96 > 0.95 * 1.0         -> not
+    # node 2: 0.6 < 0.75                -> not
+    # node 3: 0.75 >= 0.75 but 0.75 <= 0.95*0.75 fails -> not
+    dat, cgo, _ = oracle.nci_cutoffs(crho, cg, rhocut=0.2, dimcut=2.0, rhoplot=1.0, rhoat=rhoat, rhofrag=rf)
+    assert dat.tolist() == [[-0.01, 0.5]]
+    assert cgo.tolist() == [0.5, 100.0, 100.0, 100.0]
+    # with RHO_VOID the void sums ignore the fragment test
+    dat, cgo, sums = oracle.nci_cutoffs(crho, cg, rhocut=0.2, dimcut=2.0, rhoplot=1.0, rhoat=rhoat, rhofrag=rf,
+                                        rho_void=2.0)
+    assert sums.tolist() == [0.04, 4.0, 4.0]
