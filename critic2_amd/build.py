@@ -82,11 +82,13 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
 
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_HOST = os.path.join(FORTRAN_DIR, "c2gpu_host")
+FORTRAN_NCI_HOST = os.path.join(FORTRAN_DIR, "c2gpu_nci_host")
 
 
 def build_fortran(verbose: bool = False) -> str | None:
-    """Compile the iso_c_binding module and the Fortran stand-in host with
-    amdflang (skipped, returning None, when amdflang is absent)."""
+    """Compile the iso_c_binding module and the Fortran stand-in hosts (Bader
+    and nciplot) with amdflang (skipped, returning None, when amdflang is
+    absent)."""
     fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
     if not os.path.exists(fc):
         return None
@@ -97,6 +99,9 @@ def build_fortran(verbose: bool = False) -> str | None:
          os.path.join(objdir, "c2gpu_mod.o"), "-module-dir", objdir],
         [fc, "-O2", "-I", objdir, os.path.join(FORTRAN_DIR, "c2gpu_host.f90"),
          os.path.join(objdir, "c2gpu_mod.o"), "-o", FORTRAN_HOST, "-L", HERE, "-lc2gpu",
+         "-Wl,-rpath,$ORIGIN/.."],
+        [fc, "-O2", "-I", objdir, os.path.join(FORTRAN_DIR, "c2gpu_nci_host.f90"),
+         os.path.join(objdir, "c2gpu_mod.o"), "-o", FORTRAN_NCI_HOST, "-L", HERE, "-lc2gpu",
          "-Wl,-rpath,$ORIGIN/.."],
     ]
     for cmd in steps:

@@ -46,6 +46,7 @@ module c2gpu
   public :: c2gpu_basin_sums, c2gpu_fft_deriv, c2gpu_relabel
   public :: c2gpu_cube_read_header, c2gpu_cube_read, c2gpu_bincube_read, c2gpu_cube_write
   public :: c2gpu_nci_cutoff, c2gpu_multipoles
+  public :: c2gpu_nci, c2gpu_nci_promol_nodes, c2gpu_nci_fragments, c2gpu_nci_cutoff_frag
 
   integer(c_int), parameter, public :: C2GPU_FP64 = 0, C2GPU_FP32 = 1
 
@@ -56,6 +57,33 @@ module c2gpu
      integer(c_int) :: onlyneg
      real(c_double) :: rho_void
   end type c2gpu_nci_cut
+
+  integer(c_int), parameter, public :: C2GPU_NCI_FFT = 0, C2GPU_NCI_GRID = 1, C2GPU_NCI_PROMOL = 2
+
+  ! radial density table of a species (grid1, grid1mod.f90:26-48); the
+  ! arrays r, f, fp, fpp (ngrid each) by address
+  type, bind(c), public :: c2gpu_radial_table
+     integer(c_int) :: ngrid
+     real(c_double) :: a, b, rmax, cutrad
+     type(c_ptr) :: r, f, fp, fpp
+  end type c2gpu_radial_table
+
+  ! promolecular density: nspecies tables, nat atoms (fractional xat(3,nat),
+  ! species(nat) table indices from 0)
+  type, bind(c), public :: c2gpu_promol
+     integer(c_int) :: nspecies
+     type(c_ptr) :: tables
+     integer(c_int) :: nat
+     type(c_ptr) :: xat, species
+  end type c2gpu_promol
+
+  ! nciplot's nodes x0 + i*xmat(:,1) + j*xmat(:,2) + k*xmat(:,3)
+  ! (nci@proc.f90:412-446, :548)
+  type, bind(c), public :: c2gpu_nci_nodes
+     integer(c_int) :: nstep(3)
+     real(c_double) :: x0(3)
+     real(c_double) :: xmat(9)
+  end type c2gpu_nci_nodes
 
   interface
      function c2gpu_ctx_create(ctx, device) bind(c, name="c2gpu_ctx_create")
@@ -190,6 +218,69 @@ module c2gpu
        real(c_double), intent(out) :: sums(3)
        integer(c_int) :: c2gpu_nci_cutoff
      end function c2gpu_nci_cutoff
+
+     function c2gpu_nci(ctx, rho, n, x2c, scheme, precision, promol, zfast, flags, crho_out, cgrad_out) &
+        bind(c, name="c2gpu_nci")
+       import :: c_ptr, c_int, c_double, c2gpu_promol
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: rho
+       integer(c_int), intent(in) :: n(3)
+       real(c_double), intent(in) :: x2c(9)
+       integer(c_int), value :: scheme, precision
+       type(c2gpu_promol), intent(in) :: promol
+       integer(c_int), value :: zfast, flags
+       real(c_double), intent(out) :: crho_out(*), cgrad_out(*)
+       integer(c_int) :: c2gpu_nci
+     end function c2gpu_nci
+
+     function c2gpu_nci_promol_nodes(ctx, nodes, x2c, promol, precision, flags, crho_out, cgrad_out, &
+        rhoat_out) bind(c, name="c2gpu_nci_promol_nodes")
+       import :: c_ptr, c_int, c_double, c2gpu_promol, c2gpu_nci_nodes
+       type(c_ptr), value :: ctx
+       type(c2gpu_nci_nodes), intent(in) :: nodes
+       real(c_double), intent(in) :: x2c(9)
+       type(c2gpu_promol), intent(in) :: promol
+       integer(c_int), value :: precision, flags
+       type(c_ptr), value :: crho_out, cgrad_out, rhoat_out
+       integer(c_int) :: c2gpu_nci_promol_nodes
+     end function c2gpu_nci_promol_nodes
+
+     function c2gpu_nci_fragments(ctx, nodes, nspecies, tables, nfrag, frag_nat, xyz, species, precision, &
+        flags, rhofrag_out) bind(c, name="c2gpu_nci_fragments")
+       import :: c_ptr, c_int, c_double, c2gpu_nci_nodes, c2gpu_radial_table
+       type(c_ptr), value :: ctx
+       type(c2gpu_nci_nodes), intent(in) :: nodes
+       integer(c_int), value :: nspecies
+       type(c2gpu_radial_table), intent(in) :: tables(*)
+       integer(c_int), value :: nfrag
+       integer(c_int), intent(in) :: frag_nat(*)
+       real(c_double), intent(in) :: xyz(*)
+       integer(c_int), intent(in) :: species(*)
+       integer(c_int), value :: precision, flags
+       type(c_ptr), value :: rhofrag_out
+       integer(c_int) :: c2gpu_nci_fragments
+     end function c2gpu_nci_fragments
+
+     function c2gpu_nci_cutoff_frag(ctx, crho, cgrad, rhoat, rhofrag, nfrag, rhoparam, rhoparam2, n, &
+        precision, opts, flags, dat, cap, len, sums) bind(c, name="c2gpu_nci_cutoff_frag")
+       import :: c_ptr, c_int, c_int64_t, c_double, c_char, c2gpu_nci_cut
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: crho(*)
+       real(c_double), intent(inout) :: cgrad(*)
+       real(c_double), intent(in) :: rhoat(*)
+       real(c_double), intent(in) :: rhofrag(*)
+       integer(c_int), value :: nfrag
+       real(c_double), value :: rhoparam, rhoparam2
+       integer(c_int), intent(in) :: n(3)
+       integer(c_int), value :: precision
+       type(c2gpu_nci_cut), intent(in) :: opts
+       integer(c_int), value :: flags
+       character(kind=c_char), intent(out) :: dat(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: len
+       real(c_double), intent(out) :: sums(3)
+       integer(c_int) :: c2gpu_nci_cutoff_frag
+     end function c2gpu_nci_cutoff_frag
 
      function c2gpu_multipoles(ctx, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out) &
         bind(c, name="c2gpu_multipoles")

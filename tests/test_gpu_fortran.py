@@ -77,3 +77,56 @@ def test_fortran_host_reads_cube(ctx, tmp_path):
     _, g = oracle.read_cube(str(path))
     ref, info, _, _ = oracle.bader(g, cell.n, cell.x2c, cell.xat)
     assert int(np.count_nonzero(lab != ref)) == 0
+
+
+def _nci_host_input(path, tables, x2c, xat, species, nodes, frags, opts):
+    """The stream file c2gpu_nci_host.f90 reads (its header lists the layout)."""
+    i4 = lambda *v: np.asarray(v, dtype=np.int32).tobytes()  # noqa: E731
+    f8 = lambda *v: np.asarray(v, dtype=np.float64).reshape(-1).tobytes()  # noqa: E731
+    b = [i4(len(tables))]
+    for g in tables:
+        b += [i4(g.ngrid), f8(g.a, g.b, g.rmax, g.cutrad), f8(g.r), f8(g.f), f8(g.fp), f8(g.fpp)]
+    b += [f8(np.asarray(x2c).reshape(-1, order="F")), i4(len(species)), f8(np.asarray(xat).reshape(-1)),
+          i4(*species)]
+    b += [i4(*nodes.nstep), f8(*nodes.x0), f8(*nodes.xmat)]
+    b += [i4(len(frags)), i4(*[len(s) for _, s in frags])]
+    b += [f8(np.concatenate([x.reshape(-1) for x, _ in frags])), i4(*np.concatenate([s for _, s in frags]))]
+    b += [f8(opts["rhocut"], opts["dimcut"], opts["rhoplot"], -1e30, 1e30), i4(int(opts["onlyneg"])),
+          f8(opts.get("rho_void", -1.0), opts["rhoparam"], opts["rhoparam2"])]
+    path.write_bytes(b"".join(b))
+
+
+def test_fortran_nci_host_fragments(ctx, tmp_path):
+    """The nciplot stand-in host (c2gpu_nci_host.f90) on the reference's
+    006_nciplot_fragment case through the c2gpu module: the same cubes and
+    masks as the reference, and bit for bit what the library gives the
+    Python host."""
+    if not os.path.exists(cb.FORTRAN_NCI_HOST):
+        pytest.skip("Fortran stand-in host not built (amdflang absent at build time)")
+    import json
+
+    from critic2_amd import nci as gn
+    from tests.helpers import GOLDEN, fmt6, urea
+    d, x2c, tables, xat, species = urea()
+    f = json.load(open(os.path.join(GOLDEN, "urea_frag.json")))
+    nb = gn.box_nodes(x2c, *f["cube_cryst"], nstep=f["nstep"])
+    frags = [gn.fragment_atoms(x2c, xat, species, np.array(fr)) for fr in f["fragments_angstrom"]]
+    opts = dict(f["options"], onlyneg=f["onlyneg"])
+    print("[case] fortran nci host 006_nciplot_fragment", flush=True)
+    _nci_host_input(tmp_path / "in.bin", tables, x2c, xat, species, nb, frags, opts)
+    r = subprocess.run([cb.FORTRAN_NCI_HOST, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
+    assert r.returncode == 0, r.stdout.decode(errors="replace")
+    raw = (tmp_path / "out.bin").read_bytes()
+    dlen = int(np.frombuffer(raw[:8], dtype=np.int64)[0])
+    text = raw[8:8 + dlen]
+    vals = np.frombuffer(raw[8 + dlen:], dtype=np.float64)
+    npts = int(np.prod(f["nstep"]))
+    crho, cg, rhoat = vals[:npts], vals[npts:2 * npts], vals[2 * npts:3 * npts]
+    rf = vals[3 * npts:3 * npts + len(frags) * npts].reshape(len(frags), npts)
+    assert [fmt6(v) for v in crho] == f["dens_cube"]
+    assert [fmt6(v) for v in rhoat] == f["pdens_cube"]
+    assert cg.tolist() == f["grad_cube"] and text == b""
+    c1, g1, a1 = gn.promol_nodes(ctx, nb, x2c, tables, xat, species, precision="fp64")
+    r1 = gn.fragments(ctx, nb, tables, frags, precision="fp64")
+    assert np.array_equal(crho, c1) and np.array_equal(rhoat, a1) and np.array_equal(rf, r1)
