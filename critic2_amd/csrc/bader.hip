@@ -207,13 +207,37 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   uint64_t *d_toff = buf<uint64_t>(ctx, "bader_toff", (size_t)N + 1);
   unsigned long long *d_top = buf<unsigned long long>(ctx, "bader_pooltop", 1);
   C2_HIP(hipMemsetAsync(d_len + N, 0, sizeof(uint32_t), st));
-  auto &poolbuf = ctx->bufs["bader_pool"];
+  // the walk pool lives in the arena that later holds the sweep's event
+  // lists (the walk is over once the CSR copy is made), so large cells do
+  // not hold both or allocate either anew on every call
+  auto &poolbuf = ctx->bufs["bader_arena"];
+  // the walk pool: 48 points per start, or up to 80 when the device can hold
+  // that pool and the CSR copy of it side by side (long ascents in large
+  // cells would otherwise need a second walk)
   uint64_t cap = (uint64_t)N * 48;
   if (poolbuf.ptr && poolbuf.bytes / 4 > cap) cap = poolbuf.bytes / 4;
+  {
+    // keep room for the sweep's per-point state (about 112 bytes a point)
+    // unless an earlier call left it allocated
+    auto lq = ctx->bufs.find("sweep_longq");
+    const bool have_sweep = lq != ctx->bufs.end() && lq->second.ptr && lq->second.bytes >= 4ull * N;
+    const uint64_t spare = (8ull << 30) + (have_sweep ? 0 : 112ull * N), most = (uint64_t)N * 80;
+    const uint64_t own = poolbuf.ptr ? poolbuf.bytes : 0;
+    auto tp = ctx->bufs.find("bader_tp");
+    const uint64_t tp_own = (tp != ctx->bufs.end() && tp->second.ptr) ? tp->second.bytes : 0;
+    size_t fr = 0, tot = 0;
+    C2_HIP(hipMemGetInfo(&fr, &tot));
+    // pool (4 B a point) now, and the CSR copy (at most as large) next to it
+    const uint64_t room = fr + own > spare ? fr + own - spare : 0;
+    uint64_t afford = (room + tp_own) / 8;
+    if (afford > room / 4) afford = room / 4;
+    const uint64_t want = afford < most ? afford : most;
+    if (want > cap) cap = want;
+  }
   uint64_t ntraj = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
-    uint32_t *pool = buf<uint32_t>(ctx, "bader_pool", cap);
-    cap = ctx->bufs["bader_pool"].bytes / 4;
+    uint32_t *pool = buf<uint32_t>(ctx, "bader_arena", cap);
+    cap = ctx->bufs["bader_arena"].bytes / 4;
     C2_HIP(hipMemsetAsync(d_top, 0, 8, st));
     C2_LAUNCH(ctx, k_traj, dim3(nth / B), dim3(B), 0, P, N, scratch, MAX_PATH, pool, (unsigned long long)cap,
               d_top, d_pos, d_len, d_err);
@@ -225,10 +249,8 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   exclusive_scan(ctx, "len", d_len, d_toff, (int64_t)N + 1);
   uint32_t *d_tp = buf<uint32_t>(ctx, "bader_tp", ntraj);
   C2_LAUNCH(ctx, k_traj_reorder, dim3(gN), dim3(B), 0, N, d_pos, d_toff,
-            (const uint32_t *)ctx->bufs["bader_pool"].ptr, d_tp);
+            (const uint32_t *)ctx->bufs["bader_arena"].ptr, d_tp);
   S.traj_points = (int64_t)ntraj;
-  // a large walk pool is not kept for the next call: the sweep's lists need the room
-  if (ctx->bufs["bader_pool"].bytes > (16ull << 30)) release(ctx, "bader_pool");
 
   // endpoints (grid maxima and no-move points) -> attractors, on the host
   uint8_t *d_isend = buf<uint8_t>(ctx, "bader_isend", N);
@@ -307,10 +329,6 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     h_lab.swap(lab);
     upload_labels();
     SR = sweep_run(ctx, P, N, d_toff, d_tp, d_maxlab, V, known);
-  }
-  {  // a large event pool is not kept either: refinement needs the room
-    auto it = ctx->bufs.find("sweep_pool");
-    if (it != ctx->bufs.end() && it->second.bytes > (16ull << 30)) release(ctx, "sweep_pool");
   }
   S.sweep_iters = SR.iters;
   S.events_total = SR.events_total;

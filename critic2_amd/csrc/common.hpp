@@ -99,19 +99,23 @@ inline void release(c2gpu_ctx *ctx, const std::string &name) {
 }
 
 // A buffer of at least min_bytes and at most want_bytes: as much of
-// want_bytes as the device's free memory allows, keeping `spare` bytes free
-// for later allocations.  The old contents are not kept.
+// want_bytes as the device's memory allows (the buffer's own bytes count as
+// available), keeping `spare` bytes free for later allocations.  A held
+// buffer at least that large is kept as it is; otherwise the old contents
+// are not kept.
 inline void *buffer_within(c2gpu_ctx *ctx, const std::string &name, size_t min_bytes, size_t want_bytes,
                            size_t spare) {
-  auto &b = ctx->bufs[name];
-  if (b.ptr && b.bytes >= want_bytes) return b.ptr;
-  release(ctx, name);
+  auto it = ctx->bufs.find(name);
+  const size_t own = (it != ctx->bufs.end() && it->second.ptr) ? it->second.bytes : 0;
+  if (own >= want_bytes) return it->second.ptr;
   size_t fr = 0, tot = 0;
   C2_HIP(hipMemGetInfo(&fr, &tot));
-  size_t bytes = want_bytes;
-  if (fr < spare + bytes) bytes = fr > spare + min_bytes ? fr - spare : min_bytes;
+  size_t bytes = fr + own > spare ? fr + own - spare : 0;
+  if (bytes > want_bytes) bytes = want_bytes;
   bytes &= ~(size_t)255;
   if (bytes < min_bytes) bytes = min_bytes;
+  if (own >= bytes) return it->second.ptr;  // already as large as it can get
+  release(ctx, name);
   return buffer(ctx, name, bytes);
 }
 

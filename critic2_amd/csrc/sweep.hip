@@ -919,8 +919,9 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     const uint64_t want = reserved + (reserved >> 2) + (uint64_t)N;
     if (want > pool_cap) {
       const uint64_t least = reserved + (reserved >> 5) + (uint64_t)N / 8;
-      L.ev = static_cast<uint64_t *>(buffer_within(ctx, "sweep_pool", 8 * least, 8 * want, 6ull << 30));
-      pool_cap = ctx->bufs["sweep_pool"].bytes / 8;
+      // the pool is the arena that held the walk (no longer needed)
+      L.ev = static_cast<uint64_t *>(buffer_within(ctx, "bader_arena", 8 * least, 8 * want, 6ull << 30));
+      pool_cap = ctx->bufs["bader_arena"].bytes / 8;
     }
     pool_top = reserved;
     C2_HIP(hipMemsetAsync(cur, 0, sizeof(uint32_t) * N, st));
