@@ -158,10 +158,15 @@ def _nodes(nstep, x0, xmat) -> _lib.NciNodes:
     return nd
 
 
-def cell_nodes(x2c, nstep) -> _lib.NciNodes:
+def cell_nodes(x2c, nstep=None, xinc=0.1) -> _lib.NciNodes:
     """The periodic cell's nodes (nci@proc.f90:413-426): x0 = 0,
-    xmat(:,c) = a_c / nstep(c).  x2c: lattice vectors as columns."""
+    xmat(:,c) = a_c / nstep(c), with nstep(c) = ceiling(|a_c| / xinc(c))
+    when not given (a grid field passes its own n).  x2c: lattice vectors as
+    columns (bohr); xinc in bohr (critic2's default 0.1)."""
     a = np.asarray(x2c, dtype=np.float64)
+    if nstep is None:
+        xi = np.broadcast_to(np.asarray(xinc, dtype=np.float64), (3,))
+        nstep = [int(np.ceil(np.sqrt(a[:, c] @ a[:, c]) / xi[c])) for c in range(3)]
     xm = np.stack([a[:, c] / float(nstep[c]) for c in range(3)], axis=1)
     return _nodes(nstep, (0.0, 0.0, 0.0), xm)
 

@@ -144,3 +144,14 @@ def test_oracle_fragment_cutoff_rule_hand_cases():
     dat, cgo, sums = oracle.nci_cutoffs(crho, cg, rhocut=0.2, dimcut=2.0, rhoplot=1.0, rhoat=rhoat, rhofrag=rf,
                                         rho_void=2.0)
     assert sums.tolist() == [0.04, 4.0, 4.0]
+
+
+def test_cell_nodes_from_xinc():
+    """NSTEP from the default increment: ceiling(|a_i| / 0.1 bohr) per axis,
+    and the step vectors a_i / nstep_i (nci@proc.f90:413-426)."""
+    x2c = np.array([[10.05, 5.0, 0.0], [0.0, 7.25, 0.0], [0.0, 0.0, 3.33]])  # |a| = 10.05, 8.807, 3.33
+    nd = gn.cell_nodes(x2c)
+    assert list(nd.nstep) == [101, 89, 34]
+    xm = np.array(list(nd.xmat)).reshape(3, 3).T
+    np.testing.assert_allclose(xm * np.array(list(nd.nstep))[None, :], x2c, rtol=1e-15)
+    assert list(gn.cell_nodes(x2c, xinc=0.5).nstep) == [21, 18, 7]
