@@ -67,13 +67,17 @@ __global__ void k_traj(BParams P, uint32_t N, uint32_t *scratch, int maxlen, uin
   }
 }
 
-// Pool -> CSR in start order (toff = exclusive scan of len).
+// Pool -> CSR in start order (toff = exclusive scan of len): REORDER_LANES
+// lanes per path, so each copy moves whole 64-byte runs.
+constexpr int REORDER_LANES = 16;
 __global__ void k_traj_reorder(uint32_t N, const uint64_t *__restrict__ pos, const uint64_t *__restrict__ toff,
                                const uint32_t *__restrict__ pool, uint32_t *__restrict__ tp) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t s = g / REORDER_LANES;
   if (s >= N) return;
+  const uint32_t r = (uint32_t)(g % REORDER_LANES);
   const uint64_t a = pos[s], b = toff[s], n = toff[s + 1] - b;
-  for (uint64_t q = 0; q < n; q++) tp[b + q] = pool[a + q];
+  for (uint64_t q = r; q < n; q += REORDER_LANES) tp[b + q] = pool[a + q];
 }
 
 __global__ void k_mark_ends(uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
@@ -248,7 +252,7 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   if (read_scalar(ctx, d_err) & 1) fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: ascent path longer than 2048 points");
   exclusive_scan(ctx, "len", d_len, d_toff, (int64_t)N + 1);
   uint32_t *d_tp = buf<uint32_t>(ctx, "bader_tp", ntraj);
-  C2_LAUNCH(ctx, k_traj_reorder, dim3(gN), dim3(B), 0, N, d_pos, d_toff,
+  C2_LAUNCH(ctx, k_traj_reorder, dim3(grid_for((int64_t)N * REORDER_LANES, B)), dim3(B), 0, N, d_pos, d_toff,
             (const uint32_t *)ctx->bufs["bader_arena"].ptr, d_tp);
   S.traj_points = (int64_t)ntraj;
 
