@@ -192,14 +192,47 @@ struct PathFilter {
   }
 };
 
-template <class Stop, class Visit>
-__device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path, int maxlen,
-                                Stop stop, Visit visit) {
+// Path storage of walk_path: set(q, v) appends entry q, get(q, len) reads
+// entry q < len.  FlatPath is a plain array.
+struct FlatPath {
+  uint32_t *p;
+  __device__ __forceinline__ void set(int q, uint32_t v) const { p[q] = v; }
+  __device__ __forceinline__ uint32_t get(int q, int) const { return p[q]; }
+};
+
+// The newest entries in an LDS ring of RING words per thread (interleaved
+// over the workgroup's threads), written to the thread's scratch a whole
+// ring (64 bytes) at a time: the scratch lines are written once and whole
+// instead of a word per step.  Entries below len rounded down to RING are in
+// the scratch, the others in the ring.
+constexpr int RING = 16;
+struct RingPath {
+  uint32_t *ring;   // this thread's slot 0; slot j at ring[j * stride]
+  uint32_t *spill;  // this thread's scratch (16-word aligned)
+  uint32_t stride;
+  __device__ __forceinline__ void set(int q, uint32_t v) const {
+    ring[(q & (RING - 1)) * stride] = v;
+    if ((q & (RING - 1)) == RING - 1) {
+      uint4 *d = reinterpret_cast<uint4 *>(spill + (q - (RING - 1)));
+#pragma unroll
+      for (int u = 0; u < RING / 4; u++)
+        d[u] = make_uint4(ring[(4 * u) * stride], ring[(4 * u + 1) * stride], ring[(4 * u + 2) * stride],
+                          ring[(4 * u + 3) * stride]);
+    }
+  }
+  __device__ __forceinline__ uint32_t get(int q, int len) const {
+    return q < (len & ~(RING - 1)) ? spill[q] : ring[(q & (RING - 1)) * stride];
+  }
+};
+
+template <class Stop, class Visit, class Path = FlatPath>
+__device__ inline int walk_path(const BParams &P, uint32_t start, Path path, int maxlen, Stop stop,
+                                Visit visit) {
   int p[3];
   coords(P, start, p[0], p[1], p[2]);
   double dr[3] = {0.0, 0.0, 0.0};
   int len = 1;
-  path[0] = start;
+  path.set(0, start);
   PathFilter filt;
   filt.add(start);
   visit(start);
@@ -210,7 +243,7 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path
     bool seen = false;
     if (filt.maybe(lm))
       for (int q = len - 1; q >= 0; q--)
-        if (path[q] == lm) {
+        if (path.get(q, len) == lm) {
           seen = true;
           break;
         }
@@ -224,7 +257,7 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, uint32_t *path
     }
     if (pm[0] == p[0] && pm[1] == p[1] && pm[2] == p[2]) break;  // no move
     if (len >= maxlen) return -1;
-    path[len++] = lm;
+    path.set(len++, lm);
     filt.add(lm);
     visit(lm);
     p[0] = pm[0];

@@ -167,7 +167,7 @@ __global__ void k_reassign(BParams P, uint32_t N, const int32_t *__restrict__ V,
       continue;
     }
     int L = walk_path(
-        P, s, path, maxlen, [&](uint32_t x) { return known[x] == 2; },
+        P, s, FlatPath{path}, maxlen, [&](uint32_t x) { return known[x] == 2; },
         [&](uint32_t x) {
           if (known[x] != 2) known[x] = 0;
         });
