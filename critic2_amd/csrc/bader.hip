@@ -74,6 +74,16 @@ __global__ void __launch_bounds__(TRAJ_B) k_traj(BParams P, uint32_t N, uint32_t
   }
 }
 
+// The step direction table of BParams::G, one record per point.
+__global__ void k_step_dirs(BParams P, uint32_t N, double4 *__restrict__ G) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int p[3];
+  coords(P, s, p[0], p[1], p[2]);
+  double g[3];
+  G[s] = step_dir(P, p, g) ? make_double4(g[0], g[1], g[2], 0.0) : make_double4(0.0, 0.0, 0.0, 1.0);
+}
+
 // Pool -> CSR in start order (toff = exclusive scan of len): REORDER_LANES
 // lanes per path, so each copy moves whole 64-byte runs.
 constexpr int REORDER_LANES = 16;
@@ -165,6 +175,7 @@ static BParams make_params(const double *d_rho, const int n[3], const double car
   P.inv3 = 1.0 / n[2];
   for (int q = 0; q < 9; q++) P.c[q] = car2lat[q];
   for (int q = 0; q < 27; q++) P.lid[q] = lat_i_dist[q];
+  P.G = nullptr;
   return P;
 }
 
@@ -203,7 +214,7 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     C2_HIP(hipMemcpyAsync(d, rho, sizeof(double) * N, hipMemcpyHostToDevice, st));
     d_rho = d;
   }
-  const BParams P = make_params(d_rho, n, car2lat, lat_i_dist);
+  BParams P = make_params(d_rho, n, car2lat, lat_i_dist);
   const int B = 256;
   const unsigned gN = grid_for(N, B);
   int *d_err = buf<int>(ctx, "bader_err", 1);
@@ -211,6 +222,12 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
 
   // 1) free trajectories: one walk into a pool, then CSR in start order
   Timer ttraj(st);
+  {  // every point's step direction, once, for the walks of this call
+    double4 *G = buf<double4>(ctx, "bader_gdir", N);
+    C2_LAUNCH(ctx, k_step_dirs, dim3(gN), dim3(B), 0, P, N, G);
+    add_kernel_bytes(ctx, "k_step_dirs", 40.0 * (double)N);
+    P.G = G;
+  }
   const int nth = persistent_threads(ctx->device, B);
   uint32_t *scratch = buf<uint32_t>(ctx, "bader_scratch", (size_t)nth * MAX_PATH);
   uint32_t *d_len = buf<uint32_t>(ctx, "bader_len", (size_t)N + 1);

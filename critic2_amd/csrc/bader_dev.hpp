@@ -15,6 +15,9 @@ struct BParams {
   double inv1, inv2, inv3;  // 1/n1, 1/n2, 1/n3 (index arithmetic)
   double c[9];    // car2lat, column-major
   double lid[27]; // lat_i_dist
+  // when set: each point's normalised step direction (x, y, z; w = 1 where
+  // max|g| < 1e-30), so a step reads one record instead of seven values
+  const double4 *__restrict__ G;
 };
 
 // x / d and x % d for x < 2^32 from the FP64 reciprocal: the product is
@@ -139,14 +142,34 @@ __device__ __forceinline__ int fnint(double x) { return (int)round(x); }
 // known(pm)==1 test, which needs the caller's path.  Returns false when the
 // is_max early return fires (no move, known(p) untouched); otherwise pm
 // holds the wrapped candidate target.
-__device__ inline bool step_near_candidate(const BParams &P, const int p[3], double dr[3],
-                                           int pm[3]) {
-  double g[3];
+// g of step_neargrid after g = g / max|g| (bader@proc.f90:466-478); false
+// when max|g| < 1e-30.
+__device__ __forceinline__ bool step_dir(const BParams &P, const int p[3], double g[3]) {
   grad_dir(P, p, g);
   double gm = fabs(g[0]);
   if (fabs(g[1]) > gm) gm = fabs(g[1]);
   if (fabs(g[2]) > gm) gm = fabs(g[2]);
-  if (gm < 1e-30) {
+  if (gm < 1e-30) return false;
+  const double coeff = 1.0 / gm;
+#pragma unroll
+  for (int c = 0; c < 3; c++) g[c] = coeff * g[c];
+  return true;
+}
+
+__device__ inline bool step_near_candidate(const BParams &P, const int p[3], double dr[3],
+                                           int pm[3]) {
+  double g[3];
+  bool moving;
+  if (P.G) {
+    const double4 t = P.G[lin3(P, p[0], p[1], p[2])];
+    g[0] = t.x;
+    g[1] = t.y;
+    g[2] = t.z;
+    moving = t.w == 0.0;
+  } else {
+    moving = step_dir(P, p, g);
+  }
+  if (!moving) {
     dr[0] = dr[1] = dr[2] = 0.0;
     if (is_max(P, p[0], p[1], p[2])) return false;
     pm[0] = p[0];
@@ -155,10 +178,8 @@ __device__ inline bool step_near_candidate(const BParams &P, const int p[3], dou
     step_ongrid(P, pm);
     return true;
   }
-  const double coeff = 1.0 / gm;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    g[c] = coeff * g[c];
     const int ng = fnint(g[c]);
     int q = p[c] + ng;
     dr[c] = (dr[c] + g[c]) - (double)ng;
