@@ -189,3 +189,34 @@ def test_bader_256_orthorhombic_64_atoms(ctx):
     c_gpu, s_gpu = gb.basin_sums(ctx, bas.idg, bas.nattr, [f])
     np.testing.assert_array_equal(c_gpu, c_ref)
     np.testing.assert_allclose(s_gpu, s_ref, rtol=1e-10)
+
+
+EDGE = [  # name, seed, cell edges (A), angles, atoms, grid, quantisation digits (plateaus)
+    ("tiny-3x5x7", 31, (4.0, 5.0, 6.0), (90, 90, 90), 2, (3, 5, 7), None),
+    ("odd-17x3x11", 32, (7.0, 4.0, 6.0), (80, 100, 95), 3, (17, 3, 11), None),
+    ("cube-5", 33, (5.0, 5.0, 5.0), (90, 90, 90), 1, (5, 5, 5), None),
+    ("prime-13x19x23", 34, (7.0, 8.0, 9.0), (85, 92, 101), 4, (13, 19, 23), None),
+    ("plateaus-3digits-24x26x28", 35, (8.0, 8.5, 9.0), (90, 90, 90), 5, (24, 26, 28), 3),
+    ("plateaus-2digits-20x22x24", 36, (8.0, 8.5, 9.0), (90, 90, 90), 4, (20, 22, 24), 2),
+    ("plateaus-4digits-30^3", 37, (9.0, 9.0, 9.0), (70, 80, 90), 6, (30, 30, 30), 4),
+]
+
+
+@pytest.mark.parametrize("name,seed,edge,ang,nat,n,q", EDGE, ids=[e[0] for e in EDGE])
+def test_bader_edge_grids_bit_exact(ctx, name, seed, edge, ang, nat, n, q):
+    """The smallest grids (3 points on an axis), odd and prime sizes, and
+    grids rounded to q significant digits of their maximum, whose plateaus
+    exercise the strict-> tie rules (first maximum of step_ongrid, plateau
+    maxima of is_max, zeroed gradient components)."""
+    cell = random_cell(seed, edge, ang, nat, (1, 6, 7, 8), n)
+    print(f"[case] {name} n={cell.n}", flush=True)
+    f = _grid(ctx, cell)
+    if q:
+        mx = f.max()
+        f = np.round(f / mx * 10 ** q) / 10 ** q * mx
+    ref, info, xattr, _ = oracle.bader(f, cell.n, cell.x2c, cell.xat)
+    bas = gb.bader_integrate(ctx, f, cell.n, cell.x2c, cell.xat)
+    assert int(np.count_nonzero(bas.idg != ref)) == 0
+    assert bas.nattr == info["nattr"]
+    assert bas.stats["refine_iters"] == info["refine_iters"]
+    np.testing.assert_array_equal(bas.xattr, xattr)
