@@ -236,88 +236,130 @@ __device__ __forceinline__ int yt_fracs(const YtGeo &G, uint32_t p, const double
   return nh;
 }
 
-// Weights of surface point p (slot t) from its higher neighbours', all
-// final, accumulated in registers; entries sorted by basin.  Returns false
-// (nothing written) when more than KW basins meet at p.
-__device__ bool yt_point_weights(const YtGeo &G, uint32_t p, uint32_t t, const double *__restrict__ g,
-                                 const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
-                                 const int32_t *__restrict__ slot, WBuf W, int *err) {
-  uint32_t hq[14];
-  double fr[14];
-  const int nh = yt_fracs(G, p, g, hmask, hq, fr);
-  uint32_t bb[KW];
-  double ww[KW];
-  int na = 0;
-  bool over = false;
-  auto add = [&](uint32_t basin, double x) {
-    for (int e = 0; e < na; e++)
-      if (bb[e] == basin) {
-        ww[e] = ww[e] + x;
-        return;
-      }
-    if (na < KW) {
-      bb[na] = basin;
-      ww[na] = 0.0 + x;
-      na++;
-    } else {
-      over = true;
-    }
-  };
-  for (int a = 0; a < nh && !over; a++) {
-    const uint32_t jn = hq[a];
-    const int32_t lj = lab[jn];
-    if (lj != 0) {
-      add((uint32_t)lj, fr[a] * 1.0);
-    } else {
-      const int32_t s = slot[jn];
-      const int ce = W.c[s];
-      const uint64_t so = W.off[s];
-      for (int e = 0; e < ce; e++) add(W.b[so + e], fr[a] * W.w[so + e]);
-    }
-  }
-  if (over) return false;
-  for (int a = 1; a < na; a++) {
-    const uint32_t kb = bb[a];
-    const double kw = ww[a];
-    int b = a;
-    while (b > 0 && bb[b - 1] > kb) {
-      bb[b] = bb[b - 1];
-      ww[b] = ww[b - 1];
-      b--;
-    }
-    bb[b] = kb;
-    ww[b] = kw;
-  }
-  const uint64_t o = atomicAdd(W.top, (unsigned long long)na);
-  if (o + (uint64_t)na > W.cap) {  // pool full: the host enlarges it and runs the weights again
-    atomicOr(err, 4);
-    W.c[t] = 0;
-    W.off[t] = 0;
-    return true;
-  }
-  W.c[t] = (uint16_t)na;
-  W.off[t] = o;
-  for (int e = 0; e < na; e++) {
-    W.b[o + e] = bb[e];
-    W.w[o + e] = ww[e];
-  }
-  return true;
-}
-
-// The frontier's weights, one thread per point; points where more than KW
-// basins meet go to the wide list.
+// The frontier's weights, GL lanes per surface point p.  Lane a takes p's
+// a-th higher neighbour in facet order: its fraction chi_a / max(sum chi,
+// vsmall) (the sum formed in facet order, as yt_fracs) and its rank in
+// decreasing rho (stable, as yt_fracs orders them).  The neighbours' weight
+// lists (one basin for an interior neighbour, else its sorted entries) are
+// merged basin by basin in increasing order; each basin's weight adds the
+// neighbours' contributions in rank order, a neighbour without the basin
+// adding +0.0, which leaves the sum's bits as yt_weights forms them.  Entry
+// e lives in lane e % GL until written; points where more than KW basins
+// meet go to the wide list.  No per-lane arrays: nothing spills to scratch.
+constexpr int GL = 16;
 __global__ void __launch_bounds__(YB) k_yt_w_narrow(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
                                                      const double *__restrict__ g, const uint16_t *__restrict__ hmask,
                                                      const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
                                                      WBuf W, uint32_t *__restrict__ wide, uint32_t *nwide, int *err) {
-  const uint32_t t = blockIdx.x * YB + threadIdx.x;
-  bool w = false;
-  uint32_t p = 0;
-  if (t < nf) {
-    p = front[t];
-    w = !yt_point_weights(G, p, (uint32_t)slot[p], g, hmask, lab, slot, W, err);
+  const uint32_t gi = (blockIdx.x * YB + threadIdx.x) / GL;  // the group's point in the frontier
+  const int r = (int)(threadIdx.x % GL);
+  if (gi >= nf) return;  // whole groups leave together
+  const uint32_t p = front[gi];
+  int i, j, k;
+  coords3(G, p, i, j, k);
+  const double gp = g[p];
+  const uint32_t mask = hmask[p];
+  const int nh = __popc(mask);
+  // this lane's neighbour: the r-th set bit of the mask
+  int q = -1;
+  for (int u = 0, c = 0; u < G.nvec; u++)
+    if (mask & (1u << u)) {
+      if (c == r) q = u;
+      c++;
+    }
+  uint32_t jn = 0;
+  double gj = 0.0, chi = 0.0;
+  if (q >= 0) {
+    jn = nbr(G, i, j, k, q);
+    gj = g[jn];
+    chi = fmax(G.al[q] * (gj - gp), VSMALL);
   }
-  yt_append(w, p, wide, nwide);
+  double csum = 0.0;
+  for (int u = 0; u < nh; u++) csum = csum + __shfl(chi, u, GL);
+  const double fr = chi / fmax(csum, VSMALL);
+  int rank = 0;
+  for (int u = 0; u < nh; u++) {
+    const double gu = __shfl(gj, u, GL);
+    rank += (gu > gj || (gu == gj && u < r)) ? 1 : 0;
+  }
+  // the lane of each rank, four bits each
+  uint64_t src = 0;
+  for (int kk = 0; kk < nh; kk++) {
+    int who = (q >= 0 && rank == kk) ? r : 0;
+    for (int o = GL / 2; o > 0; o >>= 1) who = max(who, __shfl_xor(who, o, GL));
+    src |= (uint64_t)who << (4 * kk);
+  }
+  // this lane's list: one basin (interior neighbour) or its weight entries
+  const int32_t lj = q >= 0 ? lab[jn] : 0;
+  int ce = 0;
+  uint64_t so = 0;
+  if (q >= 0 && lj == 0) {
+    const int32_t s = slot[jn];
+    ce = W.c[s];
+    so = W.off[s];
+  }
+  const int len = q < 0 ? 0 : (lj != 0 ? 1 : ce);
+  int head = 0;
+  uint32_t hb = head < len ? (lj != 0 ? (uint32_t)lj : W.b[so]) : 0xffffffffu;
+  int nb = 0;
+  bool over = false;
+  uint32_t eb0 = 0, eb1 = 0;
+  double ew0 = 0.0, ew1 = 0.0;
+  for (;;) {
+    uint32_t m = hb;
+    for (int o = GL / 2; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, GL));
+    if (m == 0xffffffffu) break;  // every list consumed
+    if (nb == KW) {
+      over = true;
+      break;
+    }
+    const bool has = hb == m;
+    const double x = has ? fr * (lj != 0 ? 1.0 : W.w[so + head]) : 0.0;
+    double acc = 0.0;
+    for (int kk = 0; kk < nh; kk++) acc = acc + __shfl(x, (int)((src >> (4 * kk)) & 15u), GL);
+    if (r == nb % GL) {
+      if (nb < GL) {
+        eb0 = m;
+        ew0 = acc;
+      } else {
+        eb1 = m;
+        ew1 = acc;
+      }
+    }
+    nb++;
+    if (has) {
+      head++;
+      hb = head < len ? W.b[so + head] : 0xffffffffu;  // an interior neighbour has one entry
+    }
+  }
+  const uint32_t t = (uint32_t)slot[p];
+  if (over) {
+    if (r == 0) wide[atomicAdd(nwide, 1u)] = p;
+    return;
+  }
+  unsigned long long o = 0;
+  if (r == 0) o = atomicAdd(W.top, (unsigned long long)nb);
+  o = __shfl(o, 0, GL);
+  if (o + (unsigned long long)nb > W.cap) {  // pool full: the host enlarges it and runs the weights again
+    if (r == 0) {
+      atomicOr(err, 4);
+      W.c[t] = 0;
+      W.off[t] = 0;
+    }
+    return;
+  }
+  if (r == 0) {
+    W.c[t] = (uint16_t)nb;
+    W.off[t] = o;
+  }
+  if (r < nb) {
+    W.b[o + r] = eb0;
+    W.w[o + r] = ew0;
+  }
+  if (r + GL < nb) {
+    W.b[o + r + GL] = eb1;
+    W.w[o + r + GL] = ew1;
+  }
 }
 
 // Wide points, one workgroup each: a dense accumulator per basin in LDS; the
@@ -630,8 +672,8 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
       while (wf) {
         uint32_t *cf = fr[wpasses & 1], *nx = fr[(wpasses & 1) ^ 1];
         C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
-        C2_LAUNCH(ctx, k_yt_w_narrow, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, cf, g, hmask, lab, slot, W, wide,
-                  d_cnt, d_err);
+        C2_LAUNCH(ctx, k_yt_w_narrow, dim3(grid_for((int64_t)wf * GL, YB)), dim3(YB), 0, G, wf, cf, g, hmask, lab,
+                  slot, W, wide, d_cnt, d_err);
         // each point: its and its higher neighbours' rho, masks, labels and slots
         add_kernel_bytes(ctx, "k_yt_w_narrow", (double)wf * (4.0 + 2.0 + 8.0 * (G.nvec + 1) + 8.0 * G.nvec));
         C2_LAUNCH(ctx, k_yt_w_step, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, cf, g, lab, pend, nx, d_cnt + 1);
