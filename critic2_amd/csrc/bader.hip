@@ -89,12 +89,15 @@ __global__ void k_step_dirs(BParams P, uint32_t N, double4 *__restrict__ G) {
 constexpr int REORDER_LANES = 16;
 __global__ void k_traj_reorder(uint32_t N, const uint64_t *__restrict__ pos, const uint64_t *__restrict__ toff,
                                const uint32_t *__restrict__ pool, uint32_t *__restrict__ tp) {
-  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t s = g / REORDER_LANES;
-  if (s >= N) return;
-  const uint32_t r = (uint32_t)(g % REORDER_LANES);
-  const uint64_t a = pos[s], b = toff[s], n = toff[s + 1] - b;
-  for (uint64_t q = r; q < n; q += REORDER_LANES) tp[b + q] = pool[a + q];
+  // grid-stride: N * REORDER_LANES lanes exceed one dispatch's 2^32 work-items on large grids
+  const uint64_t nlanes = (uint64_t)N * REORDER_LANES;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nlanes;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = g / REORDER_LANES;
+    const uint32_t r = (uint32_t)(g % REORDER_LANES);
+    const uint64_t a = pos[s], b = toff[s], n = toff[s + 1] - b;
+    for (uint64_t q = r; q < n; q += REORDER_LANES) tp[b + q] = pool[a + q];
+  }
 }
 
 __global__ void k_mark_ends(uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
@@ -276,9 +279,16 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   if (read_scalar(ctx, d_err) & 1) fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: ascent path longer than 2048 points");
   exclusive_scan(ctx, "len", d_len, d_toff, (int64_t)N + 1);
   uint32_t *d_tp = buf<uint32_t>(ctx, "bader_tp", ntraj);
-  C2_LAUNCH(ctx, k_traj_reorder, dim3(grid_for((int64_t)N * REORDER_LANES, B)), dim3(B), 0, N, d_pos, d_toff,
+  const unsigned gR = (unsigned)std::min<int64_t>(((int64_t)N * REORDER_LANES + B - 1) / B, int64_t(1) << 20);
+  C2_LAUNCH(ctx, k_traj_reorder, dim3(gR), dim3(B), 0, N, d_pos, d_toff,
             (const uint32_t *)ctx->bufs["bader_arena"].ptr, d_tp);
   S.traj_points = (int64_t)ntraj;
+  if (ctx->bufs["bader_gdir"].bytes > (8ull << 30)) {
+    // a large direction table leaves the event lists short of room: the
+    // refinement's few walks compute their directions in place instead
+    release(ctx, "bader_gdir");
+    P.G = nullptr;
+  }
 
   // endpoints (grid maxima and no-move points) -> attractors, on the host
   uint8_t *d_isend = buf<uint8_t>(ctx, "bader_isend", N);

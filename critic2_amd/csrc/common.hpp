@@ -206,10 +206,14 @@ inline void require_device_ptr(const void *p, const char *what) {
     fail(C2GPU_ERR_ARG, std::string(what) + " is not device memory (C2GPU_PTR_DEVICE set)");
 }
 
+// Workgroups of `block` threads covering n work-items.  One dispatch holds
+// fewer than 2^32 work-items (the grid size is a 32-bit field of the
+// dispatch packet); larger launches must loop over a bounded grid.
 inline unsigned grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
-  if (g > 0x7fffffff) g = 0x7fffffff;
+  if (g * (int64_t)block >= (int64_t(1) << 32))
+    fail(C2GPU_ERR_ARG, "launch of " + std::to_string(n) + " work-items exceeds one dispatch (2^32)");
   return (unsigned)g;
 }
 
