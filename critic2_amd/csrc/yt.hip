@@ -145,27 +145,33 @@ __device__ __forceinline__ uint64_t yt_lohi_of(const YtGeo &G, uint32_t p, const
 // One frontier of the labels: every point of `front` is final; each lower
 // neighbour loses one pending higher neighbour, and the thread that takes
 // the last computes it and puts it on the next frontier.
-__global__ void __launch_bounds__(YB) k_yt_lab_step(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
+// (The frontier's size is read from nfp, written by the step before, so the
+// host need not read it back between frontiers; a grid-stride loop covers
+// it, and an empty frontier makes the launch a no-op.)
+__global__ void __launch_bounds__(YB) k_yt_lab_step(YtGeo G, const uint32_t *nfp, const uint32_t *__restrict__ front,
                                                      const double *__restrict__ g,
                                                      const uint16_t *__restrict__ hmask, uint32_t *__restrict__ pend,
                                                      uint64_t *lohi, uint32_t *__restrict__ next, uint32_t *nnext) {
-  const uint32_t t = blockIdx.x * YB + threadIdx.x;
-  uint32_t done[14];
-  int nd = 0;
-  if (t < nf) {
-    const uint32_t p = front[t];
-    int i, j, k;
-    coords3(G, p, i, j, k);
-    const double gp = g[p];
-    for (int q = 0; q < G.nvec; q++) {
-      const uint32_t jn = nbr(G, i, j, k, q);
-      if (g[jn] < gp && atomicSub(&pend[jn], 1u) == 1u) {
-        lohi[jn] = yt_lohi_of(G, jn, hmask, lohi);
-        done[nd++] = jn;
+  const uint32_t nf = *nfp;
+  for (uint32_t b0 = blockIdx.x * YB; b0 < nf; b0 += gridDim.x * YB) {  // uniform per workgroup
+    const uint32_t t = b0 + threadIdx.x;
+    uint32_t done[14];
+    int nd = 0;
+    if (t < nf) {
+      const uint32_t p = front[t];
+      int i, j, k;
+      coords3(G, p, i, j, k);
+      const double gp = g[p];
+      for (int q = 0; q < G.nvec; q++) {
+        const uint32_t jn = nbr(G, i, j, k, q);
+        if (g[jn] < gp && atomicSub(&pend[jn], 1u) == 1u) {
+          lohi[jn] = yt_lohi_of(G, jn, hmask, lohi);
+          done[nd++] = jn;
+        }
       }
     }
+    for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
   }
-  for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
 }
 
 __global__ void k_yt_label(uint32_t N, const uint64_t *__restrict__ lohi, int32_t *__restrict__ lab,
@@ -247,14 +253,11 @@ __device__ __forceinline__ int yt_fracs(const YtGeo &G, uint32_t p, const double
 // e lives in lane e % GL until written; points where more than KW basins
 // meet go to the wide list.  No per-lane arrays: nothing spills to scratch.
 constexpr int GL = 16;
-__global__ void __launch_bounds__(YB) k_yt_w_narrow(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
-                                                     const double *__restrict__ g, const uint16_t *__restrict__ hmask,
-                                                     const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
-                                                     WBuf W, uint32_t *__restrict__ wide, uint32_t *nwide, int *err) {
-  const uint32_t gi = (blockIdx.x * YB + threadIdx.x) / GL;  // the group's point in the frontier
-  const int r = (int)(threadIdx.x % GL);
-  if (gi >= nf) return;  // whole groups leave together
-  const uint32_t p = front[gi];
+__device__ __forceinline__ void yt_narrow_point(const YtGeo &G, uint32_t p, int r, const double *__restrict__ g,
+                                                const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
+                                                const int32_t *__restrict__ slot, const WBuf &W,
+                                                uint32_t *__restrict__ wide, uint32_t *nwide,
+                                                unsigned long long *widetot, int *err) {
   int i, j, k;
   coords3(G, p, i, j, k);
   const double gp = g[p];
@@ -334,7 +337,10 @@ __global__ void __launch_bounds__(YB) k_yt_w_narrow(YtGeo G, uint32_t nf, const 
   }
   const uint32_t t = (uint32_t)slot[p];
   if (over) {
-    if (r == 0) wide[atomicAdd(nwide, 1u)] = p;
+    if (r == 0) {
+      wide[atomicAdd(nwide, 1u)] = p;
+      atomicAdd(widetot, 1ull);
+    }
     return;
   }
   unsigned long long o = 0;
@@ -362,22 +368,29 @@ __global__ void __launch_bounds__(YB) k_yt_w_narrow(YtGeo G, uint32_t nf, const 
   }
 }
 
-// Wide points, one workgroup each: a dense accumulator per basin in LDS; the
-// neighbours are added one after the other in decreasing rho (a neighbour's
-// entries have distinct basins, so its additions do not collide), then the
-// basins present are written in increasing order.
-__global__ void __launch_bounds__(YB) k_yt_w_wide(YtGeo G, const uint32_t *__restrict__ wide, int nattr,
-                                                   const double *__restrict__ g, const uint16_t *__restrict__ hmask,
-                                                   const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
-                                                   WBuf W, int *err) {
-  extern __shared__ double acc[];  // nattr doubles, then nattr flags
-  uint8_t *pres = reinterpret_cast<uint8_t *>(acc + nattr);
-  __shared__ uint32_t hq[14];
-  __shared__ double fr[14];
-  __shared__ int nh;
-  __shared__ uint32_t wsum[YB / 64];
-  __shared__ unsigned long long base;
-  const uint32_t p = wide[blockIdx.x];
+// The frontier (its size read from nfp) in GL-lane groups, grid-stride.
+__global__ void __launch_bounds__(YB) k_yt_w_narrow(YtGeo G, const uint32_t *nfp, const uint32_t *__restrict__ front,
+                                                     const double *__restrict__ g, const uint16_t *__restrict__ hmask,
+                                                     const int32_t *__restrict__ lab, const int32_t *__restrict__ slot,
+                                                     WBuf W, uint32_t *__restrict__ wide, uint32_t *nwide,
+                                                     unsigned long long *widetot, int *err) {
+  const uint64_t nl = (uint64_t)*nfp * GL;
+  for (uint64_t b0 = (uint64_t)blockIdx.x * YB; b0 < nl; b0 += (uint64_t)gridDim.x * YB) {
+    const uint64_t l = b0 + threadIdx.x;
+    if (l < nl)  // whole groups: GL divides YB
+      yt_narrow_point(G, front[l / GL], (int)(l % GL), g, hmask, lab, slot, W, wide, nwide, widetot, err);
+  }
+}
+
+// One wide point p by the whole workgroup: a dense accumulator per basin in
+// LDS (acc, pres); the neighbours are added one after the other in
+// decreasing rho (a neighbour's entries have distinct basins, so its
+// additions do not collide), then the basins present are written in
+// increasing order.  Every thread calls it.
+__device__ void yt_wide_point(const YtGeo &G, uint32_t p, int nattr, const double *__restrict__ g,
+                              const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
+                              const int32_t *__restrict__ slot, const WBuf &W, int *err, double *acc, uint8_t *pres,
+                              uint32_t *hq, double *fr, int &nh, uint32_t *wsum, unsigned long long &base) {
   for (int a = threadIdx.x; a < nattr; a += YB) {
     acc[a] = 0.0;
     pres[a] = 0;
@@ -437,6 +450,26 @@ __global__ void __launch_bounds__(YB) k_yt_w_wide(YtGeo G, const uint32_t *__res
   }
 }
 
+// The frontier's wide points (their count read from nwp), each by one
+// workgroup (yt_wide_point), the workgroups striding over the list.
+__global__ void __launch_bounds__(YB) k_yt_w_wide(YtGeo G, const uint32_t *__restrict__ wide, const uint32_t *nwp,
+                                                   int nattr, const double *__restrict__ g,
+                                                   const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
+                                                   const int32_t *__restrict__ slot, WBuf W, int *err) {
+  extern __shared__ double acc[];  // nattr doubles, then nattr flags
+  uint8_t *pres = reinterpret_cast<uint8_t *>(acc + nattr);
+  __shared__ uint32_t hq[14];
+  __shared__ double fr[14];
+  __shared__ int nh;
+  __shared__ uint32_t wsum[YB / 64];
+  __shared__ unsigned long long base;
+  const uint32_t nw = *nwp;
+  for (uint32_t qi = blockIdx.x; qi < nw; qi += gridDim.x) {  // one wide point per workgroup at a time
+    yt_wide_point(G, wide[qi], nattr, g, hmask, lab, slot, W, err, acc, pres, hq, fr, nh, wsum, base);
+    __syncthreads();
+  }
+}
+
 // Pending surface neighbours above each surface point; those with none
 // form the first frontier.
 __global__ void __launch_bounds__(YB) k_yt_w_init(YtGeo G, uint32_t nias, const uint32_t *__restrict__ ias,
@@ -464,24 +497,27 @@ __global__ void __launch_bounds__(YB) k_yt_w_init(YtGeo G, uint32_t nias, const 
 
 // The next frontier: each surface point below a frontier point loses one
 // pending neighbour; the thread that takes the last lists it.
-__global__ void __launch_bounds__(YB) k_yt_w_step(YtGeo G, uint32_t nf, const uint32_t *__restrict__ front,
+__global__ void __launch_bounds__(YB) k_yt_w_step(YtGeo G, const uint32_t *nfp, const uint32_t *__restrict__ front,
                                                    const double *__restrict__ g, const int32_t *__restrict__ lab,
                                                    uint32_t *__restrict__ pend, uint32_t *__restrict__ next,
                                                    uint32_t *nnext) {
-  const uint32_t t = blockIdx.x * YB + threadIdx.x;
-  uint32_t done[14];
-  int nd = 0;
-  if (t < nf) {
-    const uint32_t p = front[t];
-    int i, j, k;
-    coords3(G, p, i, j, k);
-    const double gp = g[p];
-    for (int q = 0; q < G.nvec; q++) {
-      const uint32_t jn = nbr(G, i, j, k, q);
-      if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
+  const uint32_t nf = *nfp;
+  for (uint32_t b0 = blockIdx.x * YB; b0 < nf; b0 += gridDim.x * YB) {  // uniform per workgroup
+    const uint32_t t = b0 + threadIdx.x;
+    uint32_t done[14];
+    int nd = 0;
+    if (t < nf) {
+      const uint32_t p = front[t];
+      int i, j, k;
+      coords3(G, p, i, j, k);
+      const double gp = g[p];
+      for (int q = 0; q < G.nvec; q++) {
+        const uint32_t jn = nbr(G, i, j, k, q);
+        if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
+      }
     }
+    for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
   }
-  for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
 }
 
 struct YtFields {
@@ -514,6 +550,45 @@ __global__ void __launch_bounds__(YB) k_yt_surface_sums(uint32_t nias, const uin
   for (int q = threadIdx.x; q < nb; q += YB)
     if (bins[q] != 0.0) atomicAdd(&acc[q], bins[q]);
 }
+
+// Frontier sizes kept on the device: at(p) holds the size of frontier p
+// (the step over frontier p counts frontier p + 1 there), so the host reads
+// a size only every FRONT_BATCH frontiers; steps past the last frontier see
+// size 0 and do nothing.  reserve(n) makes indices below n exist (zeroed);
+// finish() reads the sizes back and returns the number of non-empty
+// frontiers and their sum.
+constexpr int FRONT_BATCH = 32;
+struct FrontCounts {
+  c2gpu_ctx *ctx;
+  std::string name;
+  uint32_t *d = nullptr;
+  size_t cap = 0;
+  FrontCounts(c2gpu_ctx *c, const char *nm) : ctx(c), name(nm) { reserve(4096); }
+  void reserve(size_t n) {
+    if (n <= cap) return;
+    size_t nc = cap ? cap : 4096;
+    while (nc < n) nc *= 2;
+    uint32_t *nd = buf<uint32_t>(ctx, name + "_" + std::to_string(nc), nc);
+    C2_HIP(hipMemsetAsync(nd, 0, 4 * nc, ctx->stream));
+    if (cap) C2_HIP(hipMemcpyAsync(nd, d, 4 * cap, hipMemcpyDeviceToDevice, ctx->stream));
+    d = nd;
+    cap = nc;
+  }
+  uint32_t *at(size_t p) const { return d + p; }
+  void set0(uint32_t v) {
+    C2_HIP(hipMemcpyAsync(d, &v, 4, hipMemcpyHostToDevice, ctx->stream));
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  int finish(int upto, uint64_t &sum) {
+    std::vector<uint32_t> h((size_t)upto + 1);
+    C2_HIP(hipMemcpyAsync(h.data(), d, 4 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    int z = 0;
+    sum = 0;
+    while (z <= upto && h[z]) sum += h[z++];
+    return z;
+  }
+};
 
 }  // namespace
 
@@ -574,6 +649,7 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
   C2_HIP(hipMemsetAsync(d_err, 0, 8, st));
   const unsigned gN = grid_for(N, YB);
+  const unsigned gF = (unsigned)persistent_threads(ctx->device, YB) / YB;  // frontier kernels, grid-stride
   uint32_t *pend = buf<uint32_t>(ctx, "yt_pend", N);
   C2_LAUNCH(ctx, k_yt_class, dim3(gN), dim3(YB), 0, G, N, g, hmask, pend, maxl, d_cnt, d_err);
   add_kernel_bytes(ctx, "k_yt_class", (double)N * (8.0 * (G.nvec + 1) + 2.0));
@@ -622,18 +698,21 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   // frontier after frontier from the maxima
   uint32_t *fr[2] = {buf<uint32_t>(ctx, "yt_front0", N), buf<uint32_t>(ctx, "yt_front1", N)};
   C2_HIP(hipMemcpyAsync(fr[0], maxl, 4 * (size_t)nmax, hipMemcpyDeviceToDevice, st));
-  uint32_t nf = nmax;
-  uint64_t ndone = nmax;
+  // the frontier sizes stay on the device (fc[p] = size of frontier p); the
+  // host looks at them every FRONT_BATCH frontiers only
+  FrontCounts fc(ctx, "yt_fcnt");
+  fc.set0(nmax);
   int passes = 0;
-  while (nf) {
-    C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
-    C2_LAUNCH(ctx, k_yt_lab_step, dim3(grid_for(nf, YB)), dim3(YB), 0, G, nf, fr[passes & 1], g, hmask, pend, lohi,
-              fr[(passes & 1) ^ 1], d_cnt + 1);
-    add_kernel_bytes(ctx, "k_yt_lab_step", (double)nf * (4.0 + 8.0 * G.nvec * 2.0));
-    passes++;
-    nf = read_scalar(ctx, d_cnt + 1);
-    ndone += nf;
+  for (;;) {
+    fc.reserve(passes + FRONT_BATCH + 1);
+    for (int u = 0; u < FRONT_BATCH; u++, passes++)
+      C2_LAUNCH(ctx, k_yt_lab_step, dim3(gF), dim3(YB), 0, G, fc.at(passes), fr[passes & 1], g, hmask, pend, lohi,
+                fr[(passes & 1) ^ 1], fc.at(passes + 1));
+    if (read_scalar(ctx, fc.at(passes)) == 0) break;
   }
+  uint64_t ndone = 0;
+  passes = fc.finish(passes, ndone);
+  add_kernel_bytes(ctx, "k_yt_lab_step", (double)ndone * (4.0 + 8.0 * G.nvec * 2.0));
   if (ndone != N) fail(C2GPU_ERR_ARG, "c2gpu_yt: internal inconsistency in the label order");
   S.label_passes = passes;
   int32_t *lab = dev && label_out ? label_out : buf<int32_t>(ctx, "yt_lab", N);
@@ -653,6 +732,10 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
     W.c = buf<uint16_t>(ctx, "yt_wc", nias);
     W.top = buf<unsigned long long>(ctx, "yt_wtop", 1);
     uint32_t *wide = buf<uint32_t>(ctx, "yt_wide", nias);
+    unsigned long long *widetot = buf<unsigned long long>(ctx, "yt_widetot", 1);
+    // the wide kernel runs every frontier: one workgroup per CU strides over
+    // however many wide points the frontier has (16 workgroups were 11x slower)
+    const unsigned gWide = (unsigned)persistent_threads(ctx->device, YB) / YB / 8;
     const size_t wide_lds = 9 * (size_t)nattr;
     if (wide_lds > 120 * 1024) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: too many attractors for the LDS accumulator");
     uint64_t cap = 8ull * nias;
@@ -663,32 +746,30 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
       W.w = buf<double>(ctx, "yt_ww", cap);
       C2_HIP(hipMemsetAsync(W.top, 0, 8, st));
       C2_HIP(hipMemsetAsync(d_err, 0, 4, st));
-      C2_HIP(hipMemsetAsync(d_cnt + 1, 0, 4, st));
+      C2_HIP(hipMemsetAsync(widetot, 0, 8, st));
+      FrontCounts wc(ctx, "yt_wcnt");
       C2_LAUNCH(ctx, k_yt_w_init, dim3(grid_for(nias, YB)), dim3(YB), 0, G, nias, ias, hmask, lab, pend, fr[0],
-                d_cnt + 1);
-      uint32_t wf = read_scalar(ctx, d_cnt + 1);
-      uint64_t wdone = wf;
+                wc.at(0));
       wpasses = 0;
-      while (wf) {
-        uint32_t *cf = fr[wpasses & 1], *nx = fr[(wpasses & 1) ^ 1];
-        C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
-        C2_LAUNCH(ctx, k_yt_w_narrow, dim3(grid_for((int64_t)wf * GL, YB)), dim3(YB), 0, G, wf, cf, g, hmask, lab,
-                  slot, W, wide, d_cnt, d_err);
-        // each point: its and its higher neighbours' rho, masks, labels and slots
-        add_kernel_bytes(ctx, "k_yt_w_narrow", (double)wf * (4.0 + 2.0 + 8.0 * (G.nvec + 1) + 8.0 * G.nvec));
-        C2_LAUNCH(ctx, k_yt_w_step, dim3(grid_for(wf, YB)), dim3(YB), 0, G, wf, cf, g, lab, pend, nx, d_cnt + 1);
-        uint32_t cnt2[2];
-        C2_HIP(hipMemcpyAsync(cnt2, d_cnt, 8, hipMemcpyDeviceToHost, st));
-        C2_HIP(hipStreamSynchronize(st));
-        if (cnt2[0]) {
-          C2_LAUNCH(ctx, k_yt_w_wide, dim3(cnt2[0]), dim3(YB), wide_lds, G, wide, nattr, g, hmask, lab, slot, W,
-                    d_err);
-          S.wide += cnt2[0];
+      for (;;) {
+        wc.reserve(wpasses + FRONT_BATCH + 1);
+        for (int u = 0; u < FRONT_BATCH; u++, wpasses++) {
+          uint32_t *cf = fr[wpasses & 1], *nx = fr[(wpasses & 1) ^ 1];
+          C2_HIP(hipMemsetAsync(d_cnt, 0, 4, st));  // the frontier's wide points
+          C2_LAUNCH(ctx, k_yt_w_narrow, dim3(gF), dim3(YB), 0, G, wc.at(wpasses), cf, g, hmask, lab, slot, W, wide,
+                    d_cnt, widetot, d_err);
+          C2_LAUNCH(ctx, k_yt_w_step, dim3(gF), dim3(YB), 0, G, wc.at(wpasses), cf, g, lab, pend, nx,
+                    wc.at(wpasses + 1));
+          C2_LAUNCH(ctx, k_yt_w_wide, dim3(gWide), dim3(YB), wide_lds, G, wide, (const uint32_t *)d_cnt, nattr, g,
+                    hmask, lab, slot, W, d_err);
         }
-        wpasses++;
-        wf = cnt2[1];
-        wdone += wf;
+        if (read_scalar(ctx, wc.at(wpasses)) == 0) break;
       }
+      uint64_t wdone = 0;
+      wpasses = wc.finish(wpasses, wdone);
+      S.wide = (int)read_scalar(ctx, widetot);
+      // each point: its and its higher neighbours' rho, masks, labels and slots
+      add_kernel_bytes(ctx, "k_yt_w_narrow", (double)wdone * (4.0 + 2.0 + 8.0 * (G.nvec + 1) + 8.0 * G.nvec));
       if (wdone != nias) fail(C2GPU_ERR_ARG, "c2gpu_yt: internal inconsistency in the weight order");
       // weight entries: each written once and read back by the points below
       add_kernel_bytes(ctx, "k_yt_w_narrow", 24.0 * (double)read_scalar(ctx, W.top));
