@@ -1,7 +1,8 @@
 """Write the golden fixtures this repository's tests read.
 
 Run in a checkout that has the reference mounted at /root/reference (the
-GPU box does not; the fixtures are committed).  Everything written is data
+machines that run the GPU tests need not have it; the fixtures are
+committed).  Everything written is data
 from the reference's own test and data files:
 
   urea_nci.json      urea cell and atoms (dat/lib/crystal.dat 'structure urea')
