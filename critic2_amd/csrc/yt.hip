@@ -19,12 +19,14 @@
 //                 maxima); equal means the point's basin (a point joins a
 //                 basin exactly when every upward path ends in it),
 //                 different means a surface point;
-//   k_yt_w_step   the surface points' weights, a sparse vector of up to 8
-//                 basins each, in the same topological order over the
-//                 surface points; each basin's weight sums the contributions
-//                 in decreasing rho of the higher neighbour, the order in
-//                 which yt_weights adds them, so the weights are the
-//                 reference's bit for bit;
+//   k_yt_w_narrow / k_yt_w_wide_step
+//                 the surface points' weights, a sparse vector of basins
+//                 each (16 lanes per point up to 32 basins, a workgroup
+//                 beyond), in the same topological order over the surface
+//                 points; each basin's weight sums the contributions in
+//                 decreasing rho of the higher neighbour, the order in which
+//                 yt_weights adds them, so the weights are the reference's
+//                 bit for bit;
 //   sums          basin interiors through the per-basin sums kernel, surface
 //                 points through k_yt_surface_sums.
 #include <hip/hip_runtime.h>
@@ -450,12 +452,20 @@ __device__ void yt_wide_point(const YtGeo &G, uint32_t p, int nattr, const doubl
   }
 }
 
-// The frontier's wide points (their count read from nwp), each by one
-// workgroup (yt_wide_point), the workgroups striding over the list.
-__global__ void __launch_bounds__(YB) k_yt_w_wide(YtGeo G, const uint32_t *__restrict__ wide, const uint32_t *nwp,
-                                                   int nattr, const double *__restrict__ g,
-                                                   const uint16_t *__restrict__ hmask, const int32_t *__restrict__ lab,
-                                                   const int32_t *__restrict__ slot, WBuf W, int *err) {
+// The frontier's wide points, each by one workgroup (yt_wide_point), the
+// workgroups striding over their list; then the next frontier: each surface
+// point below a frontier point loses one pending neighbour, and the thread
+// that takes the last lists it.  One launch: the two parts touch different
+// data (weight entries; pending counts and the next list), and it runs every
+// frontier, where a wide launch of its own would most often find nothing.
+__global__ void __launch_bounds__(YB) k_yt_w_wide_step(YtGeo G, const uint32_t *__restrict__ wide,
+                                                        const uint32_t *nwp, int nattr, const double *__restrict__ g,
+                                                        const uint16_t *__restrict__ hmask,
+                                                        const int32_t *__restrict__ lab,
+                                                        const int32_t *__restrict__ slot, WBuf W, int *err,
+                                                        const uint32_t *nfp, const uint32_t *__restrict__ front,
+                                                        uint32_t *__restrict__ pend, uint32_t *__restrict__ next,
+                                                        uint32_t *nnext) {
   extern __shared__ double acc[];  // nattr doubles, then nattr flags
   uint8_t *pres = reinterpret_cast<uint8_t *>(acc + nattr);
   __shared__ uint32_t hq[14];
@@ -464,9 +474,26 @@ __global__ void __launch_bounds__(YB) k_yt_w_wide(YtGeo G, const uint32_t *__res
   __shared__ uint32_t wsum[YB / 64];
   __shared__ unsigned long long base;
   const uint32_t nw = *nwp;
-  for (uint32_t qi = blockIdx.x; qi < nw; qi += gridDim.x) {  // one wide point per workgroup at a time
+  for (uint32_t qi = blockIdx.x; qi < nw; qi += gridDim.x) {
     yt_wide_point(G, wide[qi], nattr, g, hmask, lab, slot, W, err, acc, pres, hq, fr, nh, wsum, base);
     __syncthreads();
+  }
+  const uint32_t nf = *nfp;
+  for (uint32_t b0 = blockIdx.x * YB; b0 < nf; b0 += gridDim.x * YB) {  // uniform per workgroup
+    const uint32_t t = b0 + threadIdx.x;
+    uint32_t done[14];
+    int nd = 0;
+    if (t < nf) {
+      const uint32_t p = front[t];
+      int i, j, k;
+      coords3(G, p, i, j, k);
+      const double gp = g[p];
+      for (int q = 0; q < G.nvec; q++) {
+        const uint32_t jn = nbr(G, i, j, k, q);
+        if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
+      }
+    }
+    for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
   }
 }
 
@@ -493,31 +520,6 @@ __global__ void __launch_bounds__(YB) k_yt_w_init(YtGeo G, uint32_t nias, const 
     ready = c == 0;
   }
   yt_append(ready, p, front, nfront);
-}
-
-// The next frontier: each surface point below a frontier point loses one
-// pending neighbour; the thread that takes the last lists it.
-__global__ void __launch_bounds__(YB) k_yt_w_step(YtGeo G, const uint32_t *nfp, const uint32_t *__restrict__ front,
-                                                   const double *__restrict__ g, const int32_t *__restrict__ lab,
-                                                   uint32_t *__restrict__ pend, uint32_t *__restrict__ next,
-                                                   uint32_t *nnext) {
-  const uint32_t nf = *nfp;
-  for (uint32_t b0 = blockIdx.x * YB; b0 < nf; b0 += gridDim.x * YB) {  // uniform per workgroup
-    const uint32_t t = b0 + threadIdx.x;
-    uint32_t done[14];
-    int nd = 0;
-    if (t < nf) {
-      const uint32_t p = front[t];
-      int i, j, k;
-      coords3(G, p, i, j, k);
-      const double gp = g[p];
-      for (int q = 0; q < G.nvec; q++) {
-        const uint32_t jn = nbr(G, i, j, k, q);
-        if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
-      }
-    }
-    for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
-  }
 }
 
 struct YtFields {
@@ -733,9 +735,6 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
     W.top = buf<unsigned long long>(ctx, "yt_wtop", 1);
     uint32_t *wide = buf<uint32_t>(ctx, "yt_wide", nias);
     unsigned long long *widetot = buf<unsigned long long>(ctx, "yt_widetot", 1);
-    // the wide kernel runs every frontier: one workgroup per CU strides over
-    // however many wide points the frontier has (16 workgroups were 11x slower)
-    const unsigned gWide = (unsigned)persistent_threads(ctx->device, YB) / YB / 8;
     const size_t wide_lds = 9 * (size_t)nattr;
     if (wide_lds > 120 * 1024) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: too many attractors for the LDS accumulator");
     uint64_t cap = 8ull * nias;
@@ -758,10 +757,9 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
           C2_HIP(hipMemsetAsync(d_cnt, 0, 4, st));  // the frontier's wide points
           C2_LAUNCH(ctx, k_yt_w_narrow, dim3(gF), dim3(YB), 0, G, wc.at(wpasses), cf, g, hmask, lab, slot, W, wide,
                     d_cnt, widetot, d_err);
-          C2_LAUNCH(ctx, k_yt_w_step, dim3(gF), dim3(YB), 0, G, wc.at(wpasses), cf, g, lab, pend, nx,
+          C2_LAUNCH(ctx, k_yt_w_wide_step, dim3(gF), dim3(YB), wide_lds, G, wide, (const uint32_t *)d_cnt, nattr,
+                    g, hmask, lab, slot, W, d_err, (const uint32_t *)wc.at(wpasses), cf, pend, nx,
                     wc.at(wpasses + 1));
-          C2_LAUNCH(ctx, k_yt_w_wide, dim3(gWide), dim3(YB), wide_lds, G, wide, (const uint32_t *)d_cnt, nattr, g,
-                    hmask, lab, slot, W, d_err);
         }
         if (read_scalar(ctx, wc.at(wpasses)) == 0) break;
       }
