@@ -73,3 +73,28 @@ def test_yt_refuses_equal_neighbours(ctx):
     with pytest.raises(_lib.C2GpuError) as e:
         gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat)
     assert e.value.code == 8
+
+
+YT_EDGE = [  # name, seed, cell edges (A), angles, atoms, grid
+    ("tiny-3x5x7", 71, (4.0, 5.0, 6.0), (90, 90, 90), 2, (3, 5, 7)),
+    ("odd-17x3x11", 72, (7.0, 4.0, 6.0), (80, 100, 95), 3, (17, 3, 11)),
+    ("prime-13x19x23", 73, (7.0, 8.0, 9.0), (85, 92, 101), 4, (13, 19, 23)),
+    ("cube-5", 74, (5.0, 5.0, 5.0), (90, 90, 90), 1, (5, 5, 5)),
+]
+
+
+@pytest.mark.parametrize("name,seed,edge,ang,nat,n", YT_EDGE, ids=[e[0] for e in YT_EDGE])
+def test_yt_edge_grids(ctx, name, seed, edge, ang, nat, n):
+    """The smallest grids (3 points on an axis, where opposite lattice
+    neighbours wrap onto nearby points), odd and prime sizes, against the
+    oracle."""
+    cell = random_cell(seed, edge, ang, nat, (1, 6, 7, 8), n)
+    print(f"[case] yt {name} n={cell.n}", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    ref, got = _run_both(ctx, cell, f, {})
+    idg, nattr, xattr, vol, sums = ref
+    assert got["nattr"] == nattr
+    np.testing.assert_array_equal(got["xattr"], xattr)
+    assert int(np.count_nonzero(got["labels"] != idg)) == 0
+    np.testing.assert_allclose(got["vol"], vol, rtol=1e-10)
+    np.testing.assert_allclose(got["sums"], sums, rtol=1e-10)
