@@ -15,6 +15,9 @@ pytestmark = pytest.mark.gpu
     (71, (8.0, 8.5, 9.0), (90, 90, 90), (32, 34, 36), 5, False),
     (72, (8.5, 9.0, 8.0), (80, 97, 105), (34, 32, 36), 5, True),
     (73, (9.0, 9.0, 9.5), (75, 100, 110), (30, 32, 34), 8, False),
+    (74, (7.0, 7.5, 8.0), (85, 95, 100), (24, 26, 28), 0, False),   # monopole only
+    (75, (7.5, 8.0, 7.0), (90, 90, 90), (26, 24, 28), 10, True),    # the largest lmax
+    (76, (5.0, 6.0, 7.0), (80, 100, 95), (7, 11, 13), 3, False),    # small odd grid
 ])
 def test_multipoles_match_oracle(ctx, seed, edge, ang, n, lmax, device):
     cell = random_cell(seed, edge, ang, 4, (1, 6, 7, 8), n)
@@ -33,3 +36,14 @@ def test_multipoles_match_oracle(ctx, seed, edge, ang, n, lmax, device):
         scale = np.max(np.abs(ref[:, s]), axis=1, keepdims=True)
         err = np.abs(got[:, s] - ref[:, s]) / np.maximum(scale, 1e-300)
         assert err.max() < 1e-10, (l, err.max())
+
+
+
+def test_monopole_is_basin_sum(ctx):
+    """lmax = 0: Q_00 of each basin is its plain sum of f (R_00 = 1)."""
+    cell = random_cell(77, (8.0, 8.0, 8.5), (90, 90, 90), 4, (1, 6, 7, 8), (28, 28, 30))
+    f = gb.synth_slater(ctx, cell)
+    bas = gb.bader_integrate(ctx, f, cell.n, cell.x2c, cell.xat)
+    q = gb.multipoles(ctx, bas.idg, f, cell.n, cell.x2c, bas.xattr[:bas.nattr], 0)
+    _, sums = gb.basin_sums(ctx, bas.idg, bas.nattr, [f])
+    np.testing.assert_allclose(q[:, 0], np.asarray(sums).reshape(-1), rtol=1e-12)
