@@ -142,14 +142,18 @@ def test_bader_labels_follow_steepest_ascent_in_the_interior():
         assert lab[p[0] + n[0] * (p[1] + n[1] * p[2])] == a + 1
 
 
-def test_fft_derivatives_closed_form():
+@pytest.mark.parametrize("n,mode", [((12, 14, 16), (1, -2, 3)), ((11, 13, 15), (1, -2, 3)),
+                                    ((12, 14, 16), (6, 0, 0)), ((12, 13, 16), (0, 0, 8))])
+def test_fft_derivatives_closed_form(n, mode):
     """grid3%fft restatement against a single Fourier mode in a triclinic cell:
-    rho = 2 + cos(G.r) -> |grad| = |G sin|, d2/dx2 = -Gx^2 cos, lap = -|G|^2 cos."""
+    rho = 2 + cos(G.r) -> |grad| = |G sin|, d2/dx2 = -Gx^2 cos, lap = -|G|^2 cos.
+    Even and odd grids; modes on the Nyquist plane (|m| = n/2) keep their
+    second derivatives (a dropped Nyquist bin would zero them) and have no
+    gradient at the nodes."""
     from critic2_amd.synth import lattice_from_params
     x2c = lattice_from_params(7, 8, 9, 80, 100, 110)
-    n = (12, 14, 16)
     B = 2 * math.pi * np.linalg.inv(x2c)  # rows: reciprocal vectors
-    m = np.array([1, -2, 3])
+    m = np.array(mode)
     G = m @ B
     ii = [np.arange(q) / q for q in n]
     I, J, K = np.meshgrid(*ii, indexing="ij")
