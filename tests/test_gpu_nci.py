@@ -97,8 +97,13 @@ def test_fft_deriv_matches_oracle(ctx, kind, name, grid):
     np.testing.assert_allclose(g, o, rtol=1e-9, atol=1e-9 * np.abs(o).max())
 
 
-def test_nci_fft_scheme_matches_oracle(ctx):
-    cell = config(1)
+@pytest.mark.parametrize("which", ["cubic64", "triclinic_odd"])
+def test_nci_fft_scheme_matches_oracle(ctx, which):
+    if which == "cubic64":
+        cell = config(1)
+    else:
+        cell = random_cell(64, (6.5, 7.5, 8.5), (75, 95, 115), 5, (1, 6, 7, 8), (27, 25, 31))
+    print(f"[case] nci fft scheme {which} n={cell.n}", flush=True)
     f = gb.synth_slater(ctx, cell)
     crho, s = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, zfast=False)
     cr_ref, s_ref = oracle.nci_grid_fft(f, cell.n, cell.x2c)
@@ -112,14 +117,16 @@ def test_nci_fft_scheme_matches_oracle(ctx):
     cst = 2 * (3 * math.pi ** 2) ** (1 / 3)
     g, g_ref = s * cst * f ** (4 / 3), s_ref * cst * f ** (4 / 3)
     assert np.abs(g - g_ref)[ok].max() <= 1e-12 * g_ref.max()
-    assert np.mean(np.sign(crho) == np.sign(cr_ref)) > 0.9999
+    # the lambda2 sign may differ only where a diagonal Hessian element is
+    # round-off close to zero
+    assert np.count_nonzero(np.sign(crho) != np.sign(cr_ref)) <= max(2, 1e-4 * f.size)
     # z-fastest layout of the reference's crho/cgrad arrays
     cz, sz = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, zfast=True)
     n = cell.n
     assert np.array_equal(cz.reshape(n[0], n[1], n[2]), crho.reshape(n[2], n[1], n[0]).transpose(2, 1, 0))
 
 
-@pytest.mark.parametrize("which", ["cubic64", "triclinic"])
+@pytest.mark.parametrize("which", ["cubic64", "triclinic", "triclinic_odd"])
 def test_nci_grid_interpolation_scheme_matches_oracle(ctx, which):
     """INTERPOLATION GRID (nci@proc.f90:565-571): tricubic value, gradient
     and Hessian at the node as field%grd locates it.  The node search is
@@ -131,8 +138,10 @@ def test_nci_grid_interpolation_scheme_matches_oracle(ctx, which):
     extrapolation), and the lambda2 sign everywhere."""
     if which == "cubic64":
         cell = config(1)
-    else:
+    elif which == "triclinic":
         cell = random_cell(61, (7.0, 7.5, 8.0), (80, 100, 110), 4, (1, 6, 7, 8), (24, 26, 28))
+    else:
+        cell = random_cell(63, (6.5, 7.5, 8.5), (75, 95, 115), 5, (1, 6, 7, 8), (23, 25, 29))
     print(f"[case] nci grid scheme {which} n={cell.n}", flush=True)
     f = gb.synth_slater(ctx, cell)
     crho, s = gn.nciplot_grid(ctx, f, cell.n, cell.x2c, interpolation="grid")
