@@ -1120,6 +1120,11 @@ void nci_promol_nodes_run(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nd, const doubl
   const bool f32 = precision == C2GPU_FP32;
   const size_t N = (size_t)G.n1 * G.n2 * G.n3, osz = f32 ? 4 : 8;
   const bool dev = flags & C2GPU_PTR_DEVICE;
+  if (dev) {
+    require_device_ptr(crho_out, "crho_out");
+    require_device_ptr(cgrad_out, "cgrad_out");
+    if (rhoat_out) require_device_ptr(rhoat_out, "rhoat_out");
+  }
   hipStream_t st = ctx->stream;
   double lo[3], hi[3];
   box_frac_range(G, x2c, lo, hi);
@@ -1217,7 +1222,8 @@ void nci_fragments_run(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nd, int nspecies, 
     fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: precision must be C2GPU_FP64 or C2GPU_FP32");
   std::vector<int> fstart(nfrag + 1, 0);
   for (int f = 0; f < nfrag; f++) {
-    if (frag_nat[f] < 0) fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: negative atom count");
+    if (frag_nat[f] < 0 || (int64_t)fstart[f] + frag_nat[f] > (int64_t)INT32_MAX)
+      fail(C2GPU_ERR_ARG, "c2gpu_nci_fragments: bad atom count");
     fstart[f + 1] = fstart[f] + frag_nat[f];
   }
   const int nat = fstart[nfrag];
@@ -1237,6 +1243,7 @@ void nci_fragments_run(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nd, int nspecies, 
   const uint64_t npts = (uint64_t)G.n1 * G.n2 * G.n3;
   const size_t osz = precision == C2GPU_FP32 ? 4 : 8;
   const bool dev = flags & C2GPU_PTR_DEVICE;
+  if (dev) require_device_ptr(rhofrag_out, "rhofrag_out");
   void *out = dev ? rhofrag_out : buffer(ctx, "frag_out", osz * npts * nfrag);
   const unsigned g = grid_for((int64_t)npts, 256);
   if (precision == C2GPU_FP32)

@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+from critic2_amd import _lib
 from critic2_amd import nci as gn
 from critic2_amd.radial import slater_grid1
 from critic2_amd.synth import random_cell
@@ -146,3 +147,31 @@ def test_cutoff_frag_matches_oracle(ctx, dtype, device, opts):
     assert text == want
     assert np.array_equal(g.astype(np.float64), cgo)
     np.testing.assert_allclose(sums, sums_o, rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_nodes_and_fragments_device_outputs(ctx, precision):
+    """C2GPU_PTR_DEVICE: outputs written in place in device memory equal the
+    host-copy path's; host arrays under the flag are an argument error
+    raised before any kernel runs."""
+    cell = random_cell(44, (6.5, 7.0, 7.5), (85, 95, 100), 4, (1, 6, 8), (8, 8, 8))
+    tables, species = _slater_setup(cell)
+    nb = gn.box_nodes(cell.x2c, (-0.1, 0.0, 0.2), (0.8, 1.1, 0.9), nstep=(7, 6, 5))
+    npts = 7 * 6 * 5
+    dt = np.float32 if precision == "fp32" else np.float64
+    host = gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision=precision)
+    douts = tuple(ctx.to_device(np.zeros(npts, dtype=dt)) for _ in range(3))
+    gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision=precision, outs=douts)
+    for h, d in zip(host, douts):
+        assert np.array_equal(d.numpy(), h)
+    pos, idx = images(cell.x2c, cell.xat, lim=1)
+    frags = [(pos[:5], species[idx][:5]), (pos[5:], species[idx][5:])]
+    rf = gn.fragments(ctx, nb, tables, frags, precision=precision)
+    dfr = ctx.to_device(np.zeros((2, npts), dtype=dt))
+    gn.fragments(ctx, nb, tables, frags, precision=precision, out=dfr)
+    assert np.array_equal(dfr.numpy(), rf)
+    with pytest.raises(_lib.C2GpuError):
+        gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision=precision,
+                        outs=tuple(np.zeros(npts, dtype=dt) for _ in range(3)))
+    with pytest.raises(_lib.C2GpuError):
+        gn.fragments(ctx, nb, tables, frags, precision=precision, out=np.zeros((2, npts), dtype=dt))
