@@ -144,7 +144,9 @@ typedef struct {
  * new ones numbered in the serial sweep's discovery order.  If *nattr exceeds
  * xattr_cap only the first xattr_cap positions are written.  The DISCARD
  * expression filter has no counterpart here (the caller keeps critic2's own
- * path for it). */
+ * path for it).  Limits: every grid edge >= 3 points, fewer than 2^32 - 1
+ * points, and ascent paths of at most 2048 points (a longer one returns
+ * C2GPU_ERR_PATHLEN; the reference's path array grows without bound). */
 int c2gpu_bader(c2gpu_ctx *ctx, const double *rho, const int n[3],
                 const double car2lat[9], const double lat_i_dist[27],
                 const c2gpu_attractor_opts *opts, int flags, int32_t *label_out,
