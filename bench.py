@@ -78,6 +78,45 @@ def run_step(ctx, cell, rho, lab, outs, tables, species, precision):
     return bas
 
 
+def run_host_buffers(ctx, args):
+    """The same Bader + Volume/Pop + FP32 NCI step with every array in host
+    memory, as a Fortran caller hands them over: the grid goes to the device
+    and labels, sums and the NCI outputs come back inside each call (the
+    PCIe-inclusive rate; the headline keeps the grid resident)."""
+    cell = config(args.config, n_override=(args.n,) * 3 if args.config == 3 else None)
+    N = cell.npts
+    tables, species = species_of(cell)
+    rho_d = ctx.empty((N,), np.float64)
+    gb.synth_slater(ctx, cell, device_out=rho_d)
+    rho = rho_d.numpy()
+    rho_d.free()
+    print(f"[bench] grid {cell.n} in host memory, {N} points", file=sys.stderr, flush=True)
+
+    def step_():
+        bas = gb.bader_integrate(ctx, rho, cell.n, cell.x2c, cell.xat)
+        gb.basin_sums(ctx, bas.idg, bas.nattr, [rho])
+        gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp32", out_f32=True)
+        ctx.sync()
+        return bas
+
+    for w in range(args.warmup):
+        step_()
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        step_()
+        print(f"[bench] step {s_ + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    dt = (time.perf_counter() - t0) / args.steps
+    out = {"metric": f"Gpoints/s Bader+NCI @{cell.n[0]}^3, host buffers (PCIe-inclusive)",
+           "value": round(N / dt / 1e9, 6), "unit": "Gpoints/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+           "config": {"workload": "the Bader + Volume/Pop + FP32 promolecular NCI step with the grid, labels and "
+                                  "NCI outputs in host memory", "grid": list(cell.n),
+                      "parallelism": "single GPU"}}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(ctx, cell, tables, species):
     """The oracle (scalar C restatement of critic2's algorithm, 1 thread) on a
     bounded sample of the same workload: the Bader sweep+refine on the same
@@ -388,6 +427,9 @@ def main():
     ap.add_argument("--cube", action="store_true",
                     help="instead of the Bader+NCI step, time the text cube parser on the --n^3 grid's "
                          "precisecube text")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="time the same step with the grid, labels and NCI outputs in host memory "
+                         "(PCIe-inclusive rate)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a streaming kernel of known traffic (k_sqrt of the FFT |grad| grid: "
                          "8 B read + 8 B written per point, 256^3) to scale rocprof FETCH_SIZE/WRITE_SIZE")
@@ -404,6 +446,9 @@ def main():
         return
     if args.yt:
         run_yt(ctx, args)
+        return
+    if args.host_buffers:
+        run_host_buffers(ctx, args)
         return
     if args.copy_probe:
         probe = config(2)
