@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY (the oracle).  Loaded by tests/, smoke() and the
  * cpu_baseline leg of bench.py as the checker; never shipped as the product.
  *
- * Follows /root/reference/src/bader@proc.f90 statement by statement:
+ * Follows critic2's src/bader@proc.f90 statement by statement:
  *   bader_integrate       bader@proc.f90:80-234
  *   refine_edge           bader@proc.f90:300-422
  *   max_neargrid          bader@proc.f90:427-450
@@ -24,6 +24,14 @@
  * Indices are 0-based; the iteration order (k innermost, then j, then i) and
  * every comparison, rounding and accumulation order is kept.  Compile without
  * FP contraction (-ffp-contract=off): the reference build emits no FMA.
+ *
+ * Pinning: the grid metrics equal explicit norms and inverses; one maximum
+ * gives one basin; a grid-symmetric pair splits evenly; counts sum to N and
+ * populations to the grid sum; each atom's nearest grid point carries its
+ * own label; the attractor rules follow their hand-worked cases.  The exact
+ * order-dependent labels of the serial sweep and refinement are parity
+ * unpinned against the reference itself: its Bader golden outputs need
+ * input grids that its checkout does not hold.
  */
 #include "c2oracle.h"
 #include <math.h>

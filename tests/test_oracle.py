@@ -262,3 +262,31 @@ def test_nci_grid_scheme_at_nodes():
         assert np.sign(crho[o]) == np.sign(l2)
         checked += 1
     assert checked == 4
+
+
+def test_bader_attractor_rules_hand_worked():
+    """bader@proc.f90:159-199 on two maxima, one on an atom (A at x = 1/4) and
+    one off every atom (B at x = 5/8, 4.5 bohr from A): the nearest-atom rule
+    sends both to A; RATOM 1 keeps A and appends B as a non-nuclear maximum at
+    its grid position, RATOM 5 reaches A from B; NOATOMS numbers the maxima in
+    discovery order (the sweep starts at the origin, 3 bohr from A and 4.5
+    from B, so A first) and merges B into A when ratom exceeds 4.5 bohr."""
+    x2c = np.diag([12.0, 12.0, 12.0])
+    n = (24, 24, 24)
+    a, b = (0.25, 0.5, 0.5), (0.625, 0.5, 0.5)
+    f = _gauss_cell(n, [a, b], [2.0, 2.0], x2c)
+    ia = 6 + 24 * (12 + 24 * 12)
+    ib = 15 + 24 * (12 + 24 * 12)
+    lab, info, xattr, _ = oracle.bader(f, n, x2c, [a], mode=0)
+    assert info["nattr"] == 1 and np.all(lab == 1)
+    lab, info, xattr, _ = oracle.bader(f, n, x2c, [a], mode=1, ratom=1.0)
+    assert info["nattr"] == 2 and lab[ia] == 1 and lab[ib] == 2
+    np.testing.assert_array_equal(xattr, [a, b])
+    lab, info, xattr, _ = oracle.bader(f, n, x2c, [a], mode=1, ratom=5.0)
+    assert info["nattr"] == 1 and np.all(lab == 1)
+    lab, info, xattr, _ = oracle.bader(f, n, x2c, [a], mode=2, ratom=4.0)
+    assert info["nattr"] == 2 and lab[ia] == 1 and lab[ib] == 2
+    np.testing.assert_array_equal(xattr, [a, b])
+    lab, info, xattr, _ = oracle.bader(f, n, x2c, [a], mode=2, ratom=5.0)
+    assert info["nattr"] == 1 and np.all(lab == 1)
+    np.testing.assert_array_equal(xattr, [a])
