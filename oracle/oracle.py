@@ -11,6 +11,23 @@ of grid3%fft (grid3mod@proc.f90:1757-1872) are restated here with numpy's
 FFT: same transform conventions (forward exp(-i), scaled 1/N; backward
 exp(+i), unscaled), same reciprocal vectors and Nyquist handling; only the
 round-off of the FFT itself differs from fftpack5.
+
+What pins each part (the checks live in tests/):
+- FFT grids: single Fourier modes in a triclinic cell, even and odd grids,
+  Nyquist-plane modes.
+- Promolecular NCI and the cutoff pass: the reference's urea outputs
+  (005_nciplot_basic cubes and .dat, 006_nciplot_fragment cubes).
+- Fragments: the partition identity, table nodes and hand-worked cutoff
+  cases.
+- Tricubic interpolation: golden a = c.b vectors from the reference's table,
+  and exact quadratics.
+- Cube readers: the reference's cube files, and Python's float().
+- YT: the partition of unity and the one-maximum limit.
+- Facets: closed-form Voronoi cells.
+- Multipoles: the closed-form harmonics, the sum rule and the addition
+  theorem.
+- Bader: see bader.c.  Its order-dependent labels are parity unpinned
+  against the reference itself.
 """
 from __future__ import annotations
 
