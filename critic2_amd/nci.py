@@ -222,6 +222,30 @@ def fragment_atoms(x2c, xat, species, coords_ang):
     return np.array(keep).reshape(-1, 3), np.array(spc, dtype=np.int32)
 
 
+def _out_flags(outs, dtype, size):
+    """C2GPU_PTR_DEVICE for device outputs, 0 for host arrays (written in
+    place); a mix of the two is refused.  Every output must hold `size`
+    values of `dtype`."""
+    dev = [hasattr(o, "data_ptr") for o in outs if o is not None]
+    for o in outs:
+        if o is None:
+            continue
+        try:
+            odt = np.dtype(o.dtype)
+        except TypeError:  # a torch dtype
+            odt = np.dtype(str(o.dtype).replace("torch.", ""))
+        if odt != np.dtype(dtype) or int(np.prod(tuple(o.shape))) != size:
+            raise ValueError(f"outputs must hold {size} values of {np.dtype(dtype).name}")
+        if not hasattr(o, "data_ptr") and not (isinstance(o, np.ndarray) and o.flags.c_contiguous
+                                               and o.flags.writeable):
+            raise ValueError("host outputs must be writeable C-contiguous numpy arrays")
+    if all(dev):
+        return _lib.PTR_DEVICE
+    if not any(dev):
+        return 0
+    raise ValueError("outputs must be all device arrays or all host arrays")
+
+
 def promol_nodes(ctx: Context, nodes, x2c, tables: list[Grid1], xat, species, precision="fp64", outs=None):
     """The promolecular NCI block on an nciplot node set (c2gpu_nci_promol_nodes):
     (crho, cgrad, rhoat), z-fastest, of the precision's type."""
@@ -229,12 +253,10 @@ def promol_nodes(ctx: Context, nodes, x2c, tables: list[Grid1], xat, species, pr
     npts = int(np.prod(list(nodes.nstep)))
     pm, keep = _promol_struct(tables, xat, species)
     x = _lib.mat_f(x2c)
+    dt = np.float32 if prec == _lib.FP32 else np.float64
     if outs is None:
-        dt = np.float32 if prec == _lib.FP32 else np.float64
         outs = tuple(np.zeros(npts, dtype=dt) for _ in range(3))
-        flags = 0
-    else:
-        flags = _lib.PTR_DEVICE
+    flags = _out_flags(outs, dt, npts)
     rc = ctx.lib.c2gpu_nci_promol_nodes(ctx.h, _lib.C.byref(nodes), x.ctypes.data, _lib.C.byref(pm), prec, flags,
                                         *(_lib.ptr(o) for o in outs))
     ctx.check(rc, "c2gpu_nci_promol_nodes")
@@ -253,11 +275,10 @@ def fragments(ctx: Context, nodes, tables: list[Grid1], frags, precision="fp64",
                                .reshape(-1)) if nat.sum() else np.zeros(3)
     spc = np.ascontiguousarray(np.concatenate([np.asarray(f[1], dtype=np.int32).reshape(-1) for f in frags])) \
         if nat.sum() else np.zeros(1, dtype=np.int32)
+    dt = np.float32 if prec == _lib.FP32 else np.float64
     if out is None:
-        out = np.zeros((len(frags), npts), dtype=np.float32 if prec == _lib.FP32 else np.float64)
-        flags = 0
-    else:
-        flags = _lib.PTR_DEVICE
+        out = np.zeros((len(frags), npts), dtype=dt)
+    flags = _out_flags((out,), dt, len(frags) * npts)
     rc = ctx.lib.c2gpu_nci_fragments(ctx.h, _lib.C.byref(nodes), pm.nspecies, pm.tables, len(frags),
                                      nat.ctypes.data, xyz.ctypes.data, spc.ctypes.data, prec, flags, _lib.ptr(out))
     ctx.check(rc, "c2gpu_nci_fragments")

@@ -170,8 +170,28 @@ def test_nodes_and_fragments_device_outputs(ctx, precision):
     dfr = ctx.to_device(np.zeros((2, npts), dtype=dt))
     gn.fragments(ctx, nb, tables, frags, precision=precision, out=dfr)
     assert np.array_equal(dfr.numpy(), rf)
-    with pytest.raises(_lib.C2GpuError):
+    # host arrays given as outputs are written in place
+    houts = tuple(np.zeros(npts, dtype=dt) for _ in range(3))
+    gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision=precision, outs=houts)
+    for h, o in zip(host, houts):
+        assert np.array_equal(o, h)
+    with pytest.raises(ValueError):
         gn.promol_nodes(ctx, nb, cell.x2c, tables, cell.xat, species, precision=precision,
-                        outs=tuple(np.zeros(npts, dtype=dt) for _ in range(3)))
-    with pytest.raises(_lib.C2GpuError):
-        gn.fragments(ctx, nb, tables, frags, precision=precision, out=np.zeros((2, npts), dtype=dt))
+                        outs=(douts[0], douts[1], np.zeros(npts, dtype=dt)))
+    with pytest.raises(ValueError):
+        gn.fragments(ctx, nb, tables, frags, precision=precision, out=np.zeros((2, npts - 1), dtype=dt))
+    # the C ABI itself: host memory under C2GPU_PTR_DEVICE is an argument error
+    pm, keep = gn._promol_struct(tables, cell.xat, species)
+    x = _lib.mat_f(cell.x2c)
+    prec = _lib.FP32 if precision == "fp32" else _lib.FP64
+    rc = ctx.lib.c2gpu_nci_promol_nodes(ctx.h, _lib.C.byref(nb), x.ctypes.data, _lib.C.byref(pm), prec,
+                                        _lib.PTR_DEVICE, *(o.ctypes.data for o in houts))
+    assert rc == 1
+    nat = np.array([5, len(idx) - 5], dtype=np.int32)
+    xyz = np.ascontiguousarray(pos.reshape(-1))
+    spc = np.ascontiguousarray(species[idx], dtype=np.int32)
+    hfr = np.zeros((2, npts), dtype=dt)
+    rc = ctx.lib.c2gpu_nci_fragments(ctx.h, _lib.C.byref(nb), pm.nspecies, pm.tables, 2, nat.ctypes.data,
+                                     xyz.ctypes.data, spc.ctypes.data, prec, _lib.PTR_DEVICE, hfr.ctypes.data)
+    assert rc == 1
+    del keep
