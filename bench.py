@@ -523,7 +523,8 @@ def main():
     value = N / dt / 1e9
     top = sorted(ktimes.items(), key=lambda kv: -kv[1][0])[:8]
     out = {
-        "metric": "Gpoints/s Bader+NCI @512^3, mixed vs FP64 GPU (labels exact); %HBM peak" if args.config == 3
+        "metric": f"Gpoints/s Bader+NCI @{cell.n[0]}^3, mixed vs FP64 GPU (labels exact); %HBM peak"
+                  if args.config == 3
                   else f"Gpoints/s Bader+NCI @{cell.n[0]}^3 triclinic, mixed (labels exact); %HBM peak",
         "value": round(value, 6),
         "unit": "Gpoints/s",

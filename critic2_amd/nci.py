@@ -49,16 +49,16 @@ def _run(ctx, rho, n, x2c, scheme, precision, pm, zfast, out_f32, outs=None):
     nn = np.asarray(n, dtype=np.int32)
     npts = int(nn.prod())
     x = _lib.mat_f(x2c)
-    dev = outs is not None
-    if dev:
-        crho, cgrad = outs
-    else:
-        dt = np.float32 if out_f32 else np.float64
-        crho = np.zeros(npts, dtype=dt)
-        cgrad = np.zeros(npts, dtype=dt)
-    if rho is not None and not hasattr(rho, "data_ptr"):
-        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
-    flags = _lib.PTR_DEVICE if dev else 0
+    dt = np.float32 if out_f32 else np.float64
+    if outs is None:
+        outs = (np.zeros(npts, dtype=dt), np.zeros(npts, dtype=dt))
+    crho, cgrad = outs
+    flags = _out_flags(outs, dt, npts)
+    if rho is not None:
+        if hasattr(rho, "data_ptr") != (flags == _lib.PTR_DEVICE):
+            raise ValueError("the grid and the outputs must both be device arrays or both host arrays")
+        if flags == 0:
+            rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
     fn = ctx.lib.c2gpu_nci_f32 if out_f32 else ctx.lib.c2gpu_nci
     rc = fn(ctx.h, _lib.ptr(rho) if rho is not None else None, nn.ctypes.data, x.ctypes.data, scheme,
             precision, _lib.C.byref(pm) if pm is not None else None, 1 if zfast else 0, flags,
