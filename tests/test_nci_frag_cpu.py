@@ -155,3 +155,30 @@ def test_cell_nodes_from_xinc():
     xm = np.array(list(nd.xmat)).reshape(3, 3).T
     np.testing.assert_allclose(xm * np.array(list(nd.nstep))[None, :], x2c, rtol=1e-15)
     assert list(gn.cell_nodes(x2c, xinc=0.5).nstep) == [21, 18, 7]
+
+
+def test_output_memory_and_shape_checks():
+    """The mirror passes C2GPU_PTR_DEVICE only when every output is device
+    memory, 0 when every output is a host array (written in place), and
+    refuses a mix, a wrong type or a wrong size before calling the library."""
+    class Dev:  # stands in for a DeviceArray: an address, a shape and a type
+        def __init__(self, n, dt):
+            self.shape, self.dtype = (n,), np.dtype(dt)
+
+        def data_ptr(self):
+            return 1
+
+    n = 12
+    assert gn._out_flags((Dev(n, np.float64), Dev(n, np.float64)), np.float64, n) == gn._lib.PTR_DEVICE
+    assert gn._out_flags((np.zeros(n), np.zeros(n)), np.float64, n) == 0
+    bad = [((Dev(n, np.float64), np.zeros(n)), np.float64, n),
+           ((np.zeros(n, dtype=np.float32),), np.float64, n),
+           ((np.zeros(n - 1),), np.float64, n),
+           ((np.zeros((n, 2))[:, 0],), np.float64, n),
+           ((Dev(n, np.float32),), np.float64, n)]
+    for outs, dt, size in bad:
+        try:
+            gn._out_flags(outs, dt, size)
+        except ValueError:
+            continue
+        raise AssertionError(f"accepted {outs!r}")
