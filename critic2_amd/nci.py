@@ -103,18 +103,38 @@ def cutoffs(ctx: Context, crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srho
     sums (charge, promolecular charge, node count)."""
     import ctypes as C
     dev = hasattr(crho, "data_ptr")
-    dt = np.dtype(crho.dtype)
+    dt = _np_dtype(crho.dtype)
+    if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ValueError("crho must be float64 or float32")
     prec = _lib.FP64 if dt == np.float64 else _lib.FP32
+    npts = int(np.prod(crho.shape))
+    nfrag = 0 if rhofrag is None else int(rhofrag.shape[0])
+    # the library reads and writes npts values of every array (cgrad in place):
+    # sizes, the memory space and the value type are checked before the call
+    need = [("cgrad", cgrad, (npts,))]
+    if rhoat is not None:
+        need.append(("rhoat", rhoat, (npts,)))
+    elif nfrag or rho_void > 0:
+        raise ValueError("rhoat is required with fragments or rho_void > 0")
+    if rhofrag is not None:
+        if len(tuple(rhofrag.shape)) != 2:
+            raise ValueError("rhofrag must have shape (nfrag, npts)")
+        need.append(("rhofrag", rhofrag, (nfrag, npts)))
+    for name, a, shape in need:
+        if hasattr(a, "data_ptr") != dev:
+            raise ValueError(f"{name} and crho must both be device arrays or both host arrays")
+        if _np_dtype(a.dtype) != dt:
+            raise ValueError(f"{name} must have crho's type {dt}")
+        if int(np.prod(tuple(a.shape))) != int(np.prod(shape)) or (len(shape) == 2 and tuple(a.shape) != shape):
+            raise ValueError(f"{name} must hold {shape} values (crho has {npts})")
     if not dev:
-        if not (isinstance(cgrad, np.ndarray) and cgrad.dtype == dt and cgrad.flags.c_contiguous):
+        if not (isinstance(cgrad, np.ndarray) and cgrad.flags.c_contiguous):
             raise ValueError("cgrad must be a contiguous array of crho's type (it is masked in place)")
         crho = np.ascontiguousarray(crho)
         if rhoat is not None:
             rhoat = np.ascontiguousarray(rhoat, dtype=dt)
         if rhofrag is not None:
             rhofrag = np.ascontiguousarray(rhofrag, dtype=dt)
-    npts = int(np.prod(crho.shape))
-    nfrag = 0 if rhofrag is None else int(rhofrag.shape[0])
     n = np.array([npts, 1, 1], dtype=np.int32)
     o = _lib.NciCut(rhocut, dimcut, rhoplot, (C.c_double * 2)(*srhorange), int(bool(onlyneg)), rho_void)
     flags = _lib.PTR_DEVICE if dev else 0
@@ -220,6 +240,14 @@ def fragment_atoms(x2c, xat, species, coords_ang):
             keep.append(x)
             spc.append(int(sp[i]))
     return np.array(keep).reshape(-1, 3), np.array(spc, dtype=np.int32)
+
+
+def _np_dtype(dt):
+    """numpy dtype of a numpy, DeviceArray or torch array's dtype."""
+    try:
+        return np.dtype(dt)
+    except TypeError:  # a torch dtype
+        return np.dtype(str(dt).replace("torch.", ""))
 
 
 def _out_flags(outs, dtype, size):

@@ -182,3 +182,42 @@ def test_output_memory_and_shape_checks():
         except ValueError:
             continue
         raise AssertionError(f"accepted {outs!r}")
+
+
+def test_cutoff_array_checks():
+    """The cutoff pass reads npts values of every array and masks cgrad in
+    place: a short cgrad, rhoat or rhofrag, a second value type, a mix of
+    host and device arrays, or a missing rhoat where fragments or RHO_VOID
+    need it is refused before the library is called (ctx is never used)."""
+    class Dev:  # stands in for a DeviceArray: an address, a shape and a type
+        def __init__(self, shape, dt):
+            self.shape, self.dtype = tuple(np.atleast_1d(shape)), np.dtype(dt)
+
+        def data_ptr(self):
+            return 1
+
+    n = 10
+    z = np.zeros
+    bad = [
+        dict(crho=z(n), cgrad=z(n - 1)),                                   # short cgrad
+        dict(crho=z(n), cgrad=z(n, dtype=np.float32)),                     # second type
+        dict(crho=z(n), cgrad=z(n), rhoat=z(n - 1), rho_void=1.0),         # short rhoat
+        dict(crho=z(n), cgrad=z(n), rhoat=z(n, dtype=np.float32), rho_void=1.0),
+        dict(crho=z(n), cgrad=z(n), rho_void=1.0),                         # RHO_VOID without rhoat
+        dict(crho=z(n), cgrad=z(n), rhoat=z(n), rhofrag=z((2, n - 1))),   # short fragments
+        dict(crho=z(n), cgrad=z(n), rhoat=z(n), rhofrag=z(2 * n)),        # fragments not 2-D
+        dict(crho=z(n), cgrad=z(n), rhofrag=z((2, n))),                    # fragments without rhoat
+        dict(crho=z(n), cgrad=Dev(n, np.float64)),                         # host + device
+        dict(crho=Dev(n, np.float64), cgrad=Dev(n, np.float64), rhoat=z(n), rho_void=1.0),
+        dict(crho=Dev(n, np.float64), cgrad=Dev(n - 1, np.float64)),
+        dict(crho=Dev(n, np.float32), cgrad=Dev(n, np.float64)),
+        dict(crho=Dev(n, np.float64), cgrad=Dev(n, np.float64), rhoat=Dev(n, np.float64),
+             rhofrag=Dev((3, n), np.float32)),
+        dict(crho=z(n, dtype=np.int32), cgrad=z(n, dtype=np.int32)),      # not a real type
+    ]
+    for kw in bad:
+        try:
+            gn.cutoffs(None, **kw)
+        except ValueError:
+            continue
+        raise AssertionError(f"accepted {kw!r}")
