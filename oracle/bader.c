@@ -55,6 +55,24 @@ typedef struct {
   int64_t nsteps;
 } bstate;
 
+/* Mutation switches (tests only): each bit replaces one rule of the
+ * reference by a plausible mistake, so that a test can show that the pins
+ * of the Bader restatement fail on it.  0 (the default) is the reference. */
+static int orc_mut = 0;
+void oracle_set_mutation(int mask) { orc_mut = mask; }
+
+/* Points re-ascended by each refine_edge call of the last run (the negative
+ * points of its reassign loop, bader@proc.f90:391-417); tests only. */
+#define ORC_MAXPASS 64
+static int64_t orc_reascend[ORC_MAXPASS];
+static int orc_npass = 0;
+int oracle_refine_counts(int64_t *out, int max) {
+  int m = orc_npass < max ? orc_npass : max;
+  for (int q = 0; q < m; q++) out[q] = orc_reascend[q];
+  return orc_npass;
+}
+int oracle_get_mutation(void) { return orc_mut; }
+
 static inline int wrap(int p, int n) {
   /* pbc: repeated +n / -n until 1..n (0..n-1 here) */
   while (p < 0) p += n;
@@ -70,7 +88,8 @@ static inline double rho_val(const bstate *s, int i, int j, int k) {
 static inline int32_t volnum_val(const bstate *s, int i, int j, int k) {
   return s->volnum[lin(s, wrap(i, s->n[0]), wrap(j, s->n[1]), wrap(k, s->n[2]))];
 }
-static inline long fnint(double x) { return lround(x); } /* half away from 0 */
+/* Fortran nint: round half away from 0 (ORC_MUT_RINT: half to even) */
+static inline long fnint(double x) { return (orc_mut & ORC_MUT_RINT) ? lrint(x) : lround(x); }
 
 /* rho_grad_dir, bader@proc.f90:532-567 */
 static void rho_grad_dir(const bstate *s, const int p[3], double res[3]) {
@@ -125,7 +144,7 @@ static void step_ongrid(const bstate *s, int p[3]) {
         int pt0 = p[0] + d1, pt1 = p[1] + d2, pt2 = p[2] + d3;
         double rho_tmp = rho_val(s, pt0, pt1, pt2);
         rho_tmp = rho_ctr + (rho_tmp - rho_ctr) * s->lid[(d1 + 1) + 3 * (d2 + 1) + 9 * (d3 + 1)];
-        if (rho_tmp > rho_max) {
+        if (rho_tmp > rho_max || ((orc_mut & ORC_MUT_ONGRID_GE) && rho_tmp == rho_max && rho_tmp > rho_ctr)) {
           rho_max = rho_tmp;
           pm[0] = pt0; pm[1] = pt1; pm[2] = pt2;
         }
@@ -140,7 +159,7 @@ static void step_neargrid(bstate *s, int p[3]) {
   double g[3];
   int pm[3];
   s->nsteps++;
-  if (s->pnum == 1) { s->dr[0] = s->dr[1] = s->dr[2] = 0.0; }
+  if (s->pnum == 1 || (orc_mut & ORC_MUT_DR_RESET)) { s->dr[0] = s->dr[1] = s->dr[2] = 0.0; }
   rho_grad_dir(s, p, g);
   double gmax = fabs(g[0]);
   if (fabs(g[1]) > gmax) gmax = fabs(g[1]);
@@ -166,7 +185,7 @@ static void step_neargrid(bstate *s, int p[3]) {
   pm[0] = wrap(pm[0], s->n[0]);
   pm[1] = wrap(pm[1], s->n[1]);
   pm[2] = wrap(pm[2], s->n[2]);
-  if (s->known[lin(s, pm[0], pm[1], pm[2])] == 1) {
+  if (s->known[lin(s, pm[0], pm[1], pm[2])] == 1 && !(orc_mut & ORC_MUT_NO_LOOP_CHECK)) {
     pm[0] = p[0]; pm[1] = p[1]; pm[2] = p[2];
     step_ongrid(s, pm);
     s->dr[0] = s->dr[1] = s->dr[2] = 0.0;
@@ -194,7 +213,8 @@ static void max_neargrid(bstate *s, int p[3]) {
     const int *last = &s->path[3 * (s->pnum - 1)];
     if (p[0] == last[0] && p[1] == last[1] && p[2] == last[2]) break;
     path_push(s, p);
-    if (s->known[lin(s, p[0], p[1], p[2])] == 2) break;
+    if (s->known[lin(s, p[0], p[1], p[2])] == 2 && !(orc_mut & ORC_MUT_NO_KNOWN2_STOP)) break;
+    if (orc_mut && s->pnum > 4 * s->N + 16) break; /* a mutant may cycle: stop it */
   }
 }
 
@@ -287,6 +307,7 @@ static int64_t refine_edge(bstate *s, int ref_itrs) {
                   }
                 }
             /* pt keeps its last value, p + (1,1,1) wrapped (bader@proc.f90:359) */
+            if (orc_mut & ORC_MUT_NO_PT_QUIRK) { pt[0] = i; pt[1] = j; pt[2] = k; }
             if (s->known[lin(s, pt[0], pt[1], pt[2])] != -2)
               s->volnum[id] = abs(s->volnum[id]);
           }
@@ -306,6 +327,12 @@ static int64_t refine_edge(bstate *s, int ref_itrs) {
   }
 
   int64_t num_reassign = 0;
+  {
+    int64_t neg = 0;
+    for (int64_t q = 0; q < s->N; q++) neg += s->volnum[q] < 0;
+    if (ref_itrs <= ORC_MAXPASS) orc_reascend[ref_itrs - 1] = neg;
+    orc_npass = ref_itrs;
+  }
   for (int i = 0; i < n1; i++)
     for (int j = 0; j < n2; j++)
       for (int k = 0; k < n3; k++) {
@@ -532,7 +559,7 @@ static int first_sweep(bstate *s, const orc_geom *g, double **xattr, int *xcap) 
         for (int64_t l = 0; l < s->pnum; l++) {
           int pt[3] = {s->path[3 * l], s->path[3 * l + 1], s->path[3 * l + 2]};
           int64_t it = lin(s, pt[0], pt[1], pt[2]);
-          if (s->volnum[it] != -1) s->volnum[it] = pv;
+          if (s->volnum[it] != -1 && !((orc_mut & ORC_MUT_KEEP_LABEL) && s->volnum[it] != 0)) s->volnum[it] = pv;
           if (s->known[it] != 2) s->known[it] = 0;
           assign_surrounding_pts(s, pt);
         }
@@ -560,13 +587,14 @@ static int run_bader(const double *f, const int n[3], const double car2lat[9],
   double *xattr = (double *)malloc(sizeof(double) * 3 * xcap);
 
   int64_t nstarts = 0;
+  orc_npass = 0;
   {
     /* count starts inside first_sweep through nsteps bookkeeping */
     nstarts = first_sweep(&s, geom, &xattr, &xcap);
   }
   int64_t sweep_steps = s.nsteps;
   int status = 0, iters = 0;
-  if (do_refine) {
+  if (do_refine && !(orc_mut & ORC_MUT_NO_REFINE)) {
     int ref_itrs = 1;
     for (;;) {
       int64_t nr = refine_edge(&s, ref_itrs);

@@ -35,6 +35,27 @@ typedef struct {
   double ratom;       /* bohr                                               */
 } orc_geom;
 
+/* Mutation switches of the Bader restatement (tests only; 0 = the
+ * reference's rules).  Each replaces one rule by a plausible mistake:
+ *   DR_RESET       dr reset at every step_neargrid, not only at a path start
+ *   NO_KNOWN2_STOP max_neargrid does not stop at a known == 2 point
+ *   RINT           nint rounds half to even instead of half away from zero
+ *   NO_PT_QUIRK    refine pass >= 2 reads known(p), not known(pt) after the loop
+ *   NO_REFINE      no refine_edge passes
+ *   ONGRID_GE      step_ongrid takes the last of equal maxima (>= not >)
+ *   NO_LOOP_CHECK  step_neargrid does not fall back on an on-grid step when
+ *                  the next point is already on the path (known == 1)
+ *   KEEP_LABEL     a committed path does not relabel already labelled points */
+enum {
+  ORC_MUT_DR_RESET = 1, ORC_MUT_NO_KNOWN2_STOP = 2, ORC_MUT_RINT = 4, ORC_MUT_NO_PT_QUIRK = 8,
+  ORC_MUT_NO_REFINE = 16, ORC_MUT_ONGRID_GE = 32, ORC_MUT_NO_LOOP_CHECK = 64, ORC_MUT_KEEP_LABEL = 128
+};
+void oracle_set_mutation(int mask);
+/* Number of points each refine_edge call of the last Bader run re-ascended
+ * (its negative points); returns the number of calls. */
+int oracle_refine_counts(int64_t *out, int max);
+int oracle_get_mutation(void);
+
 /* Outputs of oracle_bader. */
 typedef struct {
   int nattr;          /* number of attractors (bas%nattr after bader)       */

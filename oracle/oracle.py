@@ -80,8 +80,46 @@ def load():
         lib.oracle_frag_density.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(OrcRgrid), vp]
         lib.oracle_nci_point.argtypes = [C.c_double, C.c_double, vp, C.c_int, vp, vp]
         lib.oracle_slater_grid.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
+        lib.oracle_set_mutation.argtypes = [C.c_int]
+        lib.oracle_refine_counts.argtypes = [C.c_void_p, C.c_int]
         _lib = lib
     return _lib
+
+
+# Mutation switches of the Bader restatement (c2oracle.h): each replaces one
+# rule of bader@proc.f90 by a plausible mistake.  Tests only.
+MUTATIONS = {
+    "dr_reset": 1,        # dr reset at every step (:459-465 resets it only when pnum == 1)
+    "no_known2_stop": 2,  # max_neargrid ignores known == 2 (:447)
+    "rint": 4,            # nint rounds half to even (Fortran nint: half away from zero)
+    "no_pt_quirk": 8,     # refine pass >= 2 reads known(p), not known(pt) after the loop (:359)
+    "no_refine": 16,      # no refine_edge passes (:218-224)
+    "ongrid_ge": 32,      # step_ongrid takes the last of equal maxima (:518 is a strict >)
+    "no_loop_check": 64,  # no on-grid fallback when the next point is on the path (:487-491)
+    "keep_label": 128,    # a committed path keeps earlier labels (:203-206 overwrites them)
+}
+
+
+class mutation:
+    """Context manager: run the Bader restatement with one mutation switched on."""
+
+    def __init__(self, name):
+        self.mask = MUTATIONS[name]
+
+    def __enter__(self):
+        load().oracle_set_mutation(self.mask)
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_set_mutation(0)
+        return False
+
+
+def refine_counts():
+    """Points re-ascended by each refine_edge pass of the last bader() run."""
+    out = np.zeros(64, dtype=np.int64)
+    n = load().oracle_refine_counts(out.ctypes.data, 64)
+    return [int(v) for v in out[:min(n, 64)]]
 
 
 def _matf(m):
