@@ -25,13 +25,17 @@
  * every comparison, rounding and accumulation order is kept.  Compile without
  * FP contraction (-ffp-contract=off): the reference build emits no FMA.
  *
- * Pinning: the grid metrics equal explicit norms and inverses; one maximum
- * gives one basin; a grid-symmetric pair splits evenly; counts sum to N and
- * populations to the grid sum; each atom's nearest grid point carries its
- * own label; the attractor rules follow their hand-worked cases.  The exact
- * order-dependent labels of the serial sweep and refinement are parity
- * unpinned against the reference itself: its Bader golden outputs need
- * input grids that its checkout does not hold.
+ * Pinning (tests/test_oracle_bader_pins.py, tests/test_oracle_ref.py,
+ * tests/test_oracle.py): car2lat equals the reference's own LAPACK
+ * (src/oldlibs dgetrf/dgetri, built unchanged into oracle/_ref) bit for bit;
+ * hand-traced tiny grids decide labels through each order-dependent rule:
+ * the carried dr, nint at +-0.5, the known == 2 stop, relabelling by later
+ * paths, step_ongrid's first-maximum tie rule, refine pass 1 reassignment
+ * and pass 2's known(pt)-after-loop read (through the number of points it
+ * re-ascends); every mutation switch below fails one of them.  Also: one
+ * maximum gives one basin, counts sum to N and populations to the grid
+ * sum, the attractor rules follow hand-worked cases.  The reference's own
+ * Bader golden outputs need input grids its checkout does not hold.
  */
 #include "c2oracle.h"
 #include <math.h>
