@@ -219,22 +219,25 @@ __global__ void __launch_bounds__(256) k_promol_d(Geo G, PromolD T, const double
   if (O.rhoat) ((double *)O.rhoat)[o] = f;
 }
 
-// FP32 promolecular NCI: same sums with per-interval cubic tables.
 // FP32 promolecular NCI at node (i,j,k) from the image list lst[qb..qe)
-// (the atom sum of crystalmod@env.f90:622-753 and nci@proc.f90:565-599).
-__device__ __forceinline__ void promol_f_node(const Geo &G, const PromolF &T, const float4 *__restrict__ imgf,
+// (the atom sum of crystalmod@env.f90:622-753 and nci@proc.f90:565-599),
+// with per-interval cubic tables.  The node-atom difference is taken in
+// FP64 and then rounded: node and image coordinates reach tens of bohr,
+// where an FP32 coordinate is off by up to 4e-6 bohr, a relative density
+// error of about zeta * 4e-6 for an exponential tail; the rounded
+// difference is off by 6e-8 of itself.
+__device__ __forceinline__ void promol_f_node(const Geo &G, const PromolF &T, const double4 *__restrict__ img,
                                               const float *__restrict__ cut2, const uint32_t *__restrict__ lst,
                                               uint64_t qb, uint64_t qe, int i, int j, int k, const NciOut &O) {
   double xcd[3];
   node_cart(G, i, j, k, xcd);
-  const float xc0 = (float)xcd[0], xc1 = (float)xcd[1], xc2 = (float)xcd[2];
   float f = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
   float h00 = 0.f, h11 = 0.f, h22 = 0.f, h01 = 0.f, h02 = 0.f, h12 = 0.f;
   bool isnuc = false;
   for (uint64_t q = qb; q < qe; q++) {
-    const float4 p = imgf[lst[q]];
+    const double4 p = img[lst[q]];
     const int s = (int)p.w;
-    const float x0 = xc0 - p.x, x1 = xc1 - p.y, x2 = xc2 - p.z;
+    const float x0 = (float)(xcd[0] - p.x), x1 = (float)(xcd[1] - p.y), x2 = (float)(xcd[2] - p.z);
     const float rsq = x0 * x0 + x1 * x1 + x2 * x2;
     if (rsq <= 1e-10f) isnuc = true;
     if (rsq > cut2[s]) continue;
@@ -288,7 +291,7 @@ __device__ __forceinline__ void promol_f_node(const Geo &G, const PromolF &T, co
   if (O.rhoat) ((float *)O.rhoat)[o] = f;
 }
 
-__global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const float4 *__restrict__ imgf,
+__global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const double4 *__restrict__ img,
                                                   const float *__restrict__ cut2,
                                                   const uint64_t *__restrict__ off,
                                                   const uint32_t *__restrict__ lst, NciOut O) {
@@ -297,7 +300,7 @@ __global__ void __launch_bounds__(256) k_promol_f(Geo G, PromolF T, const float4
   const int t = threadIdx.x;
   const int i = b1 * BX + (t % BX), j = b2 * BY + ((t / BX) % BY), k = b3 * BZ + t / (BX * BY);
   if (i >= G.n1 || j >= G.n2 || k >= G.n3) return;
-  promol_f_node(G, T, imgf, cut2, lst, off[b], off[b + 1], i, j, k, O);
+  promol_f_node(G, T, img, cut2, lst, off[b], off[b + 1], i, j, k, O);
 }
 
 // ---------------------------------------------------------------- FFT scheme
@@ -502,7 +505,7 @@ static BrickLists build_bricks(c2gpu_ctx *ctx, const Geo &G, const double4 *d_im
   void *t = buffer(ctx, tag + "_bscan", tmp);
   C2_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, cnt, off, nbk + 1, ctx->stream));
   uint64_t total = 0;
-  C2_HIP(hipMemcpyAsync(&total, off + nbk, 8, hipMemcpyDeviceToHost, ctx->stream));
+  copy_to_host(ctx, &total, off + nbk, 8);
   C2_HIP(hipStreamSynchronize(ctx->stream));
   uint32_t *lst = buf<uint32_t>(ctx, tag + "_blst", total > 0 ? total : 1);
   C2_LAUNCH(ctx, k_brick_list, dim3(nbk), dim3(64), 0, G, I, cnt, off, lst);
@@ -551,7 +554,6 @@ struct PromolDev {
   PromolD D;
   PromolF F;
   double4 *img;
-  float4 *imgf;
   double *rcut;
   float *cut2f;
   int nimg;
@@ -676,16 +678,10 @@ static PromolDev upload_promol(c2gpu_ctx *ctx, const c2gpu_promol *pm, const dou
   expand_images(x2c, pm->nat, pm->xat, pm->species, rcmax, pos, isp, iat, lo, hi);
   const int nimg = (int)isp.size();
   std::vector<double4> h_img(nimg > 0 ? nimg : 1);
-  std::vector<float4> h_imgf(nimg > 0 ? nimg : 1);
-  for (int m = 0; m < nimg; m++) {
+  for (int m = 0; m < nimg; m++)
     h_img[m] = make_double4(pos[3 * m], pos[3 * m + 1], pos[3 * m + 2], (double)isp[m]);
-    h_imgf[m] = make_float4((float)pos[3 * m], (float)pos[3 * m + 1], (float)pos[3 * m + 2], (float)isp[m]);
-  }
   P.img = buf<double4>(ctx, "pm_img", h_img.size());
-  P.imgf = buf<float4>(ctx, "pm_imgf", h_imgf.size());
   C2_HIP(hipMemcpyAsync(P.img, h_img.data(), sizeof(double4) * h_img.size(), hipMemcpyHostToDevice, ctx->stream));
-  C2_HIP(hipMemcpyAsync(P.imgf, h_imgf.data(), sizeof(float4) * h_imgf.size(), hipMemcpyHostToDevice,
-                        ctx->stream));
   C2_HIP(hipStreamSynchronize(ctx->stream));
   P.nimg = nimg;
   return P;
@@ -826,7 +822,7 @@ __global__ void k_set_u32(uint32_t *p, uint32_t v) {
 // takes the next brick until the queue is drained; the last take of every
 // workgroup returns a brick number past the end, so all of them exit.
 __global__ void __launch_bounds__(256) k_promol_f_queue(const BatchCell *__restrict__ cells, int ncell,
-                                                        uint32_t nbrick, PromolF T, const float4 *__restrict__ imgf,
+                                                        uint32_t nbrick, PromolF T, const double4 *__restrict__ img,
                                                         const float *__restrict__ cut2,
                                                         const uint64_t *__restrict__ off,
                                                         const uint32_t *__restrict__ lst, uint32_t *queue) {
@@ -843,7 +839,7 @@ __global__ void __launch_bounds__(256) k_promol_f_queue(const BatchCell *__restr
     const int t = threadIdx.x;
     const int i = b1 * BX + (t % BX), j = b2 * BY + ((t / BX) % BY), k = b3 * BZ + t / (BX * BY);
     if (i < C.G.n1 && j < C.G.n2 && k < C.G.n3)
-      promol_f_node(C.G, T, imgf, cut2, lst, off[gb], off[gb + 1], i, j, k, C.O);
+      promol_f_node(C.G, T, img, cut2, lst, off[gb], off[gb + 1], i, j, k, C.O);
   }
 }
 
@@ -872,7 +868,6 @@ void batch_nci_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int nspec
   }
   std::vector<BatchCell> hc;
   std::vector<double4> himg;
-  std::vector<float4> himgf;
   uint64_t nbrick = 0;
   size_t nstage = 0;
   std::vector<size_t> stage_at;
@@ -894,7 +889,6 @@ void batch_nci_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int nspec
     B.brad = brick_radius(B.G);
     for (size_t m = 0; m < isp.size(); m++) {
       himg.push_back(make_double4(pos[3 * m], pos[3 * m + 1], pos[3 * m + 2], (double)isp[m]));
-      himgf.push_back(make_float4((float)pos[3 * m], (float)pos[3 * m + 1], (float)pos[3 * m + 2], (float)isp[m]));
     }
     B.O.is_f32 = 1;
     B.O.zfast = 1;
@@ -921,10 +915,8 @@ void batch_nci_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int nspec
   const int ncs = (int)hc.size();
   BatchCell *d_cells = buf<BatchCell>(ctx, "bq_cells", hc.size());
   double4 *d_img = buf<double4>(ctx, "bq_img", himg.size() > 0 ? himg.size() : 1);
-  float4 *d_imgf = buf<float4>(ctx, "bq_imgf", himgf.size() > 0 ? himgf.size() : 1);
   C2_HIP(hipMemcpyAsync(d_cells, hc.data(), sizeof(BatchCell) * hc.size(), hipMemcpyHostToDevice, st));
   C2_HIP(hipMemcpyAsync(d_img, himg.data(), sizeof(double4) * himg.size(), hipMemcpyHostToDevice, st));
-  C2_HIP(hipMemcpyAsync(d_imgf, himgf.data(), sizeof(float4) * himgf.size(), hipMemcpyHostToDevice, st));
   uint32_t *cnt = buf<uint32_t>(ctx, "bq_bcnt", nbrick + 1);
   uint64_t *off = buf<uint64_t>(ctx, "bq_boff", nbrick + 1);
   C2_HIP(hipMemsetAsync(cnt + nbrick, 0, sizeof(uint32_t), st));
@@ -935,7 +927,7 @@ void batch_nci_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int nspec
   void *t = buffer(ctx, "bq_bscan", tmp);
   C2_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, cnt, off, (int)nbrick + 1, st));
   uint64_t total = 0;
-  C2_HIP(hipMemcpyAsync(&total, off + nbrick, 8, hipMemcpyDeviceToHost, st));
+  copy_to_host(ctx, &total, off + nbrick, 8);
   C2_HIP(hipStreamSynchronize(st));
   uint32_t *lst = buf<uint32_t>(ctx, "bq_blst", total > 0 ? total : 1);
   C2_LAUNCH(ctx, k_brick_list_batch, dim3((unsigned)nbrick), dim3(64), 0, d_cells, ncs, d_img, P.rcut, cnt, off,
@@ -951,7 +943,7 @@ void batch_nci_run(c2gpu_ctx *ctx, int ncell, const c2gpu_cell *cells, int nspec
     C2_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, queue, 0u);  // no memset nodes in graphs
     hipLaunchKernelGGL(k_promol_f_queue, dim3(nwg), dim3(256), 0, st, d_cells, ncs, (uint32_t)nbrick, P.F,
-                       (const float4 *)d_imgf, (const float *)P.cut2f, (const uint64_t *)off,
+                       (const double4 *)d_img, (const float *)P.cut2f, (const uint64_t *)off,
                        (const uint32_t *)lst, queue);
     C2_HIP(hipStreamEndCapture(st, &g));
     C2_HIP(hipGetLastError());
@@ -1023,7 +1015,7 @@ void nci_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
     BrickLists BL = build_bricks(ctx, G, P.img, P.rcut, P.nimg, "pm");
     const int nbk = G.nb1 * G.nb2 * G.nb3;
     if (precision == C2GPU_FP32)
-      C2_LAUNCH(ctx, k_promol_f, dim3(nbk), dim3(256), 0, G, P.F, P.imgf, P.cut2f, BL.off, BL.lst, O);
+      C2_LAUNCH(ctx, k_promol_f, dim3(nbk), dim3(256), 0, G, P.F, P.img, P.cut2f, BL.off, BL.lst, O);
     else
       C2_LAUNCH(ctx, k_promol_d, dim3(nbk), dim3(256), 0, G, P.D, P.img, BL.off, BL.lst, O);
   } else if (scheme == C2GPU_NCI_FFT) {
@@ -1138,7 +1130,7 @@ void nci_promol_nodes_run(c2gpu_ctx *ctx, const c2gpu_nci_nodes *nd, const doubl
   O.rhoat = rhoat_out ? (dev ? rhoat_out : buffer(ctx, "nci_rhoat", osz * N)) : nullptr;
   const int nbk = G.nb1 * G.nb2 * G.nb3;
   if (f32)
-    C2_LAUNCH(ctx, k_promol_f, dim3(nbk), dim3(256), 0, G, P.F, P.imgf, P.cut2f, BL.off, BL.lst, O);
+    C2_LAUNCH(ctx, k_promol_f, dim3(nbk), dim3(256), 0, G, P.F, P.img, P.cut2f, BL.off, BL.lst, O);
   else
     C2_LAUNCH(ctx, k_promol_d, dim3(nbk), dim3(256), 0, G, P.D, P.img, BL.off, BL.lst, O);
   if (!dev) {
@@ -1203,8 +1195,7 @@ __global__ void __launch_bounds__(256) k_frag(Geo G, PromolD D, PromolF F, const
         rho = rho > 0.0 ? rho : 0.0;
         acc = acc + rho;
       } else {
-        const float x0 = (float)xc[0] - (float)at.x, x1 = (float)xc[1] - (float)at.y,
-                    x2 = (float)xc[2] - (float)at.z;
+        const float x0 = (float)(xc[0] - at.x), x1 = (float)(xc[1] - at.y), x2 = (float)(xc[2] - at.z);
         acc += table_rho_f(F, s, sqrtf(x0 * x0 + x1 * x1 + x2 * x2));
       }
     }

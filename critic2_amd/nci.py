@@ -312,3 +312,81 @@ def fragments(ctx: Context, nodes, tables: list[Grid1], frags, precision="fp64",
     ctx.check(rc, "c2gpu_nci_fragments")
     del keep
     return out
+
+
+# ------------------------------------------------------------------ mixed-precision error report
+
+RDG_CONST = 2.0 * (3.0 * np.pi ** 2) ** (1.0 / 3.0)  # nci@proc.f90:91
+RHO_BANDS = ((1e-6, 1e-4), (1e-4, 1e-2), (1e-2, np.inf))
+
+
+def cancellation_bound(s64, rho64, zeta_max, eps=2.0 ** -24):
+    """Relative FP32 error expected of s at each node: every atom's gradient
+    term carries a relative error of about eps, and the terms (of total
+    magnitude up to zeta_max * rho for exponential atomic densities) cancel
+    down to |grad rho| = s * 2(3 pi^2)^(1/3) * rho^(4/3), so the error of the
+    sum relative to it is eps * zeta_max / (s * 2(3 pi^2)^(1/3) * rho^(1/3)),
+    plus eps for rho itself."""
+    return eps * (1.0 + zeta_max / (np.maximum(s64, 1e-300) * RDG_CONST * np.cbrt(rho64)))
+
+
+def precision_report(c32, s32, c64, s64, s_floor=0.1, rho_min=1e-6, zeta_max=None):
+    """Error of the FP32 promolecular NCI outputs against the FP64 ones in
+    nciplot's region s < 2 (nodes with rho > rho_min): max and RMS relative
+    error of rho (|crho|/100), |grad rho| (s * 2(3 pi^2)^(1/3) rho^(4/3), from
+    each path's own rho and s) and s.  Near a critical point of rho the
+    gradient is a sum of atom terms that cancel, so the relative error of s
+    grows as 1/s for any fixed precision; s is therefore reported relative
+    for s_floor <= s < 2 and absolute for s < s_floor; "by_rho" repeats the
+    relative errors per density band (RHO_BANDS).  Host arrays; returns a
+    dict of floats and counts."""
+    c32 = np.asarray(c32, dtype=np.float64).reshape(-1)
+    c64 = np.asarray(c64, dtype=np.float64).reshape(-1)
+    s32 = np.asarray(s32, dtype=np.float64).reshape(-1)
+    s64 = np.asarray(s64, dtype=np.float64).reshape(-1)
+    r32, r64 = np.abs(c32) / 100.0, np.abs(c64) / 100.0
+    reg = (s64 < 2.0) & (r64 > rho_min)
+    out = {"points": int(reg.sum()), "s_floor": s_floor, "rho_min": rho_min}
+    if not out["points"]:
+        return out
+
+    def stats(e):
+        return (float(e.max()), float(np.sqrt(np.mean(e * e)))) if e.size else (0.0, 0.0)
+
+    e_rho = np.abs(r32[reg] - r64[reg]) / r64[reg]
+    g32 = s32[reg] * RDG_CONST * r32[reg] ** (4.0 / 3.0)
+    g64 = s64[reg] * RDG_CONST * r64[reg] ** (4.0 / 3.0)
+    hi = s64[reg] >= s_floor
+    e_g = np.abs(g32 - g64)[hi] / g64[hi]
+    e_s = np.abs(s32[reg] - s64[reg])[hi] / s64[reg][hi]
+    a_s = np.abs(s32[reg] - s64[reg])[~hi]
+    out["rho_max_rel"], out["rho_rms_rel"] = stats(e_rho)
+    out["gradmod_max_rel"], out["gradmod_rms_rel"] = stats(e_g)
+    out["s_max_rel"], out["s_rms_rel"] = stats(e_s)
+    out["s_below_floor_points"] = int(a_s.size)
+    out["s_below_floor_max_abs"] = float(a_s.max()) if a_s.size else 0.0
+    out["sign_agree"] = float(np.mean(np.sign(c32[reg]) == np.sign(c64[reg])))
+    # the same by density band: the gradient's atom terms cancel most where
+    # the density is lowest, so the FP32 error of |grad rho| and s grows there
+    r_reg = r64[reg]
+    bands = [(lo, hi, (r_reg >= lo) & (r_reg < hi)) for lo, hi in RHO_BANDS]
+    out["by_rho"] = []
+    for lo, hi, sel in bands:
+        shi = sel & (s64[reg] >= s_floor)
+        row = {"rho_lo": lo, "rho_hi": hi, "points": int(sel.sum())}
+        row["rho_max_rel"], row["rho_rms_rel"] = stats(e_rho[sel])
+        eg = np.abs(g32[shi] - g64[shi]) / g64[shi]
+        es = np.abs(s32[reg][shi] - s64[reg][shi]) / s64[reg][shi]
+        row["gradmod_max_rel"], row["gradmod_rms_rel"] = stats(eg)
+        row["s_max_rel"], row["s_rms_rel"] = stats(es)
+        out["by_rho"].append(row)
+    if zeta_max is not None:
+        # measured error of s over the cancellation bound (s >= s_floor), and
+        # the largest error where that bound is below 1e-6
+        b = cancellation_bound(s64[reg][hi], r64[reg][hi], zeta_max)
+        ratio = e_s / b
+        out["s_over_bound_max"] = float(ratio.max()) if ratio.size else 0.0
+        well = b < 1e-6
+        out["s_max_rel_well_conditioned"] = float(e_s[well].max()) if well.any() else 0.0
+        out["well_conditioned_points"] = int(well.sum())
+    return out

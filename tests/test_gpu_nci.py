@@ -69,20 +69,25 @@ def test_promol_fp64_matches_oracle(ctx):
 
 
 def test_promol_fp32_error_against_fp64(ctx):
-    """Mixed precision: rho, |grad rho| and s in FP32 against the FP64 path,
-    error reported in the NCI region (s < 2)."""
+    """Mixed precision: rho, |grad rho| and s in FP32 against the FP64 path
+    in the NCI region (s < 2): rho to 1e-5 (max, not RMS); s within a fixed
+    multiple of the cancellation bound (cancellation_bound: the atom terms of
+    the gradient cancel near critical points and in the low-density tail)
+    and within 1e-5 wherever that bound is below 1e-6; below s = 0.1, s
+    absolutely to 1e-5."""
     cell = random_cell(32, (8.0, 8.0, 8.0), (90, 90, 90), 8, (1, 6, 7, 8), (48, 48, 48))
     tables, species = _slater_setup(cell)
     c64, s64 = gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp64")
     c32, s32 = gn.nciplot_promol(ctx, cell.n, cell.x2c, tables, cell.xat, species, precision="fp32")
-    rho64, rho32 = np.abs(c64) / 100, np.abs(c32.astype(np.float64)) / 100
-    reg = (s64 < 2.0) & (rho64 > 1e-6)
-    e_rho = np.abs(rho32 - rho64)[reg] / rho64[reg]
-    e_s = np.abs(s32 - s64)[reg] / s64[reg]
-    print(f"  FP32 vs FP64 in s<2 ({reg.sum()} pts): rho max {e_rho.max():.2e} rms "
-          f"{np.sqrt((e_rho**2).mean()):.2e}; s max {e_s.max():.2e} rms {np.sqrt((e_s**2).mean()):.2e}")
-    assert e_rho.max() < 1e-5
-    assert np.sqrt((e_s ** 2).mean()) < 1e-5
+    r = gn.precision_report(c32, s32, c64, s64, zeta_max=2.0 * 8 ** (1.0 / 3.0))
+    print(f"  FP32 vs FP64 in s<2 ({r['points']} pts): rho max {r['rho_max_rel']:.2e} rms {r['rho_rms_rel']:.2e}; "
+          f"|grad rho| max {r['gradmod_max_rel']:.2e}; s (s>=0.1) max {r['s_max_rel']:.2e} rms "
+          f"{r['s_rms_rel']:.2e}; s<0.1 ({r['s_below_floor_points']} pts) max abs {r['s_below_floor_max_abs']:.2e}")
+    assert r["points"] > 1000
+    assert r["rho_max_rel"] < 1e-5
+    assert r["s_over_bound_max"] < 8.0
+    assert r["s_max_rel_well_conditioned"] < 1e-5
+    assert r["s_below_floor_max_abs"] < 1e-5
 
 
 @pytest.mark.parametrize("grid", [(24, 30, 36), (25, 27, 31), (24, 27, 32)])
