@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -41,6 +42,11 @@ struct c2gpu_ctx {
   // CU of the work-queue NCI kernel (0 = not tuned yet)
   std::vector<c2gpu_ctx *> subs;
   int queue_wg_per_cu = 0;
+  // pinned host staging for the library's own device -> host read-backs
+  // (copy_to_host): the copy engine writes it, so the CPU never reads a
+  // device address itself
+  void *pin = nullptr;
+  size_t pin_bytes = 0;
 };
 
 namespace c2 {
@@ -171,6 +177,46 @@ inline void release_all(c2gpu_ctx *ctx) {
   for (auto &kv : ctx->bufs)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   ctx->bufs.clear();
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
+  ctx->pin = nullptr;
+  ctx->pin_bytes = 0;
+}
+
+// The context's pinned staging buffer, at least `bytes` long (grow-only).
+inline void *pinned(c2gpu_ctx *ctx, size_t bytes) {
+  if (ctx->pin && ctx->pin_bytes >= bytes) return ctx->pin;
+  if (ctx->pin) {
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    C2_HIP(hipHostFree(ctx->pin));
+    ctx->pin = nullptr;
+    ctx->pin_bytes = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  hipError_t e = hipHostMalloc(&ctx->pin, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ctx->pin = nullptr;
+    fail(C2GPU_ERR_ALLOC, "hipHostMalloc(" + std::to_string(want) + " B): " + hipGetErrorString(e));
+  }
+  ctx->pin_bytes = want;
+  return ctx->pin;
+}
+
+// Device -> host copy of the library's own results (counts, offsets, small
+// tables) into any host memory, through the pinned staging buffer in chunks
+// of at most 16 MiB; returns after the data has arrived.
+inline void copy_to_host(c2gpu_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  const size_t chunk = (size_t)16 << 20;
+  char *d = static_cast<char *>(dst);
+  const char *s = static_cast<const char *>(src);
+  void *pin = pinned(ctx, bytes < chunk ? bytes : chunk);
+  for (size_t done = 0; done < bytes;) {
+    const size_t n = bytes - done < chunk ? bytes - done : chunk;
+    C2_HIP(hipMemcpyAsync(pin, s + done, n, hipMemcpyDeviceToHost, ctx->stream));
+    C2_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(d + done, pin, n);
+    done += n;
+  }
+  if (bytes == 0) C2_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 // Runs f(), converting exceptions into a status code and ctx->err.

@@ -458,7 +458,7 @@ void parse_values(c2gpu_ctx *ctx, const uint8_t *base, uint64_t lo, uint64_t hi,
     C2_LAUNCH(ctx, k_tok_parse, dim3((unsigned)nblk), dim3(TOK_BLOCK), 0, base, lo, hi, off, npts, p5, p5s,
               vals, stat, slow, cap);
     add_kernel_bytes(ctx, "k_tok_parse", (double)(hi - lo) + 8.0 * nblk + 8.0 * (double)npts);
-    C2_HIP(hipMemcpyAsync(st, stat, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    copy_to_host(ctx, st, stat, sizeof st);
     C2_HIP(hipStreamSynchronize(ctx->stream));
     if (st[1] <= cap) break;
     cap = st[1];  // room for every slow token, then once more
@@ -513,7 +513,7 @@ void format_body(c2gpu_ctx *ctx, const double *f, const int n[3], int style, int
     C2_LAUNCH(ctx, k_cube_format, dim3(grid_for((int64_t)npts, 256)), dim3(256), 0, zf, npts, L, style, p5, p5s,
               text, stat, slow, cap);
     add_kernel_bytes(ctx, "k_cube_format", 8.0 * (double)npts + (double)body_bytes(n, style));
-    C2_HIP(hipMemcpyAsync(st, stat, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    copy_to_host(ctx, st, stat, sizeof st);
     C2_HIP(hipStreamSynchronize(ctx->stream));
     if (st[1] <= cap) break;
     cap = st[1];
@@ -739,7 +739,7 @@ void cube_parse_run(c2gpu_ctx *ctx, const char *text, uint64_t len, int flags, c
     for (;;) {
       const uint64_t n = want < len ? want : len;
       hb.resize(n);
-      C2_HIP(hipMemcpyAsync(hb.data(), text, n, hipMemcpyDeviceToHost, ctx->stream));
+      copy_to_host(ctx, hb.data(), text, n);
       C2_HIP(hipStreamSynchronize(ctx->stream));
       if (parse_text_header(hb.data(), n, *h, n == len)) break;
       want *= 4;

@@ -29,11 +29,12 @@ struct Timer {
   }
 };
 
+// One value computed on the device, read back through the pinned staging
+// buffer (waits for the stream).
 template <class T>
 inline T read_scalar(c2gpu_ctx *ctx, const T *d) {
   T h;
-  C2_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-  C2_HIP(hipStreamSynchronize(ctx->stream));
+  copy_to_host(ctx, &h, d, sizeof(T));
   return h;
 }
 

@@ -167,7 +167,7 @@ void cut_run(c2gpu_ctx *ctx, uint64_t npts, const T *crho, T *cgrad, const T *rh
     sums[0] = sums[1] = sums[2] = 0;
     if (o.rho_void > 0) {
       std::vector<double> h(3 * (size_t)nb);
-      C2_HIP(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+      copy_to_host(ctx, h.data(), part, h.size() * 8);
       C2_HIP(hipStreamSynchronize(ctx->stream));
       for (unsigned b = 0; b < nb; b++)
         for (int c = 0; c < 3; c++) sums[c] += h[3 * (size_t)b + c];

@@ -132,7 +132,7 @@ void basin_sums_run(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int natt
   C2_LAUNCH(ctx, k_basin_reduce, dim3(grid_for(nb, 256)), dim3(256), 0, partial, nblocks,
                      nb, d_out);
   std::vector<double> h(nb);
-  C2_HIP(hipMemcpyAsync(h.data(), d_out, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+  copy_to_host(ctx, h.data(), d_out, 8 * (size_t)nb);
   C2_HIP(hipStreamSynchronize(st));
   if (counts_out)
     for (int a = 0; a < nattr; a++) counts_out[a] = (int64_t)h[a];

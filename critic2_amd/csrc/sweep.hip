@@ -801,7 +801,7 @@ static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const L
   C2_LAUNCH(ctx, k_ev_sort, dim3(grid_for(n, 256)), dim3(256), 0, n, list, L, uni, longlist, d_n, hugelist,
             d_n + 1, work);
   uint32_t nlh[2];
-  C2_HIP(hipMemcpyAsync(nlh, d_n, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  copy_to_host(ctx, nlh, d_n, 2 * sizeof(uint32_t));
   C2_HIP(hipStreamSynchronize(st));
   if (nlh[0]) C2_LAUNCH(ctx, k_ev_sort_long, dim3(nlh[0]), dim3(256), 0, longlist, L, uni);
   if (nlh[1]) {
@@ -814,7 +814,7 @@ static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const L
     C2_LAUNCH(ctx, k_huge_len, dim3(grid_for((int64_t)nh + 1, 64)), dim3(64), 0, nh, hugelist, L, hlen);
     exclusive_scan(ctx, "huge", hlen, hoff, (int64_t)nh + 1);
     std::vector<uint64_t> ho((size_t)nh + 1);
-    C2_HIP(hipMemcpyAsync(ho.data(), hoff, 8 * ((size_t)nh + 1), hipMemcpyDeviceToHost, st));
+    copy_to_host(ctx, ho.data(), hoff, 8 * ((size_t)nh + 1));
     C2_HIP(hipStreamSynchronize(st));
     std::vector<uint64_t> bo;  // batch-relative segment offsets
     for (uint32_t q0 = 0; q0 < nh;) {
@@ -1019,7 +1019,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     R.events_total += (int64_t)added;
     if (trace) {
       unsigned long long h[5];
-      C2_HIP(hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost));
+      copy_to_host(ctx, h, d_stats, sizeof(h));
       const auto now = std::chrono::steady_clock::now();
       std::fprintf(stderr, "[sweep] it %d %.3f ms\n", it,
                    std::chrono::duration<double, std::milli>(now - t_last).count());
@@ -1050,7 +1050,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     // once (read-modify-write twice), from the host-side launch sizes and the
     // device work counters
     unsigned long long w[16];
-    C2_HIP(hipMemcpy(w, d_work, sizeof(w), hipMemcpyDeviceToHost));
+    copy_to_host(ctx, w, d_work, sizeof(w));
     const double dN = (double)N;
     add_kernel_bytes(ctx, "k_patch_keep", 56.0 * s_nd + 9.0 * w[0] + 8.0 * w[1]);
     add_kernel_bytes(ctx, "k_mark_decide", 6.0 * s_nq + 13.0 * w[3] + 9.0 * w[2]);
