@@ -45,9 +45,13 @@ class C2GpuError(RuntimeError):
         self.code = code
 
 
+# c2gpu_discard_fn: int (*)(const double x[3], void *user)
+DISCARD_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.c_void_p)
+
+
 class AttractorOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("ratom", C.c_double), ("x2c", C.c_double * 9),
-                ("nat", C.c_int), ("xat", C.c_void_p)]
+                ("nat", C.c_int), ("xat", C.c_void_p), ("discard", DISCARD_FN), ("discard_user", C.c_void_p)]
 
 
 class BaderStats(C.Structure):

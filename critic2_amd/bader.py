@@ -42,8 +42,12 @@ def _mode(nnm: bool, ratom, noatoms: bool):
 
 
 def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noatoms=False,
-                    car2lat=None, lat_i_dist=None, labels_out=None) -> BasinData:
+                    car2lat=None, lat_i_dist=None, labels_out=None, discard=None) -> BasinData:
     """Near-grid Bader labelling of the FP64 grid rho (x fastest, n1*n2*n3).
+
+    discard (BADER's DISCARD keyword, bader@proc.f90:185-191): None, or a
+    function of the Cartesian position (bohr) of a maximum that would become
+    a new attractor, true to discard it; its basin keeps label 0.
 
     rho may be a numpy array (host) or a DeviceArray resident on the GPU;
     labels_out (optional) may likewise be a device int32 DeviceArray that
@@ -63,6 +67,10 @@ def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noato
     opts.nat = xat.shape[0]
     xat_flat = np.ascontiguousarray(xat.reshape(-1))
     opts.xat = xat_flat.ctypes.data
+    cb = None
+    if discard is not None:
+        cb = _lib.DISCARD_FN(lambda x, user: 1 if discard(np.array([x[0], x[1], x[2]])) else 0)
+        opts.discard = cb
     nn = np.asarray(n, dtype=np.int32)
     c2l = np.ascontiguousarray(car2lat, dtype=np.float64)
     lid = np.ascontiguousarray(lat_i_dist, dtype=np.float64)

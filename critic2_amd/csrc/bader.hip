@@ -187,7 +187,7 @@ static BParams make_params(const double *d_rho, const int n[3], const double car
 int nearest_atom_label(const c2gpu_attractor_opts *o, const double x[3]);
 // Attractors of maxima taken in discovery order (bader@proc.f90:159-199).
 int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
-                   int32_t *lab, std::vector<double> &xattr);
+                   int32_t *lab, std::vector<double> &xattr, std::vector<int8_t> &discarded);
 
 void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double car2lat[9],
                const double lat_i_dist[27], const c2gpu_attractor_opts *opts, int flags, int32_t *label_out,
@@ -297,7 +297,7 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   uint32_t *d_ends = buf<uint32_t>(ctx, "bader_ends", N);
   const uint32_t nends = compact_flags(ctx, "ends", d_isend, N, d_ends);
   std::vector<uint32_t> h_ends(nends);
-  C2_HIP(hipMemcpyAsync(h_ends.data(), d_ends, sizeof(uint32_t) * nends, hipMemcpyDeviceToHost, st));
+  copy_to_host(ctx, h_ends.data(), d_ends, sizeof(uint32_t) * nends);
   C2_HIP(hipStreamSynchronize(st));
   // fractional position of each maximum, dv = (p-1)/n (bader@proc.f90:161)
   std::vector<double> h_dv(3 * (size_t)nends);
@@ -312,6 +312,7 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
   }
   std::vector<int32_t> h_lab(nends);
   std::vector<double> h_xattr;
+  std::vector<int8_t> h_discarded;  // DISCARD answers per maximum, asked once
   int nbasin = opts->nat;
   uint32_t *d_first = nullptr, *d_key = nullptr;
   // discovery order of the maxima from the earliest start whose path ends at
@@ -322,13 +323,13 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     std::vector<uint32_t> key(nends), order(nends);
     if (nends) {
       C2_LAUNCH(ctx, k_gather_u32, dim3(grid_for(nends, B)), dim3(B), 0, nends, d_ends, d_first, d_key);
-      C2_HIP(hipMemcpyAsync(key.data(), d_key, sizeof(uint32_t) * nends, hipMemcpyDeviceToHost, st));
+      copy_to_host(ctx, key.data(), d_key, sizeof(uint32_t) * nends);
     }
     C2_HIP(hipStreamSynchronize(st));
     for (uint32_t q = 0; q < nends; q++) order[q] = q;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
     std::vector<int32_t> lab(nends);
-    nbasin = attractor_rule(opts, nends, order.data(), h_dv.data(), lab.data(), h_xattr);
+    nbasin = attractor_rule(opts, nends, order.data(), h_dv.data(), lab.data(), h_xattr, h_discarded);
     if (nbasin >= (1 << 20)) fail(C2GPU_ERR_ARG, "c2gpu_bader: too many attractors");
     return lab;
   };

@@ -173,11 +173,16 @@ static int identify_atom_within(const c2gpu_attractor_opts *o, const double x[3]
 // Attractor of each new maximum, taken in discovery order
 // (bader@proc.f90:159-199): an atom within ratom (if atoms exist), else a
 // known attractor within ratom (are_lclose, crystalmod@proc.f90:1118-1133),
-// else a new attractor numbered nbasin+1 at the maximum.  dv: fractional
-// positions of the maxima; order: their discovery order.  Fills lab (per
-// maximum) and xattr (3 per attractor); returns the attractor count.
+// else, unless the DISCARD callback rejects it (:185-191, at the Cartesian
+// position x2c*dv), a new attractor numbered nbasin+1 at the maximum; a
+// rejected maximum gets label 0.  dv: fractional positions of the maxima;
+// order: their discovery order; discarded: per maximum, -1 until the
+// callback was asked, then its answer (kept across calls, so the callback
+// runs at most once per maximum).  Fills lab (per maximum) and xattr (3 per
+// attractor); returns the attractor count.
 int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
-                   int32_t *lab, std::vector<double> &xattr) {
+                   int32_t *lab, std::vector<double> &xattr, std::vector<int8_t> &discarded) {
+  if (discarded.size() != nmax) discarded.assign(nmax, -1);
   const bool atexist = o->mode != C2GPU_ATTR_NOATOMS;
   const double ratom = o->mode == C2GPU_ATTR_NEAREST_ATOM ? 1e40 : o->ratom;
   xattr.clear();
@@ -200,12 +205,42 @@ int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t 
         }
       }
     }
+    if (pv == 0 && o->discard) {
+      if (discarded[m] < 0) {
+        double xc[3];
+        mat3_apply(o->x2c, x, xc);  // x = s%c%x2c(dv)
+        discarded[m] = o->discard(xc, o->discard_user) != 0;
+      }
+      if (discarded[m]) {
+        lab[m] = 0;
+        continue;
+      }
+    }
     if (pv == 0) {
       nbasin++;
       xattr.insert(xattr.end(), x, x + 3);
       pv = nbasin;
     }
     lab[m] = pv;
+  }
+  // The reference evaluates DISCARD each time a path ends at a maximum that
+  // is still unlabelled: a discarded maximum within ratom of an attractor
+  // created after it would join that attractor on its later visits (a label
+  // that changes during the sweep).  Refused rather than approximated.
+  if (o->discard && ratom > 1e-40) {
+    const int first_new = atexist ? o->nat : 0;
+    for (uint32_t q = 0; q < nmax; q++) {
+      const uint32_t m = order[q];
+      if (lab[m] != 0) continue;
+      const double *x = dv + 3 * (size_t)m;
+      for (int l = first_new; l < nbasin; l++) {
+        const double xd[3] = {x[0] - xattr[3 * l], x[1] - xattr[3 * l + 1], x[2] - xattr[3 * l + 2]};
+        if (lattice_dist(o->x2c, xd) < ratom)
+          fail(C2GPU_ERR_UNSUPPORTED,
+               "c2gpu_bader: a discarded maximum lies within ratom of an attractor found after it (the "
+               "reference would assign its later paths to that attractor)");
+      }
+    }
   }
   return nbasin;
 }

@@ -42,7 +42,7 @@
 namespace c2 {
 
 int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
-                   int32_t *lab, std::vector<double> &xattr);
+                   int32_t *lab, std::vector<double> &xattr, std::vector<int8_t> &discarded);
 int ws_facets(const double m[9], int vec[3 * 14], double area[14]);
 void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const double *const *, int, double *,
                     int64_t *);
@@ -583,7 +583,7 @@ struct FrontCounts {
   }
   int finish(int upto, uint64_t &sum) {
     std::vector<uint32_t> h((size_t)upto + 1);
-    C2_HIP(hipMemcpyAsync(h.data(), d, 4 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    copy_to_host(ctx, h.data(), d, 4 * h.size());
     C2_HIP(hipStreamSynchronize(ctx->stream));
     int z = 0;
     sum = 0;
@@ -598,6 +598,9 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
             int nfields, const double *const *fields, int flags, int32_t *label_out, int *nattr_out,
             double *xattr_out, int xattr_cap, double *vol_out, double *sums_out, c2gpu_yt_stats *stats) {
   if (!rho || !n || !x2c || !opts || !nattr_out || !xattr_out || !vol_out) fail(C2GPU_ERR_ARG, "c2gpu_yt: NULL argument");
+  // DISCARD in yt_integrate (yt@proc.f90:153-167) turns the points below a
+  // discarded maximum into surface points; not built here
+  if (opts->discard) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: the DISCARD callback is not supported");
   if (nfields < 0 || nfields > 4 || (nfields > 0 && (!fields || !sums_out)))
     fail(C2GPU_ERR_ARG, "c2gpu_yt: 0 <= nfields <= 4, with fields and sums_out");
   for (int a = 0; a < 3; a++)
@@ -665,8 +668,8 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   if (nmax) {
     double *d_gm = buf<double>(ctx, "yt_gmax", nmax);
     C2_LAUNCH(ctx, k_yt_gather, dim3(grid_for(nmax, YB)), dim3(YB), 0, nmax, maxl, g, d_gm);
-    C2_HIP(hipMemcpyAsync(h_max.data(), maxl, 4 * (size_t)nmax, hipMemcpyDeviceToHost, st));
-    C2_HIP(hipMemcpyAsync(h_gmax.data(), d_gm, 8 * (size_t)nmax, hipMemcpyDeviceToHost, st));
+    copy_to_host(ctx, h_max.data(), maxl, 4 * (size_t)nmax);
+    copy_to_host(ctx, h_gmax.data(), d_gm, 8 * (size_t)nmax);
     C2_HIP(hipStreamSynchronize(st));
   }
   std::vector<uint32_t> order(nmax);
@@ -686,7 +689,8 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   }
   std::vector<int32_t> mlab(nmax);
   std::vector<double> xattr;
-  const int nattr = attractor_rule(opts, nmax, order.data(), dv.data(), mlab.data(), xattr);
+  std::vector<int8_t> discarded;
+  const int nattr = attractor_rule(opts, nmax, order.data(), dv.data(), mlab.data(), xattr, discarded);
   if (nattr > xattr_cap) fail(C2GPU_ERR_ARG, "c2gpu_yt: xattr_cap too small for " + std::to_string(nattr));
   if (nattr < 1) fail(C2GPU_ERR_ARG, "c2gpu_yt: no attractors");
   S.nmaxima = (int)nmax;
@@ -793,7 +797,7 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
                                                    12.0 * (double)read_scalar(ctx, W.top));
   }
   std::vector<double> h(nb);
-  C2_HIP(hipMemcpyAsync(h.data(), acc, 8 * nb, hipMemcpyDeviceToHost, st));
+  copy_to_host(ctx, h.data(), acc, 8 * nb);
   if (label_out && !dev) C2_HIP(hipMemcpyAsync(label_out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
   C2_HIP(hipStreamSynchronize(st));
   for (int a = 0; a < nattr; a++) vol_out[a] = (double)counts[a] + h[a];

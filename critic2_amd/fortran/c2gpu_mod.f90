@@ -4,7 +4,8 @@
 ! lat_i_dist(-1:1,-1:1,-1:1).  Every function returns 0 on success; the
 ! caller turns a non-zero status into ferror(...,faterr).
 module c2gpu
-  use iso_c_binding, only: c_int, c_int32_t, c_int64_t, c_double, c_ptr, c_char
+  use iso_c_binding, only: c_int, c_int32_t, c_int64_t, c_double, c_ptr, c_char, c_funptr, c_null_funptr, &
+     c_null_ptr, c_null_char, c_associated, c_f_pointer
   implicit none
   private
 
@@ -22,6 +23,11 @@ module c2gpu
      real(c_double) :: x2c(9)
      integer(c_int) :: nat
      type(c_ptr) :: xat
+     ! DISCARD: a bind(c) function  integer(c_int) f(x, user)  with
+     ! real(c_double) :: x(3) (Cartesian, bohr) and type(c_ptr), value :: user,
+     ! nonzero to discard the new maximum; c_null_funptr for none
+     type(c_funptr) :: discard = c_null_funptr
+     type(c_ptr) :: discard_user = c_null_ptr
   end type c2gpu_attractor_opts
 
   type, bind(c), public :: c2gpu_bader_stats
@@ -296,6 +302,34 @@ module c2gpu
        real(c_double), intent(out) :: mpole_out(*)
        integer(c_int) :: c2gpu_multipoles
      end function c2gpu_multipoles
+
+     function c2gpu_ctx_error(ctx) bind(c, name="c2gpu_ctx_error")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr) :: c2gpu_ctx_error
+     end function c2gpu_ctx_error
   end interface
+
+  public :: c2gpu_error_message
+
+contains
+
+  ! The context's last error message (c2gpu_ctx_error) as a Fortran string,
+  ! for ferror.
+  subroutine c2gpu_error_message(ctx, msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(out) :: msg
+    character(kind=c_char), pointer :: cs(:)
+    type(c_ptr) :: p
+    integer :: i
+    msg = ""
+    p = c2gpu_ctx_error(ctx)
+    if (.not.c_associated(p)) return
+    call c_f_pointer(p, cs, (/len(msg)/))
+    do i = 1, len(msg)
+       if (cs(i) == c_null_char) exit
+       msg(i:i) = cs(i)
+    end do
+  end subroutine c2gpu_error_message
 
 end module c2gpu

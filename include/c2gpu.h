@@ -112,12 +112,24 @@ enum {
   C2GPU_ATTR_RATOM = 1,        /* NNM / RATOM r                            */
   C2GPU_ATTR_NOATOMS = 2       /* NOATOMS                                  */
 };
+/* BADER's DISCARD expression (bader@proc.f90:185-191) as a caller
+ * callback: given the Cartesian position x (bohr, x2c times the fractional
+ * position of the grid maximum), return nonzero to discard the maximum (the
+ * expression's value is not below 1d-30 in magnitude).  It is consulted only
+ * for maxima that would become new attractors (not an atom within ratom nor
+ * an earlier attractor within ratom); a discarded maximum gets no attractor
+ * and the points whose paths end there keep label 0, as in the reference.
+ * It may be called once or more per maximum and must depend on x only. */
+typedef int (*c2gpu_discard_fn)(const double x[3], void *user);
+
 typedef struct {
   int mode;           /* C2GPU_ATTR_*                                      */
   double ratom;       /* bohr (ignored for NEAREST_ATOM)                   */
   double x2c[9];      /* lattice vectors as columns (bohr), crystal m_x2c  */
   int nat;            /* atoms of the complete cell list (cpcel order)     */
   const double *xat;  /* fractional atom coordinates, 3*nat, host memory   */
+  c2gpu_discard_fn discard; /* DISCARD filter, or NULL (none)              */
+  void *discard_user;       /* passed to discard                           */
 } c2gpu_attractor_opts;
 
 typedef struct {
@@ -141,10 +153,10 @@ typedef struct {
  * Attractors of new maxima follow bader@proc.f90:159-199 in every mode: the
  * nearest atom (default), or an atom within ratom, else an earlier attractor
  * within ratom, else a new one (NNM/RATOM: atoms first; NOATOMS: none), the
- * new ones numbered in the serial sweep's discovery order.  If *nattr exceeds
- * xattr_cap only the first xattr_cap positions are written.  The DISCARD
- * expression filter has no counterpart here (the caller keeps critic2's own
- * path for it).  Limits: every grid edge >= 3 points, fewer than 2^32 - 1
+ * new ones numbered in the serial sweep's discovery order; with a discard
+ * callback (DISCARD), a maximum it rejects gets no attractor and its basin
+ * keeps label 0.  If *nattr exceeds xattr_cap only the first xattr_cap
+ * positions are written (call again with a larger xattr_out).  Limits: every grid edge >= 3 points, fewer than 2^32 - 1
  * points, and ascent paths of at most 2048 points (a longer one returns
  * C2GPU_ERR_PATHLEN; the reference's path array grows without bound). */
 int c2gpu_bader(c2gpu_ctx *ctx, const double *rho, const int n[3],
@@ -168,6 +180,7 @@ int c2gpu_relabel(c2gpu_ctx *ctx, int64_t npts, const int32_t *map, int nmap,
 int c2gpu_basin_sums(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nattr,
                      int nfields, const double *const *fields, int flags,
                      double *sums_out, int64_t *counts_out);
+
 
 /* ------------------------------------------------------------------ */
 /* FFT derivative grids and NCI                                        */

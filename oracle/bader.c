@@ -551,13 +551,23 @@ static int first_sweep(bstate *s, const orc_geom *g, double **xattr, int *xcap) 
               if (are_lclose(g, dv, &(*xattr)[3 * l], ratom)) { pv = l + 1; assigned = 1; break; }
           }
           if (!assigned) {
-            s->nbasin++;
-            if (s->nbasin > *xcap) {
-              *xcap = 2 * s->nbasin;
-              *xattr = (double *)realloc(*xattr, sizeof(double) * 3 * (*xcap));
+            /* DISCARD: the expression at x = s%c%x2c(dv), evaluated each
+             * time a path ends at this (still unlabelled) maximum */
+            int ok = 1;
+            if (g->discard) {
+              double xc[3];
+              x2c_apply(g->x2c, dv, xc);
+              ok = !g->discard(xc, g->discard_user);
             }
-            pv = s->nbasin;
-            for (int c = 0; c < 3; c++) (*xattr)[3 * (s->nbasin - 1) + c] = dv[c];
+            if (ok) {
+              s->nbasin++;
+              if (s->nbasin > *xcap) {
+                *xcap = 2 * s->nbasin;
+                *xattr = (double *)realloc(*xattr, sizeof(double) * 3 * (*xcap));
+              }
+              pv = s->nbasin;
+              for (int c = 0; c < 3; c++) (*xattr)[3 * (s->nbasin - 1) + c] = dv[c];
+            }
           }
         }
         for (int64_t l = 0; l < s->pnum; l++) {

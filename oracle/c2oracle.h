@@ -33,6 +33,10 @@ typedef struct {
   const double *xat;  /* fractional atom coordinates, 3*nat                 */
   int mode;           /* ORC_ATTR_*                                         */
   double ratom;       /* bohr                                               */
+  /* DISCARD (bader@proc.f90:185-191): nonzero discards the new maximum at
+   * Cartesian x; NULL: no expression */
+  int (*discard)(const double x[3], void *user);
+  void *discard_user;
 } orc_geom;
 
 /* Mutation switches of the Bader restatement (tests only; 0 = the
@@ -126,6 +130,11 @@ void oracle_grid1_build(int ngrid, const double *r, const double *rr0,
 void oracle_promol(const double xc[3], int nimg, const double *ximg,
                    const int *zidx, const orc_rgrid *tabs, const double *cut,
                    int nder, double *f, double g[3], double h[9]);
+/* oracle_promol + oracle_nci_point over npts nodes, OpenMP schedule(dynamic)
+ * like nciplot's loop; out: (rho, |grad|, s, crho) per node. */
+void oracle_promol_nci_batch(int64_t npts, const double *pts, int nimg, const double *ximg,
+                             const int *zidx, const orc_rgrid *tabs, const double *cut,
+                             int full_hessian, double *out);
 
 /* Fragment densities at Cartesian point xc (nci@proc.f90:580-596): for
  * each of nfrag fragments (fstart[f] .. fstart[f+1]-1 index its atoms,

@@ -43,9 +43,12 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
+DISCARD_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.c_void_p)
+
+
 class OrcGeom(C.Structure):
     _fields_ = [("x2c", C.c_double * 9), ("nat", C.c_int), ("xat", C.c_void_p), ("mode", C.c_int),
-                ("ratom", C.c_double)]
+                ("ratom", C.c_double), ("discard", DISCARD_FN), ("discard_user", C.c_void_p)]
 
 
 class OrcInfo(C.Structure):
@@ -135,9 +138,11 @@ def metrics(x2c, n):
     return l2c, c2l, lid
 
 
-def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, lat_i_dist=None):
+def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, lat_i_dist=None, discard=None):
     """Serial bader_integrate; returns (labels int32 x-fastest, info dict,
-    xattr (nattr,3), known or None)."""
+    xattr (nattr,3), known or None).  discard: None or a function of the
+    Cartesian position (bohr) of a new maximum, true to discard it (BADER's
+    DISCARD expression, bader@proc.f90:185-191)."""
     lib = load()
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
     nn = np.asarray(n, dtype=np.int32)
@@ -153,6 +158,10 @@ def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, la
     g.xat = xa.ctypes.data
     g.mode = mode
     g.ratom = ratom
+    cb = None
+    if discard is not None:
+        cb = DISCARD_FN(lambda x, user: 1 if discard(np.array([x[0], x[1], x[2]])) else 0)
+        g.discard = cb
     lab = np.zeros(f.size, dtype=np.int32)
     cap = 4 * g.nat + 256
     xattr = np.zeros(3 * cap)
@@ -230,6 +239,27 @@ def promol_nci(points_cart, images, img_species, tables, full_hessian=True):
         gm = 0.0 if d <= 1e-5 else float(np.sqrt((g * g).sum()))
         lib.oracle_nci_point(f.value, gm, h.ctypes.data, 1 if full_hessian else 0, C.byref(cr), C.byref(s))
         out[q] = (f.value, gm, s.value, cr.value)
+    return out
+
+
+def promol_nci_batch(points_cart, images, img_species, tables, threads=None):
+    """promol_nci over many nodes in C, OpenMP-parallel with schedule(dynamic)
+    as nciplot's loop (nci@proc.f90:543); threads: OMP threads (None = the
+    OpenMP default).  Returns (npts, 4): rho, |grad|, s, crho."""
+    lib = load()
+    lib.oracle_promol_nci_batch.argtypes = [C.c_int64, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.POINTER(OrcRgrid), C.c_void_p, C.c_int, C.c_void_p]
+    tabs = (OrcRgrid * len(tables))(*[t.s for t in tables])
+    cut = np.array([min(t.cutrad, t.s.rmax) for t in tables], dtype=np.float64)
+    ximg = np.ascontiguousarray(np.asarray(images, dtype=np.float64).reshape(-1))
+    zi = np.ascontiguousarray(np.asarray(img_species, dtype=np.int32))
+    pts = np.ascontiguousarray(np.asarray(points_cart, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros((len(pts), 4))
+    if threads is not None:
+        omp = C.CDLL("libgomp.so.1")
+        omp.omp_set_num_threads(int(threads))
+    lib.oracle_promol_nci_batch(len(pts), pts.ctypes.data, len(zi), ximg.ctypes.data, zi.ctypes.data, tabs,
+                                cut.ctypes.data, 1, out.ctypes.data)
     return out
 
 

@@ -66,6 +66,33 @@ def test_bader_attractor_rules_follow_discovery_order(ctx, rule, ratom):
         assert bas.nattr > len(cell.zat)  # maxima off the nuclei became non-nuclear attractors
 
 
+@pytest.mark.parametrize("rule,ratom,side", [("noatoms", 0.05, 0), ("noatoms", 0.05, 1), ("nnm", 0.1, 2),
+                                             ("nnm", 0.1, 0)])
+def test_bader_discard_matches_oracle(ctx, rule, ratom, side):
+    """DISCARD (bader@proc.f90:185-191): maxima that would become new
+    attractors are offered to the expression (here: a half-space of the
+    cell, by Cartesian coordinate `side`); discarded ones get no attractor
+    and their basins keep label 0.  Labels, attractors and positions equal
+    the oracle's, which evaluates the expression at every visit."""
+    cell = random_cell(13, (8.0, 9.0, 10.0), (85, 95, 100), 5, (1, 6, 7, 8), (32, 36, 40))
+    print(f"[case] discard {rule} ratom={ratom} side={side} n={cell.n}", flush=True)
+    f = _grid(ctx, cell)
+    mid = 0.5 * np.asarray(cell.x2c).sum(1)[side]
+
+    def discard(x):
+        return x[side] < mid
+
+    omode = _lib.ATTR_RATOM if rule == "nnm" else _lib.ATTR_NOATOMS
+    ref, info, xattr, _ = oracle.bader(f, cell.n, cell.x2c, cell.xat, mode=omode, ratom=ratom, discard=discard)
+    kw = dict(nnm=True, ratom=ratom) if rule == "nnm" else dict(noatoms=True, ratom=ratom)
+    bas = gb.bader_integrate(ctx, f, cell.n, cell.x2c, cell.xat, discard=discard, **kw)
+    assert int(np.count_nonzero(bas.idg != ref)) == 0
+    assert bas.nattr == info["nattr"]
+    np.testing.assert_array_equal(bas.xattr, xattr)
+    if rule == "noatoms":
+        assert np.count_nonzero(ref == 0) > 0  # some basins were discarded
+
+
 @pytest.mark.parametrize("mode", ["C2GPU_SWEEP_VERIFY", "C2GPU_SWEEP_FULL"])
 def test_sweep_incremental_equals_full_rebuild(ctx, mode, monkeypatch):
     """The incremental fixpoint steps against full rebuilds: VERIFY rebuilds

@@ -46,6 +46,53 @@ def test_fortran_host_bader(ctx, tmp_path):
     np.testing.assert_allclose(rows[:, 2], sums[0] * w, rtol=1e-10)
 
 
+def _write_case(path, cell):
+    with open(path, "w") as fh:
+        fh.write(" ".join(str(v) for v in cell.n) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in np.asarray(cell.x2c).reshape(-1, order="F")) + "\n")
+        fh.write(f"{len(cell.zat)}\n")
+        for x in cell.xat:
+            fh.write(" ".join(repr(float(v)) for v in x) + "\n")
+
+
+@pytest.mark.parametrize("mode,ratom,discard", [(1, 0.1, None), (2, 0.05, 0.5), (1, 0.1, 0.4)])
+def test_fortran_host_attractor_rules_and_discard(ctx, tmp_path, mode, ratom, discard):
+    """NNM with more attractors than atoms (the host's attractor array starts
+    at nat entries and grows from the count the library reports, as the
+    bader@c2gpu submodule does), and DISCARD through a bind(c) callback
+    (new maxima with Cartesian x below a fraction of the cell rejected):
+    labels, attractor count and positions equal the oracle's."""
+    if not os.path.exists(cb.FORTRAN_HOST):
+        pytest.skip("Fortran stand-in host not built (amdflang absent at build time)")
+    cell = random_cell(53, (8.0, 9.0, 10.0), (85, 95, 100), 5, (1, 6, 7, 8), (32, 36, 40))
+    print(f"[case] fortran host mode {mode} ratom {ratom} discard {discard}", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    _write_case(tmp_path / "case.txt", cell)
+    f.astype(np.float64).tofile(tmp_path / "grid.bin")
+    args = [cb.FORTRAN_HOST, str(tmp_path / "case.txt"), str(tmp_path / "grid.bin"), str(tmp_path / "lab.bin"),
+            str(tmp_path / "props.txt"), str(mode), repr(ratom)]
+    xlim = None
+    if discard is not None:
+        xlim = discard * float(np.asarray(cell.x2c)[0].sum())
+        args.append(repr(xlim))
+    r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
+    out = r.stdout.decode(errors="replace")
+    assert r.returncode == 0, out
+    lab = np.fromfile(tmp_path / "lab.bin", dtype=np.int32)
+    ref, info, xattr, _ = oracle.bader(f, cell.n, cell.x2c, cell.xat, mode=mode, ratom=ratom,
+                                       discard=(lambda x: x[0] < xlim) if xlim is not None else None)
+    assert int(np.count_nonzero(lab != ref)) == 0
+    lines = open(tmp_path / "props.txt").read().splitlines()
+    head = lines[0].split()
+    assert int(head[1]) == info["nattr"]
+    rows = np.array([[float(v) for v in ln.split()] for ln in lines[1:]]).reshape(-1, 7)
+    np.testing.assert_array_equal(rows[:, 4:7], xattr)
+    if mode == 1 and discard is None:
+        assert info["nattr"] > len(cell.zat) and int(head[-1]) == info["nattr"]  # the array was grown
+    if discard is not None and mode == 2:
+        assert np.count_nonzero(ref == 0) > 0
+
+
 def test_fortran_host_reads_cube(ctx, tmp_path):
     """The Fortran host loading its grid through c2gpu_cube_read (the
     grid3%read_cube drop-in): the grid it labels is the cube's values, so

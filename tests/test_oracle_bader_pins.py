@@ -268,3 +268,49 @@ def test_each_mutation_fails_a_pin(name):
 def test_pins_pass_without_mutation():
     for pin in PINS:
         pin()
+
+
+# ---------------------------------------------------------------- DISCARD (bader@proc.f90:185-191)
+
+def _run_noatoms(rows, discard):
+    f, n = _grid(rows)
+    x2c = np.diag([float(n[0]), 1.0, float(n[2])])
+    lab, info, xattr, _ = oracle.bader(f, n, x2c, np.zeros((0, 3)), mode=2, ratom=1e-3, discard=discard)
+    L = lab.reshape((n[0], n[2]), order="F")
+    return {(i, k): int(L[i, k]) for i in range(n[0]) for k in range(n[2])}, info, xattr
+
+
+def test_discard_keeps_rejected_basin_unlabelled():
+    """NOATOMS: the maxima A = (0,6) and B = (3,6) of the RAMP grid are new
+    attractors in discovery order.  The first start's path ends at B (see
+    test_ramp_first_path_carries_dr), so B is attractor 1.  Discarding A
+    (the expression is evaluated at x2c*dv = (0, 0, 6) bohr) leaves every
+    path that ends there with label 0: one attractor, A's maximum 0, (0,0)
+    in B."""
+    calls = []
+
+    def at_a(x):
+        calls.append(tuple(x))
+        return abs(x[0]) < 1e-9 and abs(x[2] - 6.0) < 1e-9
+
+    L, info, xattr = _run_noatoms(RAMP, at_a)
+    assert info["nattr"] == 1
+    assert np.allclose(xattr[0], [3 / 6, 0.0, 6 / 8])
+    assert L[(3, 6)] == 1 and L[(0, 6)] == 0 and L[(0, 0)] == 1
+    assert (0.0, 0.0, 6.0) in calls and (3.0, 0.0, 6.0) in calls
+    # without the expression both maxima are attractors, B first
+    L2, info2, xattr2 = _run_noatoms(RAMP, None)
+    assert info2["nattr"] == 2 and L2[(3, 6)] == 1 and L2[(0, 6)] == 2
+    # a point left at 0 belonged to A's basin without the expression
+    for p, v in L.items():
+        if v == 0:
+            assert L2[p] == 2, p
+
+
+def test_discard_first_maximum_renumbers_the_next():
+    """Discarding B, the first path's maximum, leaves that path at label 0
+    and makes A (found later) attractor 1."""
+    L, info, xattr = _run_noatoms(RAMP, lambda x: abs(x[0] - 3.0) < 1e-9 and abs(x[2] - 6.0) < 1e-9)
+    assert info["nattr"] == 1
+    assert np.allclose(xattr[0], [0.0, 0.0, 6 / 8])
+    assert L[(3, 6)] == 0 and L[(0, 6)] == 1
