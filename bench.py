@@ -33,14 +33,40 @@ from critic2_amd.radial import slater_grid1  # noqa: E402
 from critic2_amd.synth import config  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+# Compulsory bytes per grid point of each kernel family (the roofline
+# table of the path): the Bader kernels read the FP64 grid once and write an
+# int32 label (12 B); the per-basin sums read a label and each field (4 + 8
+# per field); NCI from the FP64 grid reads it and writes two FP32 values
+# (16 B); the FP32 promolecular evaluator writes two FP32 values (8 B, but it
+# is VALU-bound); the whole Bader + NCI step is 20 B a point.
+STEP_BYTES_PER_POINT = 20
+PROMOL_KERNELS = ("k_promol_f", "k_promol_d", "k_promol_f_queue")
+SUMS_KERNELS = ("k_basin_partials", "k_basin_reduce")
+NCI_GRID_KERNELS = ("k_nci_fft_point", "k_spectral", "k_real_to_complex", "k_accum_sq", "k_count_pos",
+                    "k_real_part", "k_sqrt", "k_nci_grid")
+
+
+def compulsory_bytes_per_point(name, nfields=1):
+    if name in PROMOL_KERNELS:
+        return 8
+    if name in SUMS_KERNELS:
+        return 4 + 8 * nfields
+    if name in NCI_GRID_KERNELS:
+        return 16
+    return 12  # a Bader kernel
 
 
 def kernel_bytes(name, N, ev_per_launch, traj_points, nfields=1):
-    """Algorithmic bytes per launch of the main kernels: each array element
-    the kernel must read or write, counted once (atomics count read+write)."""
-    E = ev_per_launch
+    """The kernel's own algorithmic bytes per launch (each array element the
+    kernel must read or write in its design, counted once) for the full-grid
+    kernels whose launches the library does not report itself."""
     table = {
-        "k_traj": 12 * N + 4 * traj_points,          # grid once, position+length, the path points
+        # step record of each point (32 B), path length and pool position
+        # written (4 + 8 B), every path point written (4 B)
+        "k_traj": 44 * N + 4 * traj_points,
         "k_traj_reorder": 16 * N + 8 * traj_points,  # offsets; path points read and written
         "k_promol_f": 8 * N,                         # FP32 crho + cgrad
         "k_promol_d": 16 * N,
@@ -48,6 +74,18 @@ def kernel_bytes(name, N, ev_per_launch, traj_points, nfields=1):
         "k_reassign": 8 * N,
     }
     return table.get(name)
+
+
+def measured_flops(kernel):
+    """FLOPs per launch of `kernel` from a committed rocprofv3 PMC summary
+    (the `flops_per_launch` lines of profiles/*_valu.md), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_valu.md"))):
+        for line in open(path):
+            t = line.split()
+            if len(t) == 3 and t[0] == "flops_per_launch" and t[1] == kernel:
+                return int(t[2])
+    return None
 
 
 def measured_traffic(kernel):
@@ -117,24 +155,49 @@ def run_host_buffers(ctx, args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(ctx, cell, tables, species):
-    """The oracle (scalar C restatement of critic2's algorithm, 1 thread) on a
-    bounded sample of the same workload: the Bader sweep+refine on the same
-    cell at 128^3, and the promolecular NCI at 4096 nodes of the 512^3 grid.
-    Returns Gpoints/s for Bader+NCI per point."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def host_threads():
+    """Host threads the CPU leg may use: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(ctx, cell, tables, species, n_bader=256):
+    """The oracle (C restatement of critic2's algorithm) on a bounded sample
+    of the same workload, with the reference's parallel structure: the
+    Bader sweep and refinement serially (bader@proc.f90:151-224 has no
+    OpenMP) on the same cell at n_bader^3, and the promolecular NCI point
+    loop OpenMP-parallel with schedule(dynamic) (nci@proc.f90:543) on every
+    host thread, over a 128x128x32 block of the 512^3 nodes.  The per-point
+    rates are applied to the whole grid (Gpoints/s for Bader + NCI)."""
     from oracle import oracle
 
-    small = config(3, n_override=(128, 128, 128))
+    small = config(3, n_override=(n_bader,) * 3)
     f = gb.synth_slater(ctx, small)
     t0 = time.perf_counter()
     oracle.bader(f, small.n, small.x2c, small.xat)
     t_bader = (time.perf_counter() - t0) / f.size
-    # NCI: a 16x16x16 block of nodes at the cell centre
+    del f
     n = cell.n
-    idx = np.arange(248, 264)
-    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    c0 = [v // 2 - 64 for v in n[:2]] + [n[2] // 2 - 16]
+    I, J, K = np.meshgrid(np.arange(c0[0], c0[0] + 128), np.arange(c0[1], c0[1] + 128),
+                          np.arange(c0[2], c0[2] + 32), indexing="ij")
     frac = np.stack([I / n[0], J / n[1], K / n[2]], -1).reshape(-1, 3)
-    pts = frac @ cell.x2c.T
+    pts = frac @ np.asarray(cell.x2c).T
     otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
     rc = max(min(g.cutrad, g.rmax) for g in tables)
     centre = pts.mean(0)
@@ -142,17 +205,21 @@ def cpu_baseline(ctx, cell, tables, species):
     pos, sp = [], []
     for a, x in enumerate(cell.xat):
         for t in np.ndindex(3, 3, 3):
-            p = cell.x2c @ (x + np.array(t) - 1)
+            p = np.asarray(cell.x2c) @ (x + np.array(t) - 1)
             if np.linalg.norm(p - centre) <= rad:
                 pos.append(p)
                 sp.append(species[a])
+    threads = host_threads()
     t0 = time.perf_counter()
-    oracle.promol_nci(pts, np.array(pos), np.array(sp, dtype=np.int32), otabs)
+    oracle.promol_nci_batch(pts, np.array(pos), np.array(sp, dtype=np.int32), otabs, threads=threads)
     t_nci = (time.perf_counter() - t0) / len(pts)
-    return {"value": 1e-9 / (t_bader + t_nci), "unit": "Gpoints/s", "cores": 1, "kind": "port",
-            "sample": "oracle Bader (serial sweep + refinement) on the same 30 A / 216-atom cell at "
-                      f"128^3 ({t_bader * 1e6:.2f} us/pt) + promolecular NCI at 4096 nodes of the "
-                      f"512^3 grid ({t_nci * 1e6:.2f} us/pt), 1 thread"}
+    return {"value": 1e-9 / (t_bader + t_nci), "unit": "Gpoints/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"oracle Bader (serial sweep + refinement, 1 thread, as the reference) on the same "
+                      f"30 A / 216-atom cell at {n_bader}^3: {t_bader * 1e6:.3f} us/pt; promolecular NCI "
+                      f"(OpenMP dynamic over nodes, {threads} threads) on {len(pts)} nodes of the 512^3 grid: "
+                      f"{t_nci * 1e6:.3f} us/pt; both rates applied to the whole grid",
+            "bader_us_per_point": round(t_bader * 1e6, 4), "nci_us_per_point": round(t_nci * 1e6, 4)}
 
 
 def run_batch(ctx, args):
@@ -482,30 +549,45 @@ def main():
     ctx.timing(False)
     st = bas.stats
 
-    # dominant kernel and its roofline: algorithmic bytes reported by the
-    # library for the launches it timed (list-driven kernels vary per launch),
-    # else the per-launch formula of the full-grid kernels
+    # dominant kernel and its roofline: achieved = the kernel family's
+    # compulsory bytes per point x N / the kernel's average launch time
+    # (HIP events on the library's stream); the kernel's own count of the
+    # bytes its design reads and writes is reported beside it
     dom = max(ktimes.items(), key=lambda kv: kv[1][0])
     name, (ms_tot, launches) = dom
     avg_ms = ms_tot / launches
     reported = kstats[name][2]
     if reported > 0:
-        nbytes = reported / launches
+        own_bytes = reported / launches
     else:
         ev_per_launch = st["events_total"] / max(st["sweep_iters"], 1)
-        nbytes = kernel_bytes(name, N, ev_per_launch, st["traj_points"])
-    roof = None
-    if nbytes is not None:
-        ach = nbytes / (avg_ms * 1e-3) / 1e9
-        # the committed PMC summaries are of the 512^3 step: no traffic for other workloads
-        measured = args.config == 3 and args.n == 512
-        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(name) if measured else None,
-                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps,
-                "algorithmic_bytes_per_launch": int(nbytes)}
+        own_bytes = kernel_bytes(name, N, ev_per_launch, st["traj_points"])
+    bpp = compulsory_bytes_per_point(name)
+    ach = bpp * N / (avg_ms * 1e-3) / 1e9
+    # the committed PMC summaries are of the 512^3 step: no traffic for other workloads
+    measured = args.config == 3 and args.n == 512
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 3), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6),
+            "traffic": measured_traffic(name) if measured else None,
+            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps,
+            "compulsory_bytes_per_point": bpp, "points_per_launch": N,
+            "algorithmic_bytes_per_launch": int(bpp * N),
+            "kernel_design_bytes_per_launch": int(own_bytes) if own_bytes is not None else None,
+            "achieved_kernel_design": round(own_bytes / (avg_ms * 1e-3) / 1e9, 3) if own_bytes else None}
+    # the FP32 promolecular evaluator is VALU-bound: FLOP/s against the FP32 vector peak
+    valu = None
+    pk = next((k for k in ("k_promol_f", "k_promol_f_queue") if k in ktimes), None)
+    if pk is not None:
+        pms = ktimes[pk][0] / ktimes[pk][1]
+        fl = measured_flops(pk) if measured else None
+        valu = {"kernel": pk, "avg_launch_ms": round(pms, 4),
+                "flops_per_launch": fl,
+                "achieved_tflops": round(fl / (pms * 1e-3) / 1e12, 3) if fl else None,
+                "peak_tflops_fp32": FP32_PEAK_TFLOPS,
+                "frac": round(fl / (pms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4) if fl else None}
 
     fp64 = None
+    precision = None
     if not args.no_fp64:
         outs64 = (ctx.empty((N,), np.float64), ctx.empty((N,), np.float64))
         ctx.sync()
@@ -514,6 +596,13 @@ def main():
         t64 = time.perf_counter() - t1
         fp64 = {"value": N / t64 / 1e9, "ms_per_step": t64 * 1e3}
         print("[bench] fp64 step done", file=sys.stderr, flush=True)
+        # FP32 outputs of the timed steps against the FP64 outputs, whole grid
+        precision = gn.precision_report(outs32[0].numpy(), outs32[1].numpy(), outs64[0].numpy(),
+                                        outs64[1].numpy(),
+                                        zeta_max=2.0 * max(int(z) for z in cell.zat) ** (1.0 / 3.0))
+        for o in outs64:
+            o.free()
+        print("[bench] precision report done", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -542,8 +631,10 @@ def main():
                                  f", Slater-sum FP64 grid {cell.n[0]}x{cell.n[1]}x{cell.n[2]}; Bader labels + "
                                  "Volume/Pop + FP32 promolecular NCI"),
                    "grid": list(cell.n), "atoms": int(len(cell.zat)), "parallelism": "single GPU"},
-        "hbm_frac_compulsory": round(20.0 * N / dt / 1e9 / HBM_PEAK_GBS, 6),
+        "hbm_frac_compulsory": round(STEP_BYTES_PER_POINT * N / dt / 1e9 / HBM_PEAK_GBS, 6),
         "roofline": roof,
+        "promol_valu": valu,
+        "fp32_vs_fp64": precision,
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_gpu": round(fp64["ms_per_step"] / (dt * 1e3), 3) if fp64 else None,

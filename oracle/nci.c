@@ -260,3 +260,33 @@ void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
       }
   free(pos); free(pref); free(zeta);
 }
+
+/* The promolecular branch of nciplot's per-point loop over many nodes,
+ * parallel as the reference's (nci@proc.f90:543, !$omp parallel do
+ * schedule(dynamic)): per node rho, |grad rho| (zero within 1e-5 bohr of a
+ * nucleus, fieldmod@proc.f90:1146-1155), s and sign(rho,lambda2)*100 from the
+ * full Hessian.  out: 4 doubles per node (rho, |grad|, s, crho).  The CPU
+ * baseline leg times it. */
+void oracle_promol_nci_batch(int64_t npts, const double *pts, int nimg, const double *ximg,
+                             const int *zidx, const orc_rgrid *tabs, const double *cut,
+                             int full_hessian, double *out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < npts; q++) {
+    const double *x = pts + 3 * q;
+    double f, g[3], h[9];
+    oracle_promol(x, nimg, ximg, zidx, tabs, cut, 2, &f, g, h);
+    double dmin = 1e300;
+    for (int ii = 0; ii < nimg; ii++) {
+      double d0 = x[0] - ximg[3 * ii], d1 = x[1] - ximg[3 * ii + 1], d2 = x[2] - ximg[3 * ii + 2];
+      double d = d0 * d0 + d1 * d1 + d2 * d2;
+      if (d < dmin) dmin = d;
+    }
+    double gm = (sqrt(dmin) <= 1e-5) ? 0.0 : sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    double cr, s;
+    oracle_nci_point(f, gm, h, full_hessian, &cr, &s);
+    out[4 * q] = f;
+    out[4 * q + 1] = gm;
+    out[4 * q + 2] = s;
+    out[4 * q + 3] = cr;
+  }
+}
