@@ -32,7 +32,7 @@ EXPORTS = (
     "c2gpu_ctx_sync", "c2gpu_dev_alloc", "c2gpu_dev_free", "c2gpu_memcpy", "c2gpu_ctx_timing",
     "c2gpu_ctx_kernel_time", "c2gpu_ctx_kernel_names", "c2gpu_ctx_kernel_bytes",
     "c2gpu_bader_metrics", "c2gpu_bader_step_pointers", "c2gpu_bader", "c2gpu_relabel",
-    "c2gpu_basin_sums", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
+    "c2gpu_basin_sums", "c2gpu_basin_sums_noalloc", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
     "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff", "c2gpu_ws_facets",
     "c2gpu_yt", "c2gpu_multipoles", "c2gpu_nci_promol_nodes", "c2gpu_nci_fragments", "c2gpu_nci_cutoff_frag",

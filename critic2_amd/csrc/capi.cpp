@@ -15,6 +15,9 @@ void bader_run(c2gpu_ctx *, const double *, const int *, const double *, const d
                const c2gpu_attractor_opts *, int, int32_t *, int *, double *, int, c2gpu_bader_stats *);
 void step_pointers_run(c2gpu_ctx *, const double *, const int *, const double *, const double *, int,
                        int64_t *, int64_t *, double *);
+void noalloc_init(int);
+void basin_sums_noalloc_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const double *const *, double *,
+                            int64_t *);
 void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const double *const *, int,
                     double *, int64_t *);
 void relabel_run(c2gpu_ctx *, int64_t, const int32_t *, int, int, int32_t *);
@@ -263,6 +266,14 @@ int c2gpu_ctx_create(c2gpu_ctx **ctx, int device) {
     return C2GPU_ERR_HIP;
   }
   c->own_stream = true;
+  // the device address of this device's scratch of c2gpu_basin_sums_noalloc
+  try {
+    c2::noalloc_init(device);
+  } catch (const c2::Error &) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return C2GPU_ERR_HIP;
+  }
   *ctx = c;
   return C2GPU_OK;
 }
@@ -433,6 +444,19 @@ int c2gpu_basin_sums(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nat
       for (int q = 0; q < nfields; q++) require_device_ptr(fields[q], "fields[k]");
     }
     basin_sums_run(ctx, label, npts, nattr, nfields, fields, flags, sums_out, counts_out);
+  });
+}
+
+int c2gpu_basin_sums_noalloc(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nattr, int nfields,
+                             const double *const *fields, double *sums_out, int64_t *counts_out) {
+  return guarded(ctx, [&] {
+    if (!label || (nfields > 0 && (!fields || !sums_out)) || (nfields == 0 && !counts_out))
+      fail(C2GPU_ERR_ARG, "null argument");
+    require_device_ptr(label, "label");
+    for (int q = 0; q < nfields; q++) require_device_ptr(fields[q], "fields[k]");
+    if (nfields > 0) require_device_ptr(sums_out, "sums_out");
+    if (counts_out) require_device_ptr(counts_out, "counts_out");
+    basin_sums_noalloc_run(ctx, label, npts, nattr, nfields, fields, sums_out, counts_out);
   });
 }
 

@@ -10,6 +10,7 @@
 // order, so the result is bitwise reproducible for a given grid size.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -139,6 +140,89 @@ void basin_sums_run(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int natt
   if (sums_out)
     for (int q = 0; q < nfields; q++)
       for (int a = 0; a < nattr; a++) sums_out[(size_t)q * nattr + a] = h[(size_t)(q + 1) * nattr + a];
+}
+
+// ---- the per-basin sums without allocation (c2gpu_basin_sums_noalloc)
+//
+// The block partials live in a file-scope device array instead of a
+// context buffer: every device the library runs on holds its own copy (32
+// MiB) until the process ends.  Host code never passes the array's name to a
+// kernel (in host code it names a host-side stand-in); it passes the
+// device address hipGetSymbolAddress returned for the current device, kept
+// per device from the first context created on it.
+static constexpr size_t NOALLOC_PARTIALS = (size_t)2048 * 2048;  // 2048 blocks x 2048 bins (doubles)
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+static double *scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+// Called with `device` current (c2gpu_ctx_create, or the entry's first call
+// on a device): records the device address of this device's scratch.
+void noalloc_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) fail(C2GPU_ERR_ARG, "noalloc scratch: device index out of range");
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (scratch_dev[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    fail(C2GPU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  scratch_dev[device] = static_cast<double *>(p);
+}
+
+// The block partials in bin order, added in block order (as k_basin_reduce),
+// written straight to the caller's outputs: counts (bins 0..nattr-1) as
+// int64, then each field's sums.
+__global__ void k_basin_reduce_out(const double *__restrict__ partial, int nblocks, int nattr, int nf,
+                                   double *__restrict__ sums_out, int64_t *__restrict__ counts_out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = (nf + 1) * nattr;
+  if (q >= nb) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * nb + q];
+  if (q < nattr) {
+    if (counts_out) counts_out[q] = (int64_t)s;
+  } else {
+    sums_out[q - nattr] = s;
+  }
+}
+
+void basin_sums_noalloc_run(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nattr, int nfields,
+                            const double *const *fields, double *sums_out, int64_t *counts_out) {
+  if (npts <= 0 || nattr <= 0 || nfields < 0 || nfields > 4)
+    fail(C2GPU_ERR_ARG, "c2gpu_basin_sums_noalloc: need npts > 0, nattr > 0, 0 <= nfields <= 4");
+  if ((int64_t)nattr * (nfields + 1) * SUM_WAVES * 8 > 65536)
+    fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_basin_sums_noalloc: nattr*(nfields+1) above 2048 bins");
+  FieldPtrs F{};
+  for (int q = 0; q < nfields; q++) F.p[q] = fields[q];
+  const int nb = (nfields + 1) * nattr;
+  // the chunking of basin_sums_run: identical partials, identical sums
+  const uint64_t tile = 64 * SUM_WAVES;
+  uint64_t chunk = ((uint64_t)npts + 2047) / 2048;
+  chunk = ((chunk + tile - 1) / tile) * tile;
+  if (chunk < tile) chunk = tile;
+  const int nblocks = (int)(((uint64_t)npts + chunk - 1) / chunk);
+  if ((size_t)nblocks * (size_t)nb > NOALLOC_PARTIALS)
+    fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_basin_sums_noalloc: " + std::to_string(nblocks) + " blocks x " +
+                                    std::to_string(nb) + " bins exceed the scratch of " +
+                                    std::to_string(NOALLOC_PARTIALS) + " partials");
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) fail(C2GPU_ERR_ARG, "device index out of range");
+  double *partial = nullptr;
+  {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    partial = scratch_dev[ctx->device];
+  }
+  if (!partial) {
+    noalloc_init(ctx->device);
+    partial = scratch_dev[ctx->device];
+  }
+  const size_t lds = sizeof(double) * (size_t)nb * SUM_WAVES;
+  // plain launches: no timing events (creating one may allocate)
+  hipLaunchKernelGGL(k_basin_partials, dim3(nblocks), dim3(64 * SUM_WAVES), lds, ctx->stream, label,
+                     (uint64_t)npts, nattr, nfields, F, chunk, partial);
+  C2_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_basin_reduce_out, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, partial, nblocks,
+                     nattr, nfields, sums_out, counts_out);
+  C2_HIP(hipGetLastError());
 }
 
 void relabel_run(c2gpu_ctx *ctx, int64_t npts, const int32_t *map, int nmap, int flags,

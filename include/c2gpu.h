@@ -181,6 +181,23 @@ int c2gpu_basin_sums(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nat
                      int nfields, const double *const *fields, int flags,
                      double *sums_out, int64_t *counts_out);
 
+/* No counterpart in the reference: c2gpu_basin_sums for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or other
+ * allocating routine: its block partials live in a fixed device scratch of
+ * 2048 x 2048 doubles (32 MiB, held by every device the library runs on
+ * until the process ends).  label, the nfields field grids, sums_out
+ * (nfields*nattr doubles, as c2gpu_basin_sums) and counts_out (nattr int64,
+ * may be NULL when nfields > 0) are device memory.  The work is enqueued on
+ * the context's stream and the call returns without waiting; the results
+ * equal c2gpu_basin_sums's bit for bit.  Limits: nattr*(nfields+1) <= 2048
+ * bins (as c2gpu_basin_sums), nfields <= 4; larger requests return
+ * C2GPU_ERR_UNSUPPORTED.  Every call on one device shares that device's
+ * scratch: calls that overlap in time on one GPU (two streams, two threads)
+ * race and must be serialised by the caller. */
+int c2gpu_basin_sums_noalloc(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int nattr,
+                             int nfields, const double *const *fields,
+                             double *sums_out, int64_t *counts_out);
 
 /* ------------------------------------------------------------------ */
 /* FFT derivative grids and NCI                                        */
