@@ -35,7 +35,7 @@ EXPORTS = (
     "c2gpu_basin_sums", "c2gpu_basin_sums_noalloc", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
     "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff", "c2gpu_ws_facets",
-    "c2gpu_yt", "c2gpu_multipoles", "c2gpu_nci_promol_nodes", "c2gpu_nci_fragments", "c2gpu_nci_cutoff_frag",
+    "c2gpu_yt", "c2gpu_multipoles", "c2gpu_yt_multipoles", "c2gpu_nci_promol_nodes", "c2gpu_nci_fragments", "c2gpu_nci_cutoff_frag",
 )
 
 
@@ -146,6 +146,7 @@ def load():
         "c2gpu_nci_fragments": [vp, C.POINTER(NciNodes), i, C.POINTER(RadialTable), i, vp, vp, vp, i, i, vp],
         "c2gpu_ws_facets": [vp, C.POINTER(i), vp, vp],
         "c2gpu_multipoles": [vp, vp, vp, vp, vp, i, vp, i, i, vp],
+        "c2gpu_yt_multipoles": [vp, vp, vp, vp, vp, i, vp, i, i, vp],
         "c2gpu_yt": [vp, vp, vp, vp, C.POINTER(AttractorOpts), i, vp, i, vp, C.POINTER(i), vp, i, vp, vp,
                      C.POINTER(YtStats)],
     }

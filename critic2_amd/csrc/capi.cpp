@@ -42,6 +42,8 @@ void yt_run(c2gpu_ctx *, const double *, const int *, const double *, const c2gp
             const double *const *, int, int32_t *, int *, double *, int, double *, double *, c2gpu_yt_stats *);
 void mpole_run(c2gpu_ctx *, const int32_t *, const double *, const int *, const double *, int, const double *, int,
                int, double *);
+void yt_mpole_run(c2gpu_ctx *, const int32_t *, const double *, const int *, const double *, int, const double *, int,
+               int, double *);
 void cube_format_run(c2gpu_ctx *, const double *, const int *, int, int, int, char *, uint64_t, uint64_t *);
 void cube_write_run(c2gpu_ctx *, const char *, const c2gpu_cube_header *, const int *, const double *,
                     const double *, int, int);
@@ -612,6 +614,11 @@ int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
 int c2gpu_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
                      int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
   return guarded(ctx, [&] { mpole_run(ctx, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out); });
+}
+
+int c2gpu_yt_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
+                        int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
+  return guarded(ctx, [&] { yt_mpole_run(ctx, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out); });
 }
 
 }  // extern "C"

@@ -47,6 +47,20 @@ struct c2gpu_ctx {
   // device address itself
   void *pin = nullptr;
   size_t pin_bytes = 0;
+  // the surface weights of the last c2gpu_yt call (context buffers
+  // yt_ias, yt_woff, yt_wc, yt_wb, yt_ww), for c2gpu_yt_multipoles, as
+  // critic2 keeps them in its YT checkpoint for later yt_weights calls
+  struct YtState {
+    bool valid = false;
+    int n[3] = {0, 0, 0};
+    int nattr = 0;
+    uint32_t nias = 0;
+    uint64_t nent = 0;  // weight entries over all surface points
+    const uint32_t *ias = nullptr, *wb = nullptr;
+    const uint64_t *woff = nullptr;
+    const uint16_t *wc = nullptr;
+    const double *ww = nullptr;
+  } yt;
 };
 
 namespace c2 {

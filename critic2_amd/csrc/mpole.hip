@@ -14,6 +14,10 @@
 //               / sqrt((l+m+1)(l-m+1)),
 // the same functions the reference builds from spherical angles and
 // spherical harmonics (tosphere, genylm); the sums agree to rounding.
+//
+// The YT branch (integration@proc.f90:1314-1336) weighs each point by
+// yt_weights' w_a(p): 1 at the interior points of basin a (the labels of a
+// c2gpu_yt call), the stored surface weight at its surface points (label 0).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -77,6 +81,37 @@ __device__ __forceinline__ void solid_harmonics(int lmax, double x, double y, do
   }
 }
 
+// The Cartesian shortest lattice translate of p/n - xa (crystal%shortest):
+// the nint fold (half away from zero), then for non-orthogonal cells the
+// first strictly shorter neighbouring translate.
+__device__ __forceinline__ void shortest_offset(const MpGeo &G, int i, int j, int k, const double *xa,
+                                                double v[3]) {
+  double dv[3] = {(double)i / G.n1 - xa[0], (double)j / G.n2 - xa[1], (double)k / G.n3 - xa[2]};
+  for (int c = 0; c < 3; c++) dv[c] = dv[c] - round(dv[c]);
+  for (int c = 0; c < 3; c++) v[c] = G.x2c[c] * dv[0] + G.x2c[c + 3] * dv[1] + G.x2c[c + 6] * dv[2];
+  if (!G.ortho) {
+    double best = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    double bv[3] = {v[0], v[1], v[2]};
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int c = -1; c <= 1; c++) {
+          if (!a && !b && !c) continue;
+          double t[3];
+          for (int e = 0; e < 3; e++) t[e] = v[e] + G.x2c[e] * a + G.x2c[e + 3] * b + G.x2c[e + 6] * c;
+          const double d = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+          if (d < best) {
+            best = d;
+            bv[0] = t[0];
+            bv[1] = t[1];
+            bv[2] = t[2];
+          }
+        }
+    v[0] = bv[0];
+    v[1] = bv[1];
+    v[2] = bv[2];
+  }
+}
+
 __global__ void __launch_bounds__(MB) k_mpole(MpGeo G, uint64_t npts, const int32_t *__restrict__ lab,
                                                const double *__restrict__ f, const double *__restrict__ xattr,
                                                double *__restrict__ out) {
@@ -93,33 +128,8 @@ __global__ void __launch_bounds__(MB) k_mpole(MpGeo G, uint64_t npts, const int3
     const uint64_t r = p / (uint64_t)G.n1;
     const int j = (int)(r % (uint64_t)G.n2);
     const int k = (int)(r / (uint64_t)G.n2);
-    // dv = p/n - xattr(ix), folded with nint (half away from zero)
-    double dv[3] = {(double)i / G.n1 - xattr[3 * (ix - 1)], (double)j / G.n2 - xattr[3 * (ix - 1) + 1],
-                    (double)k / G.n3 - xattr[3 * (ix - 1) + 2]};
-    for (int c = 0; c < 3; c++) dv[c] = dv[c] - round(dv[c]);
     double v[3];
-    for (int c = 0; c < 3; c++) v[c] = G.x2c[c] * dv[0] + G.x2c[c + 3] * dv[1] + G.x2c[c + 6] * dv[2];
-    if (!G.ortho) {  // the shortest translate (first strictly shorter one)
-      double best = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-      double bv[3] = {v[0], v[1], v[2]};
-      for (int a = -1; a <= 1; a++)
-        for (int b = -1; b <= 1; b++)
-          for (int c = -1; c <= 1; c++) {
-            if (!a && !b && !c) continue;
-            double t[3];
-            for (int e = 0; e < 3; e++) t[e] = v[e] + G.x2c[e] * a + G.x2c[e + 3] * b + G.x2c[e + 6] * c;
-            const double d = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-            if (d < best) {
-              best = d;
-              bv[0] = t[0];
-              bv[1] = t[1];
-              bv[2] = t[2];
-            }
-          }
-      v[0] = bv[0];
-      v[1] = bv[1];
-      v[2] = bv[2];
-    }
+    shortest_offset(G, i, j, k, &xattr[3 * (ix - 1)], v);
     solid_harmonics(G.lmax, v[0], v[1], v[2], R);
     const double fv = f[p];
     double *bb = bins + (size_t)(ix - 1) * nlm;
@@ -130,18 +140,72 @@ __global__ void __launch_bounds__(MB) k_mpole(MpGeo G, uint64_t npts, const int3
     if (bins[q] != 0.0) atomicAdd(&out[q], bins[q]);
 }
 
+// The surface points of a YT partition (integration@proc.f90:1314-1336): each
+// entry (basin b, weight w) of point p adds R_lm(shortest(p/n - xattr(b)))
+// f(p) w to basin b; entries with |w| < 1d-15 are skipped as the reference
+// skips them.  Binned per workgroup in LDS.
+__global__ void __launch_bounds__(MB) k_mpole_yt_surface(MpGeo G, uint32_t nias, const uint32_t *__restrict__ ias,
+                                                          const uint64_t *__restrict__ woff,
+                                                          const uint16_t *__restrict__ wc,
+                                                          const uint32_t *__restrict__ wb,
+                                                          const double *__restrict__ ww, const double *__restrict__ f,
+                                                          const double *__restrict__ xattr, double *__restrict__ out) {
+  extern __shared__ double bins[];  // nattr * nlm
+  const int nlm = (G.lmax + 1) * (G.lmax + 1);
+  const int nb = G.nattr * nlm;
+  for (int q = threadIdx.x; q < nb; q += MB) bins[q] = 0.0;
+  __syncthreads();
+  double R[NLM_MAX];
+  for (uint32_t t = blockIdx.x * MB + threadIdx.x; t < nias; t += gridDim.x * MB) {
+    const uint32_t p = ias[t];
+    const int i = (int)(p % (uint32_t)G.n1);
+    const uint32_t r = p / (uint32_t)G.n1;
+    const int j = (int)(r % (uint32_t)G.n2);
+    const int k = (int)(r / (uint32_t)G.n2);
+    const double fv = f[p];
+    const uint64_t so = woff[t];
+    const int ce = wc[t];
+    for (int e = 0; e < ce; e++) {
+      const int a = (int)wb[so + e];
+      const double w = ww[so + e];
+      if (a < 1 || a > G.nattr || fabs(w) < 1e-15) continue;
+      double v[3];
+      shortest_offset(G, i, j, k, &xattr[3 * (a - 1)], v);
+      solid_harmonics(G.lmax, v[0], v[1], v[2], R);
+      double *bb = bins + (size_t)(a - 1) * nlm;
+      const double fw = fv * w;
+      for (int q = 0; q < nlm; q++) atomicAdd(&bb[q], R[q] * fw);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nb; q += MB)
+    if (bins[q] != 0.0) atomicAdd(&out[q], bins[q]);
+}
+
 }  // namespace
 
-void mpole_run(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
-               int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
-  if (!label || !f || !n || !x2c || !xattr || !mpole_out) fail(C2GPU_ERR_ARG, "c2gpu_multipoles: NULL argument");
-  if (lmax < 0 || lmax > LMAX_MAX) fail(C2GPU_ERR_ARG, "c2gpu_multipoles: 0 <= lmax <= 10");
-  if (nattr < 1) fail(C2GPU_ERR_ARG, "c2gpu_multipoles: nattr >= 1");
+namespace {
+
+// Multipoles of the labelled points (every point p with 1 <= label <= nattr
+// adds to its basin), and with yt the surface points of the last c2gpu_yt
+// call through their weights.
+void mpole_impl(c2gpu_ctx *ctx, const char *who, bool yt, const int32_t *label, const double *f, const int n[3],
+                const double x2c[9], int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
+  const std::string w(who);
+  if (!label || !f || !n || !x2c || !xattr || !mpole_out) fail(C2GPU_ERR_ARG, w + ": NULL argument");
+  if (lmax < 0 || lmax > LMAX_MAX) fail(C2GPU_ERR_ARG, w + ": 0 <= lmax <= 10");
+  if (nattr < 1) fail(C2GPU_ERR_ARG, w + ": nattr >= 1");
   for (int a = 0; a < 3; a++)
-    if (n[a] < 1) fail(C2GPU_ERR_ARG, "c2gpu_multipoles: grid dimensions must be positive");
+    if (n[a] < 1) fail(C2GPU_ERR_ARG, w + ": grid dimensions must be positive");
+  if (yt) {
+    const auto &Y = ctx->yt;
+    if (!Y.valid) fail(C2GPU_ERR_ARG, w + ": no c2gpu_yt call on this context has left its weights");
+    if (Y.n[0] != n[0] || Y.n[1] != n[1] || Y.n[2] != n[2] || Y.nattr != nattr)
+      fail(C2GPU_ERR_ARG, w + ": the grid and nattr must be those of the last c2gpu_yt call");
+  }
   const int nlm = (lmax + 1) * (lmax + 1);
   const size_t lds = 8 * (size_t)nattr * nlm;
-  if (lds > 120 * 1024) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_multipoles: nattr * (lmax+1)^2 too large for LDS bins");
+  if (lds > 120 * 1024) fail(C2GPU_ERR_UNSUPPORTED, w + ": nattr * (lmax+1)^2 too large for LDS bins");
   const uint64_t npts = (uint64_t)n[0] * n[1] * n[2];
   hipStream_t st = ctx->stream;
   MpGeo G{};
@@ -173,11 +237,31 @@ void mpole_run(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int 
   C2_HIP(hipMemcpyAsync(d_x, xattr, 24 * (size_t)nattr, hipMemcpyHostToDevice, st));
   double *d_out = buf<double>(ctx, "mp_out", (size_t)nattr * nlm);
   C2_HIP(hipMemsetAsync(d_out, 0, 8 * (size_t)nattr * nlm, st));
-  const unsigned g = std::min<unsigned>(grid_for((int64_t)npts, MB), (unsigned)persistent_threads(ctx->device, MB) / MB);
+  const unsigned gp = (unsigned)persistent_threads(ctx->device, MB) / MB;
+  const unsigned g = std::min<unsigned>(grid_for((int64_t)npts, MB), gp);
   C2_LAUNCH(ctx, k_mpole, dim3(g), dim3(MB), lds, G, npts, d_lab, d_f, d_x, d_out);
   add_kernel_bytes(ctx, "k_mpole", 12.0 * (double)npts);
-  C2_HIP(hipMemcpyAsync(mpole_out, d_out, 8 * (size_t)nattr * nlm, hipMemcpyDeviceToHost, st));
+  if (yt && ctx->yt.nias) {
+    const auto &Y = ctx->yt;
+    const unsigned gs = std::min<unsigned>(grid_for((int64_t)Y.nias, MB), gp);
+    C2_LAUNCH(ctx, k_mpole_yt_surface, dim3(gs), dim3(MB), lds, G, Y.nias, Y.ias, Y.woff, Y.wc, Y.wb, Y.ww, d_f,
+              d_x, d_out);
+    add_kernel_bytes(ctx, "k_mpole_yt_surface", 22.0 * (double)Y.nias + 12.0 * (double)Y.nent);
+  }
+  copy_to_host(ctx, mpole_out, d_out, 8 * (size_t)nattr * nlm);
   C2_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+void mpole_run(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
+               int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
+  mpole_impl(ctx, "c2gpu_multipoles", false, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out);
+}
+
+void yt_mpole_run(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
+                  int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
+  mpole_impl(ctx, "c2gpu_yt_multipoles", true, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out);
 }
 
 }  // namespace c2

@@ -860,6 +860,13 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   const unsigned gW = grid_for(((int64_t)N + 15) / 16, B);
   const unsigned gCollect = gW < 2048 ? gW : 2048;
   const bool full_every_step = std::getenv("C2GPU_SWEEP_FULL") != nullptr;
+  // while a step changes more than this many decisions (a quarter of the
+  // points), the lists are rebuilt from the story instead of patched: an
+  // early step rewrites most of the story, and a full build writes each
+  // event once where the patch path moves the lists that overflow their
+  // blocks (at 512^3: 3.14 s against 3.40 s for patching every step)
+  uint64_t full_above = N / 4;
+  if (const char *fa = std::getenv("C2GPU_SWEEP_FULL_ABOVE")) full_above = std::strtoull(fa, nullptr, 10);
   const bool trace = std::getenv("C2GPU_SWEEP_TRACE") != nullptr;
   SweepResult R{0, 0, 0, nullptr};
   const size_t NF = ((size_t)N + 16) & ~(size_t)15;  // byte flags padded to whole 16-byte groups
@@ -962,6 +969,20 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     if (it >= max_iters) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_bader: first-sweep fixpoint did not converge");
     it++;
     const unsigned gC = grid_for(nc, B);
+    if ((uint64_t)nc > full_above) {
+      // a large step: take the new decisions and rebuild every list
+      C2_LAUNCH(ctx, k_apply, dim3(gC), dim3(B), 0, nc, chlist, ell, lam, ell2, lam2, chg);
+      full_build();
+      nc = decide_all();
+      if (trace) {
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[sweep] it %d %.3f ms\n", it,
+                     std::chrono::duration<double, std::milli>(now - t_last).count());
+        t_last = now;
+        std::fprintf(stderr, "[sweep] it %d full build, changed %u\n", it, nc);
+      }
+      continue;
+    }
     // 1. points whose lists change, and their new sizes
     if (trace) C2_HIP(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), st));
     C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, lam, lam2, delta, dflag, cfrom,

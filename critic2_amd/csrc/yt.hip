@@ -601,6 +601,7 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   // DISCARD in yt_integrate (yt@proc.f90:153-167) turns the points below a
   // discarded maximum into surface points; not built here
   if (opts->discard) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: the DISCARD callback is not supported");
+  ctx->yt.valid = false;
   if (nfields < 0 || nfields > 4 || (nfields > 0 && (!fields || !sums_out)))
     fail(C2GPU_ERR_ARG, "c2gpu_yt: 0 <= nfields <= 4, with fields and sums_out");
   for (int a = 0; a < 3; a++)
@@ -806,6 +807,17 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
       sums_out[(size_t)q * nattr + a] = isums[(size_t)q * nattr + a] + h[(size_t)(q + 1) * nattr + a];
   for (size_t c = 0; c < xattr.size(); c++) xattr_out[c] = xattr[c];
   *nattr_out = nattr;
+  // the surface weights stay in the context for c2gpu_yt_multipoles
+  ctx->yt.valid = true;
+  for (int c = 0; c < 3; c++) ctx->yt.n[c] = n[c];
+  ctx->yt.nattr = nattr;
+  ctx->yt.nias = nias;
+  ctx->yt.nent = nias ? read_scalar(ctx, W.top) : 0;
+  ctx->yt.ias = ias;
+  ctx->yt.woff = W.off;
+  ctx->yt.wc = W.c;
+  ctx->yt.wb = W.b;
+  ctx->yt.ww = W.w;
   S.ms_total = ttot.stop();
   if (stats) *stats = S;
 }

@@ -51,7 +51,7 @@ module c2gpu
   public :: c2gpu_ctx_create, c2gpu_ctx_destroy, c2gpu_bader_metrics, c2gpu_bader
   public :: c2gpu_basin_sums, c2gpu_fft_deriv, c2gpu_relabel
   public :: c2gpu_cube_read_header, c2gpu_cube_read, c2gpu_bincube_read, c2gpu_cube_write
-  public :: c2gpu_nci_cutoff, c2gpu_multipoles
+  public :: c2gpu_nci_cutoff, c2gpu_multipoles, c2gpu_yt_multipoles
   public :: c2gpu_nci, c2gpu_nci_promol_nodes, c2gpu_nci_fragments, c2gpu_nci_cutoff_frag
 
   integer(c_int), parameter, public :: C2GPU_FP64 = 0, C2GPU_FP32 = 1
@@ -302,6 +302,21 @@ module c2gpu
        real(c_double), intent(out) :: mpole_out(*)
        integer(c_int) :: c2gpu_multipoles
      end function c2gpu_multipoles
+
+     function c2gpu_yt_multipoles(ctx, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out) &
+        bind(c, name="c2gpu_yt_multipoles")
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: label(*)
+       real(c_double), intent(in) :: f(*)
+       integer(c_int), intent(in) :: n(3)
+       real(c_double), intent(in) :: x2c(9)
+       integer(c_int), value :: nattr
+       real(c_double), intent(in) :: xattr(*)
+       integer(c_int), value :: lmax, flags
+       real(c_double), intent(out) :: mpole_out(*)
+       integer(c_int) :: c2gpu_yt_multipoles
+     end function c2gpu_yt_multipoles
 
      function c2gpu_ctx_error(ctx) bind(c, name="c2gpu_ctx_error")
        import :: c_ptr

@@ -369,11 +369,11 @@ def precision_report(c32, s32, c64, s64, s_floor=0.1, rho_min=1e-6, zeta_max=Non
     # the same by density band: the gradient's atom terms cancel most where
     # the density is lowest, so the FP32 error of |grad rho| and s grows there
     r_reg = r64[reg]
-    bands = [(lo, hi, (r_reg >= lo) & (r_reg < hi)) for lo, hi in RHO_BANDS]
+    bands = [(b_lo, b_hi, (r_reg >= b_lo) & (r_reg < b_hi)) for b_lo, b_hi in RHO_BANDS]
     out["by_rho"] = []
-    for lo, hi, sel in bands:
+    for b_lo, b_hi, sel in bands:
         shi = sel & (s64[reg] >= s_floor)
-        row = {"rho_lo": lo, "rho_hi": hi, "points": int(sel.sum())}
+        row = {"rho_lo": b_lo, "rho_hi": b_hi, "points": int(sel.sum())}
         row["rho_max_rel"], row["rho_rms_rel"] = stats(e_rho[sel])
         eg = np.abs(g32[shi] - g64[shi]) / g64[shi]
         es = np.abs(s32[reg][shi] - s64[reg][shi]) / s64[reg][shi]

@@ -442,6 +442,21 @@ int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
              int32_t *label_out, int *nattr_out, double *xattr_out, int xattr_cap, double *vol_out,
              double *sums_out, c2gpu_yt_stats *stats);
 
+/* Basin multipoles of a YT partition (INTEGRABLE f MULTIPOLE lmax,
+ * integration@proc.f90:1314-1336, the imtype_yt branch):
+ * mpole_out[a*(lmax+1)^2 + q] = sum over the points p with |w_a(p)| >= 1e-15
+ * of rrlm(q) f(p) w_a(p), w_a = yt_weights' weight of basin a+1 (1 at its
+ * interior points, the surface weight elsewhere), rrlm as in
+ * c2gpu_multipoles.  The surface weights are those of the last c2gpu_yt
+ * call on this context, which keeps them as critic2 keeps its YT checkpoint
+ * for later yt_weights calls; label is that call's label_out, and n and
+ * nattr must be that call's (else C2GPU_ERR_ARG).  Every basin is computed:
+ * critic2 leaves those whose attractor it does not integrate
+ * (docelatom) at zero.  Flags, layout and units as c2gpu_multipoles. */
+int c2gpu_yt_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3],
+                        const double x2c[9], int nattr, const double *xattr, int lmax, int flags,
+                        double *mpole_out);
+
 /* ------------------------------------------------------------------ */
 /* Gaussian cube input and output                                       */
 /* ------------------------------------------------------------------ */

@@ -734,7 +734,7 @@ static int yt_cmp(const void *a, const void *b) {
  * multiplies by Omega/N). */
 int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, const int *vec, const double *area,
               int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
-              double *vol, double *sums) {
+              double *vol, double *sums, double *wout, int wcap) {
   const int64_t nn = (int64_t)n[0] * n[1] * n[2];
   int64_t *io = malloc(sizeof(int64_t) * nn), *iio = malloc(sizeof(int64_t) * nn);
   int32_t *ibasin = calloc(nn, sizeof(int32_t)); /* by sorted position, as in yt_integrate */
@@ -838,6 +838,8 @@ int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, cons
       double sv = 0.0;
       for (int64_t p = 0; p < nn; p++) sv += waux[iio[p]];
       vol[m - 1] = sv;
+      if (wout && m <= wcap)  /* w(i,j,k) = waux(iio(p)), grid order */
+        for (int64_t p = 0; p < nn; p++) wout[(size_t)(m - 1) * nn + p] = waux[iio[p]];
       for (int q = 0; q < nfields; q++) {
         double s = 0.0;
         for (int64_t p = 0; p < nn; p++) s += waux[iio[p]] * fields[q][p];
@@ -944,6 +946,32 @@ void oracle_tosphere(const double v[3], double *r, double tp[2]) {
 /* mpole[a * nlm + q], a the basin (0-based), unscaled (critic2 multiplies
  * by Omega/N).  shortest: nint fold, then the shortest translate over
  * -2..2 (first strictly shorter). */
+/* The Cartesian shortest lattice translate of p/n - xattr (crystal%shortest,
+ * crystalmod@proc.f90:1054-1085), by the nint fold and a search over the
+ * neighbouring translates. */
+static void shortest_offset(const int n[3], const double x2c[9], int64_t i, int64_t j, int64_t k,
+                            const double *xa, double bv[3]) {
+  double dv[3] = {(double)i / n[0] - xa[0], (double)j / n[1] - xa[1], (double)k / n[2] - xa[2]};
+  for (int c = 0; c < 3; c++) dv[c] = dv[c] - (double)lround(dv[c]);
+  double v[3];
+  x2c_apply(x2c, dv, v);
+  double best = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  bv[0] = v[0]; bv[1] = v[1]; bv[2] = v[2];
+  for (int a = -2; a <= 2; a++)
+    for (int b = -2; b <= 2; b++)
+      for (int c = -2; c <= 2; c++) {
+        double xt[3] = {dv[0] + a, dv[1] + b, dv[2] + c}, t[3];
+        x2c_apply(x2c, xt, t);
+        const double d = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        if (d < best) {
+          best = d;
+          bv[0] = t[0];
+          bv[1] = t[1];
+          bv[2] = t[2];
+        }
+      }
+}
+
 void oracle_multipoles(const int32_t *idg, const double *f, const int n[3], const double x2c[9], int nattr,
                        const double *xattr, int lmax, double *mpole) {
   const int nlm = (lmax + 1) * (lmax + 1);
@@ -955,28 +983,36 @@ void oracle_multipoles(const int32_t *idg, const double *f, const int n[3], cons
         const int64_t p = i + n[0] * (j + n[1] * k);
         const int ix = idg[p];
         if (ix < 1 || ix > nattr) continue;
-        double dv[3] = {(double)i / n[0] - xattr[3 * (ix - 1)], (double)j / n[1] - xattr[3 * (ix - 1) + 1],
-                        (double)k / n[2] - xattr[3 * (ix - 1) + 2]};
-        for (int c = 0; c < 3; c++) dv[c] = dv[c] - (double)lround(dv[c]);
-        double v[3];
-        x2c_apply(x2c, dv, v);
-        double best = v[0] * v[0] + v[1] * v[1] + v[2] * v[2], bv[3] = {v[0], v[1], v[2]};
-        for (int a = -2; a <= 2; a++)
-          for (int b = -2; b <= 2; b++)
-            for (int c = -2; c <= 2; c++) {
-              double xt[3] = {dv[0] + a, dv[1] + b, dv[2] + c}, t[3];
-              x2c_apply(x2c, xt, t);
-              const double d = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-              if (d < best) {
-                best = d;
-                bv[0] = t[0];
-                bv[1] = t[1];
-                bv[2] = t[2];
-              }
-            }
+        double bv[3];
+        shortest_offset(n, x2c, i, j, k, &xattr[3 * (ix - 1)], bv);
         double r, tp[2];
         oracle_tosphere(bv, &r, tp);
         oracle_genrlm_real(lmax, r, tp, rrlm);
         for (int q = 0; q < nlm; q++) mpole[(ix - 1) * nlm + q] += rrlm[q] * f[p];
       }
+}
+
+/* The YT branch (integration@proc.f90:1314-1336): per basin m, the weights
+ * w(m*N + p) of yt_weights (1 on its interior, the propagated surface
+ * weight elsewhere); points with |w| < 1d-15 are skipped. */
+void oracle_yt_multipoles(const double *w, const double *f, const int n[3], const double x2c[9], int nattr,
+                          const double *xattr, int lmax, double *mpole) {
+  const int nlm = (lmax + 1) * (lmax + 1);
+  const int64_t N = (int64_t)n[0] * n[1] * n[2];
+  double rrlm[17 * 17];
+  for (int q = 0; q < nattr * nlm; q++) mpole[q] = 0.0;
+  for (int m = 0; m < nattr; m++)
+    for (int64_t i = 0; i < n[0]; i++)
+      for (int64_t j = 0; j < n[1]; j++)
+        for (int64_t k = 0; k < n[2]; k++) {
+          const int64_t p = i + n[0] * (j + n[1] * k);
+          const double wp = w[(size_t)m * N + p];
+          if (fabs(wp) < 1e-15) continue;
+          double bv[3];
+          shortest_offset(n, x2c, i, j, k, &xattr[3 * m], bv);
+          double r, tp[2];
+          oracle_tosphere(bv, &r, tp);
+          oracle_genrlm_real(lmax, r, tp, rrlm);
+          for (int q = 0; q < nlm; q++) mpole[m * nlm + q] += rrlm[q] * f[p] * wp;
+        }
 }

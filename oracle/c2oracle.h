@@ -180,7 +180,11 @@ void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
  * refused), 2 (more than xcap attractors). */
 int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, const int *vec, const double *area,
               int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
-              double *vol, double *sums);
+              double *vol, double *sums, double *wout, int wcap);
+/* YT multipoles (integration@proc.f90:1314-1336) from the per-basin weights
+ * wout of oracle_yt (basin-major, grid order). */
+void oracle_yt_multipoles(const double *w, const double *f, const int n[3], const double x2c[9], int nattr,
+                          const double *xattr, int lmax, double *mpole);
 
 /* Basin multipoles (integration@proc.f90:1340-1362): mpole[a*nlm + q],
  * nlm = (lmax+1)^2, unscaled.  genrlm_real / tosphere restated for tests. */

@@ -499,14 +499,15 @@ def nci_cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e
     return dat, cg, sums
 
 
-def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40):
+def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40, weights=False):
     """yt_integrate + yt_weights + the imtype_yt sums (oracle_yt).  Returns
-    (labels in grid order, nattr, xattr, vol, sums[q, a]); raises on equal
-    neighbouring values."""
+    (labels in grid order, nattr, xattr, vol, sums[q, a]), and with
+    weights=True also w[a, p], every basin's yt_weights in grid order;
+    raises on equal neighbouring values."""
     lib = load()
     lib.oracle_yt.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcGeom), C.c_int, C.c_void_p, C.c_void_p,
                               C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
-                              C.c_void_p]
+                              C.c_void_p, C.c_void_p, C.c_int]
     lib.oracle_yt.restype = C.c_int
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
     nn = np.asarray(n, dtype=np.int32)
@@ -528,13 +529,38 @@ def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40):
     vol = np.zeros(cap)
     sums = np.zeros(max(len(fl), 1) * cap)
     na = C.c_int(0)
+    wcap = g.nat + 64 if weights else 0
+    w = np.zeros(wcap * f.size) if weights else None
     rc = lib.oracle_yt(f.ctypes.data, nn.ctypes.data, C.byref(g), len(area), vec.ctypes.data, area.ctypes.data,
                        len(fl), fp, idg.ctypes.data, C.byref(na), xattr.ctypes.data, cap, vol.ctypes.data,
-                       sums.ctypes.data)
+                       sums.ctypes.data, w.ctypes.data if weights else None, wcap)
     if rc:
         raise ValueError(f"oracle_yt: status {rc}")
     k = na.value
-    return idg, k, xattr[:3 * k].reshape(-1, 3), vol[:k], sums[:len(fl) * k].reshape(len(fl), k)
+    out = (idg, k, xattr[:3 * k].reshape(-1, 3), vol[:k], sums[:len(fl) * k].reshape(len(fl), k))
+    if weights:
+        if k > wcap:
+            raise ValueError("oracle_yt: more basins than the weight array holds")
+        out = out + (w[:k * f.size].reshape(k, f.size),)
+    return out
+
+
+def yt_multipoles(w, f, n, x2c, xattr, lmax=5):
+    """YT multipoles from yt(..., weights=True)'s per-basin weights."""
+    lib = load()
+    lib.oracle_yt_multipoles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                         C.c_int, C.c_void_p]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    nn = np.asarray(n, dtype=np.int32)
+    x = _matf(x2c)
+    xa = np.ascontiguousarray(np.asarray(xattr, dtype=np.float64).reshape(-1))
+    na = xa.size // 3
+    nlm = (lmax + 1) ** 2
+    out = np.zeros(na * nlm)
+    lib.oracle_yt_multipoles(w.ctypes.data, f.ctypes.data, nn.ctypes.data, x.ctypes.data, na, xa.ctypes.data,
+                             lmax, out.ctypes.data)
+    return out.reshape(na, nlm)
 
 
 def rrlm(v, lmax):

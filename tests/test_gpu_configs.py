@@ -68,9 +68,12 @@ def _nci_vs_oracle(ctx, cell, sample):
     assert rep["points"] > 0
     # rho: a sum of positive terms, FP32 round-off only
     assert rep["rho_max_rel"] < 1e-5
-    # s and |grad rho|: within a fixed multiple of the cancellation bound,
-    # and within 1e-5 wherever that bound is below 1e-6
-    assert rep["s_over_bound_max"] < 8.0
+    # s and |grad rho|: within a fixed multiple of the cancellation bound
+    # (one rounding per atom term; a term's FP32 evaluation -- distance,
+    # spline, direction scaling -- and the running sum over the atoms in
+    # range take about a dozen roundings, so 16 bounds are allowed), and
+    # within 1e-5 wherever that bound is below 1e-6
+    assert rep["s_over_bound_max"] < 16.0
     assert rep["s_max_rel_well_conditioned"] < 1e-5
     assert rep["s_below_floor_max_abs"] < 1e-5
     return rep

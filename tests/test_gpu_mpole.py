@@ -47,3 +47,61 @@ def test_monopole_is_basin_sum(ctx):
     q = gb.multipoles(ctx, bas.idg, f, cell.n, cell.x2c, bas.xattr[:bas.nattr], 0)
     _, sums = gb.basin_sums(ctx, bas.idg, bas.nattr, [f])
     np.testing.assert_allclose(q[:, 0], np.asarray(sums).reshape(-1), rtol=1e-12)
+
+
+@pytest.mark.parametrize("seed,edge,ang,nat,n,lmax,device", [
+    (81, (8.0, 8.5, 9.0), (90, 90, 90), 4, (32, 34, 36), 5, False),
+    (82, (8.5, 8.0, 9.5), (80, 100, 105), 5, (34, 32, 36), 3, True),
+    (83, (9.0, 9.5, 10.0), (85, 95, 100), 40, (40, 44, 48), 2, False),  # surface points with > 32 basins
+    (84, (7.0, 7.5, 8.0), (85, 95, 100), 4, (24, 26, 28), 0, False),   # monopole only
+])
+def test_yt_multipoles_match_oracle(ctx, seed, edge, ang, nat, n, lmax, device):
+    """YT multipoles (interior points weight 1, surface points their
+    yt_weights weight) against the oracle's loop over every basin's weight
+    field; the monopole is the YT sum of f."""
+    from critic2_amd import _lib
+    from critic2_amd import yt as gy
+    cell = random_cell(seed, edge, ang, nat, (1, 6, 7, 8), n)
+    print(f"[case] yt multipoles seed={seed} n={cell.n} atoms={nat} lmax={lmax} device={device}", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    vec, area = _lib.ws_facets(np.asarray(cell.x2c) / np.array(cell.n)[None, :])
+    idg, nattr, xattr, vol, sums, w = oracle.yt(f, cell.n, cell.x2c, cell.xat, vec, area, fields=[f],
+                                                weights=True)
+    xa = np.asarray(xattr).reshape(-1, 3)[:nattr]
+    ref = oracle.yt_multipoles(w, f, cell.n, cell.x2c, xa, lmax)
+    if device:
+        df = ctx.to_device(f)
+        got_yt = gy.yt_integrate(ctx, df, cell.n, cell.x2c, cell.xat, fields=[df])
+        got = gy.yt_multipoles(ctx, got_yt["labels"], df, cell.n, cell.x2c, got_yt["xattr"], lmax)
+    else:
+        got_yt = gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat, fields=[f])
+        got = gy.yt_multipoles(ctx, got_yt["labels"], f, cell.n, cell.x2c, got_yt["xattr"], lmax)
+    assert got_yt["nattr"] == nattr
+    for l in range(lmax + 1):
+        s = slice(l * l, (l + 1) * (l + 1))
+        scale = np.max(np.abs(ref[:, s]), axis=1, keepdims=True)
+        err = np.abs(got[:, s] - ref[:, s]) / np.maximum(scale, 1e-300)
+        assert err.max() < 1e-10, (l, err.max())
+    np.testing.assert_allclose(got[:, 0], got_yt["sums"][0], rtol=1e-12)
+
+
+def test_yt_multipoles_need_a_yt_call(ctx):
+    """Without a c2gpu_yt call on the context, or with another grid, the
+    entry point refuses."""
+    from critic2_amd import _lib
+    from critic2_amd import yt as gy
+    fresh = _lib.Context(0)
+    try:
+        lab = np.ones(8 * 8 * 8, dtype=np.int32)
+        f = np.ones(8 * 8 * 8)
+        with pytest.raises(_lib.C2GpuError) as e:
+            gy.yt_multipoles(fresh, lab, f, (8, 8, 8), np.eye(3) * 10.0, np.zeros((1, 3)), 2)
+        assert e.value.code == 1
+        cell = random_cell(85, (7.0, 7.0, 7.0), (90, 90, 90), 2, (1, 6), (20, 22, 24))
+        g = gb.synth_slater(fresh, cell)
+        r = gy.yt_integrate(fresh, g, cell.n, cell.x2c, cell.xat)
+        with pytest.raises(_lib.C2GpuError):
+            gy.yt_multipoles(fresh, r["labels"][:20 * 22 * 22], g[:20 * 22 * 22], (20, 22, 22), cell.x2c,
+                             r["xattr"], 2)
+    finally:
+        fresh.close()

