@@ -35,7 +35,7 @@ EXPORTS = (
     "c2gpu_basin_sums", "c2gpu_basin_sums_noalloc", "c2gpu_fft_deriv", "c2gpu_nci", "c2gpu_nci_f32", "c2gpu_batch_run",
     "c2gpu_synth_slater", "c2gpu_cube_read_header", "c2gpu_cube_read", "c2gpu_cube_parse",
     "c2gpu_bincube_read", "c2gpu_cube_format", "c2gpu_cube_write", "c2gpu_nci_cutoff", "c2gpu_ws_facets",
-    "c2gpu_yt", "c2gpu_multipoles", "c2gpu_yt_multipoles", "c2gpu_nci_promol_nodes", "c2gpu_nci_fragments", "c2gpu_nci_cutoff_frag",
+    "c2gpu_yt", "c2gpu_multipoles", "c2gpu_yt_multipoles", "c2gpu_yt_remap", "c2gpu_nci_promol_nodes", "c2gpu_nci_fragments", "c2gpu_nci_cutoff_frag",
 )
 
 
@@ -51,7 +51,8 @@ DISCARD_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.c_void_p)
 
 class AttractorOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("ratom", C.c_double), ("x2c", C.c_double * 9),
-                ("nat", C.c_int), ("xat", C.c_void_p), ("discard", DISCARD_FN), ("discard_user", C.c_void_p)]
+                ("nat", C.c_int), ("xat", C.c_void_p), ("discard", DISCARD_FN), ("discard_user", C.c_void_p),
+                ("nat_atoms", C.c_int)]
 
 
 class BaderStats(C.Structure):
@@ -147,6 +148,7 @@ def load():
         "c2gpu_ws_facets": [vp, C.POINTER(i), vp, vp],
         "c2gpu_multipoles": [vp, vp, vp, vp, vp, i, vp, i, i, vp],
         "c2gpu_yt_multipoles": [vp, vp, vp, vp, vp, i, vp, i, i, vp],
+        "c2gpu_yt_remap": [vp, vp, i, i],
         "c2gpu_yt": [vp, vp, vp, vp, C.POINTER(AttractorOpts), i, vp, i, vp, C.POINTER(i), vp, i, vp, vp,
                      C.POINTER(YtStats)],
     }

@@ -41,13 +41,35 @@ def _mode(nnm: bool, ratom, noatoms: bool):
     return _lib.ATTR_NEAREST_ATOM, 1e40
 
 
+def _attractor_opts(mode, ratom, x2c, xat, cps=None):
+    """c2gpu_attractor_opts for atoms xat and other known critical points
+    cps (both fractional); returns the struct and the array it points to."""
+    xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1, 3)
+    opts = _lib.AttractorOpts()
+    opts.mode = mode
+    opts.ratom = ratom
+    for q, v in enumerate(_lib.mat_f(x2c)):
+        opts.x2c[q] = v
+    opts.nat_atoms = xat.shape[0]
+    if cps is not None and len(cps):
+        xat = np.concatenate([xat, np.asarray(cps, dtype=np.float64).reshape(-1, 3)])
+    opts.nat = xat.shape[0]
+    xat_flat = np.ascontiguousarray(xat.reshape(-1))
+    opts.xat = xat_flat.ctypes.data
+    return opts, xat_flat
+
+
 def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noatoms=False,
-                    car2lat=None, lat_i_dist=None, labels_out=None, discard=None) -> BasinData:
+                    car2lat=None, lat_i_dist=None, labels_out=None, discard=None, cps=None) -> BasinData:
     """Near-grid Bader labelling of the FP64 grid rho (x fastest, n1*n2*n3).
 
     discard (BADER's DISCARD keyword, bader@proc.f90:185-191): None, or a
     function of the Cartesian position (bohr) of a maximum that would become
     a new attractor, true to discard it; its basin keeps label 0.
+
+    cps: the field's critical points other than the atoms (fractional,
+    (m, 3), cpcel order after the atoms), known attractors before the sweep
+    as in bader@proc.f90:109-117; atoms keep labels 1..nat, cps nat+1...
 
     rho may be a numpy array (host) or a DeviceArray resident on the GPU;
     labels_out (optional) may likewise be a device int32 DeviceArray that
@@ -58,15 +80,7 @@ def bader_integrate(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noato
     mode, ra = _mode(nnm, ratom, noatoms)
     if car2lat is None or lat_i_dist is None:
         _, car2lat, lat_i_dist = _lib.bader_metrics(x2c, n)
-    xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1, 3)
-    opts = _lib.AttractorOpts()
-    opts.mode = mode
-    opts.ratom = ra
-    for q, v in enumerate(_lib.mat_f(x2c)):
-        opts.x2c[q] = v
-    opts.nat = xat.shape[0]
-    xat_flat = np.ascontiguousarray(xat.reshape(-1))
-    opts.xat = xat_flat.ctypes.data
+    opts, xat_flat = _attractor_opts(mode, ra, x2c, xat, cps)
     cb = None
     if discard is not None:
         cb = _lib.DISCARD_FN(lambda x, user: 1 if discard(np.array([x[0], x[1], x[2]])) else 0)
@@ -155,6 +169,109 @@ def intgrid_fields(ctx: Context, bas: BasinData, rho, x2c, lap=True):
     if lap:
         out["lap"] = sums[1] * w
     return out
+
+
+TYP_NUC = -3  # critical-point type of maxima (atoms and non-nuclear maxima)
+
+
+def _shortest_dist(x2c, xd):
+    """Length of the shortest lattice translate of the fractional vector xd
+    (crystal%shortest, here by search over translations -2..2 after the
+    nint fold; exact for the reduced cells of this path)."""
+    m = np.asarray(x2c, dtype=np.float64)
+    x0 = np.asarray(xd, dtype=np.float64) - np.round(xd)
+    t = np.array([(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)], dtype=np.float64)
+    v = (x0[None, :] + t) @ m.T
+    return float(np.sqrt(np.min(np.sum(v * v, axis=1))))
+
+
+def reorder_gridout(ctx: Context, labels, xattr, x2c, cpcel, cptyp, nat_atoms, ratom, atexist=True, yt=False):
+    """int_reorder_gridout (integration@proc.f90:1027-1166) for BADER and YT:
+    the map of reorder_map applied to the labels (numpy array or
+    DeviceArray, in place) through c2gpu_relabel (the `where` loop,
+    :1113-1122) and with yt=True to the surface weights kept by the last
+    c2gpu_yt call through c2gpu_yt_remap (:1125-1158).  Returns
+    reorder_map's dict."""
+    r = reorder_map(xattr, x2c, cpcel, cptyp, nat_atoms, ratom, atexist)
+    assigned = r["assigned"]
+    nattr0 = assigned.size
+    if np.any(assigned != np.arange(1, nattr0 + 1)):
+        dev = hasattr(labels, "data_ptr")
+        lab = labels if dev else labels.reshape(-1)
+        if not dev and not (lab.dtype == np.int32 and lab.flags.c_contiguous):
+            raise ValueError("reorder_gridout: host labels must be a contiguous int32 array")
+        npts = lab.numel() if dev else lab.size
+        ctx.check(ctx.lib.c2gpu_relabel(ctx.h, npts, assigned.ctypes.data, nattr0,
+                                        _lib.PTR_DEVICE if dev else 0, _lib.ptr(lab)), "c2gpu_relabel")
+    if yt:
+        ctx.check(ctx.lib.c2gpu_yt_remap(ctx.h, assigned.ctypes.data, nattr0, r["nattr"]), "c2gpu_yt_remap")
+    return r
+
+
+def reorder_map(xattr, x2c, cpcel, cptyp, nat_atoms, ratom, atexist=True):
+    """The attractor map of int_reorder_gridout (integration@proc.f90:
+    1027-1110), host logic.
+
+    The attractors found (xattr, fractional, (nattr0, 3)) are matched to the
+    field's critical points cpcel ((ncpcel, 3), types cptyp, the first
+    nat_atoms being the atoms): an atom within ratom (identify_atom), else
+    the nearest maximum-type point if nearer than ratom (nearest_cp with
+    type typnuc); the critical points so matched become attractors 1.. in
+    cpcel order, the others new non-nuclear maxima after them, merged within
+    ratom (are_lclose).  As in the reference, attractors matched to a
+    critical point keep that point's cpcel index as their new label.
+    Returns dict(nattr, xattr, icp, assigned, new_cps): the new attractor
+    count and positions, icp (cpcel index, 0 for a new maximum), assigned
+    (new label of each old attractor) and the positions that ff%addcp would
+    add to the critical points."""
+    xattr = np.asarray(xattr, dtype=np.float64).reshape(-1, 3)
+    cpcel = np.asarray(cpcel, dtype=np.float64).reshape(-1, 3)
+    cptyp = np.asarray(cptyp, dtype=np.int32).reshape(-1)
+    nattr0 = xattr.shape[0]
+    assigned = np.zeros(nattr0, dtype=np.int32)
+    if atexist:
+        for i in range(nattr0):
+            # identify_atom(x, icrd_crys, distmax=ratom): the nearest atom
+            # (first on ties) if at most ratom away
+            d = [_shortest_dist(x2c, xattr[i] - cpcel[a]) for a in range(nat_atoms)]
+            nid = 0
+            if d:
+                a = int(np.argmin(d))
+                if d[a] <= ratom:
+                    nid = a + 1
+            if nid == 0:
+                # nearest_cp(x, nid, dist, type=typnuc): strict < keeps the first
+                best, bd = 0, 1e30
+                for j in range(cpcel.shape[0]):
+                    if cptyp[j] != TYP_NUC:
+                        continue
+                    dj = _shortest_dist(x2c, cpcel[j] - xattr[i])
+                    if dj < bd:
+                        best, bd = j + 1, dj
+                if bd < ratom:
+                    nid = best
+            assigned[i] = nid
+    nattr, icp, xnew, new_cps = 0, [], [], []
+    for i in range(1, cpcel.shape[0] + 1):
+        if np.any(assigned == i):
+            nattr += 1
+            icp.append(i)
+            xnew.append(cpcel[i - 1])
+    for i in range(nattr0):
+        if assigned[i] > 0:
+            continue
+        nattr += 1
+        assigned[i] = nattr
+        icp.append(0)
+        xnew.append(xattr[i])
+        for j in range(i + 1, nattr0):
+            if assigned[j] > 0:
+                continue
+            if _shortest_dist(x2c, xattr[i] - xattr[j]) < ratom:
+                assigned[j] = nattr
+        new_cps.append(xattr[i])
+    return {"nattr": nattr, "xattr": np.array(xnew).reshape(-1, 3), "icp": np.array(icp, dtype=np.int32),
+            "assigned": assigned, "new_cps": np.array(new_cps).reshape(-1, 3)}
 
 
 def bader(ctx: Context, rho, n, x2c, xat, nnm=False, ratom=None, noatoms=False, lap=True):

@@ -44,6 +44,7 @@ void mpole_run(c2gpu_ctx *, const int32_t *, const double *, const int *, const 
                int, double *);
 void yt_mpole_run(c2gpu_ctx *, const int32_t *, const double *, const int *, const double *, int, const double *, int,
                int, double *);
+void yt_remap_run(c2gpu_ctx *, const int32_t *, int, int);
 void cube_format_run(c2gpu_ctx *, const double *, const int *, int, int, int, char *, uint64_t, uint64_t *);
 void cube_write_run(c2gpu_ctx *, const char *, const c2gpu_cube_header *, const int *, const double *,
                     const double *, int, int);
@@ -141,10 +142,16 @@ static double lattice_dist(const double x2c[9], const double xd[3]) {
   return best;
 }
 
+// the atoms among the known attractors (the leading nat_atoms entries)
+static int atom_count(const c2gpu_attractor_opts *o) {
+  return o->nat_atoms > 0 && o->nat_atoms < o->nat ? o->nat_atoms : o->nat;
+}
+
 int nearest_atom_label(const c2gpu_attractor_opts *o, const double x[3]) {
   int best = 0;
   double bd = DBL_MAX;
-  for (int a = 0; a < o->nat; a++) {
+  const int natoms = atom_count(o);
+  for (int a = 0; a < natoms; a++) {
     const double xd[3] = {x[0] - o->xat[3 * a], x[1] - o->xat[3 * a + 1], x[2] - o->xat[3 * a + 2]};
     const double d = lattice_dist(o->x2c, xd);
     if (d < bd) {
@@ -162,7 +169,8 @@ int nearest_atom_label(const c2gpu_attractor_opts *o, const double x[3]) {
 static int identify_atom_within(const c2gpu_attractor_opts *o, const double x[3], double distmax) {
   int best = 0;
   double bd = DBL_MAX;
-  for (int a = 0; a < o->nat; a++) {
+  const int natoms = atom_count(o);
+  for (int a = 0; a < natoms; a++) {
     const double xd[3] = {x[0] - o->xat[3 * a], x[1] - o->xat[3 * a + 1], x[2] - o->xat[3 * a + 2]};
     const double d = lattice_dist(o->x2c, xd);
     if (d < bd) {
@@ -614,6 +622,10 @@ int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
 int c2gpu_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],
                      int nattr, const double *xattr, int lmax, int flags, double *mpole_out) {
   return guarded(ctx, [&] { mpole_run(ctx, label, f, n, x2c, nattr, xattr, lmax, flags, mpole_out); });
+}
+
+int c2gpu_yt_remap(c2gpu_ctx *ctx, const int32_t *map, int nmap, int nattr_new) {
+  return guarded(ctx, [&] { yt_remap_run(ctx, map, nmap, nattr_new); });
 }
 
 int c2gpu_yt_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3], const double x2c[9],

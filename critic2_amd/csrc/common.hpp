@@ -56,7 +56,8 @@ struct c2gpu_ctx {
     int nattr = 0;
     uint32_t nias = 0;
     uint64_t nent = 0;  // weight entries over all surface points
-    const uint32_t *ias = nullptr, *wb = nullptr;
+    const uint32_t *ias = nullptr;
+    uint32_t *wb = nullptr;  // basin ids, remapped by c2gpu_yt_remap
     const uint64_t *woff = nullptr;
     const uint16_t *wc = nullptr;
     const double *ww = nullptr;

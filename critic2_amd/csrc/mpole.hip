@@ -21,11 +21,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
 
 namespace c2 {
+
+int ws_facets(const double m[9], int vec[3 * 14], double area[14]);
 
 namespace {
 
@@ -215,11 +218,24 @@ void mpole_impl(c2gpu_ctx *ctx, const char *who, bool yt, const int32_t *label, 
   G.lmax = lmax;
   G.nattr = nattr;
   for (int q = 0; q < 9; q++) G.x2c[q] = x2c[q];
-  // orthogonal cells: the nint fold is already the shortest translate
-  const double d01 = x2c[0] * x2c[3] + x2c[1] * x2c[4] + x2c[2] * x2c[5];
-  const double d02 = x2c[0] * x2c[6] + x2c[1] * x2c[7] + x2c[2] * x2c[8];
-  const double d12 = x2c[3] * x2c[6] + x2c[4] * x2c[7] + x2c[5] * x2c[8];
-  G.ortho = d01 == 0.0 && d02 == 0.0 && d12 == 0.0;
+  // crystal%shortest takes the nint fold alone when the cell is orthogonal
+  // (isortho: a Wigner-Seitz cell of at most 6 facets, each across a single
+  // lattice vector, tools@proc.f90:842-849), and searches neighbouring
+  // translates otherwise
+  {
+    int vec[42];
+    double area[14];
+    const int nf = ws_facets(x2c, vec, area);
+    G.ortho = nf <= 6;
+    for (int q = 0; q < nf && G.ortho; q++) {
+      int ones = 0, zeros = 0;
+      for (int c = 0; c < 3; c++) {
+        ones += std::abs(vec[3 * q + c]) == 1;
+        zeros += vec[3 * q + c] == 0;
+      }
+      G.ortho = ones == 1 && zeros == 2;
+    }
+  }
   const int32_t *d_lab = label;
   const double *d_f = f;
   if (flags & C2GPU_PTR_DEVICE) {

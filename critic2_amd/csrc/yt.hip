@@ -592,6 +592,20 @@ struct FrontCounts {
   }
 };
 
+// The basin ids of the stored surface weights through map (1-based ids).
+__global__ void __launch_bounds__(YB) k_yt_remap(uint32_t nias, const uint64_t *__restrict__ woff,
+                                                  const uint16_t *__restrict__ wc, uint32_t *__restrict__ wb,
+                                                  const int32_t *__restrict__ map, uint32_t nmap) {
+  const uint32_t t = blockIdx.x * YB + threadIdx.x;
+  if (t >= nias) return;
+  const uint64_t so = woff[t];
+  const int ce = wc[t];
+  for (int e = 0; e < ce; e++) {
+    const uint32_t b = wb[so + e];
+    if (b >= 1 && b <= nmap) wb[so + e] = (uint32_t)map[b - 1];
+  }
+}
+
 }  // namespace
 
 void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9], const c2gpu_attractor_opts *opts,
@@ -820,6 +834,30 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   ctx->yt.ww = W.w;
   S.ms_total = ttot.stop();
   if (stats) *stats = S;
+}
+
+// The YT checkpoint update of int_reorder_gridout (integration@proc.f90:
+// 1125-1158): ibasin2 = assigned(ibasin) on the surface weights kept by the
+// last c2gpu_yt call; afterwards they belong to nattr_new attractors.
+void yt_remap_run(c2gpu_ctx *ctx, const int32_t *map, int nmap, int nattr_new) {
+  auto &Y = ctx->yt;
+  if (!map) fail(C2GPU_ERR_ARG, "c2gpu_yt_remap: NULL map");
+  if (!Y.valid) fail(C2GPU_ERR_ARG, "c2gpu_yt_remap: no c2gpu_yt call on this context has left its weights");
+  if (nmap != Y.nattr)
+    fail(C2GPU_ERR_ARG, "c2gpu_yt_remap: inconsistent number of attractors in the yt weights (" +
+                            std::to_string(nmap) + " vs " + std::to_string(Y.nattr) + ")");
+  if (nattr_new < 1) fail(C2GPU_ERR_ARG, "c2gpu_yt_remap: nattr_new >= 1");
+  for (int i = 0; i < nmap; i++)
+    if (map[i] < 1 || map[i] > nattr_new) fail(C2GPU_ERR_ARG, "c2gpu_yt_remap: map entries must lie in 1..nattr_new");
+  hipStream_t st = ctx->stream;
+  if (Y.nias) {
+    int32_t *d_map = buf<int32_t>(ctx, "yt_map", (size_t)nmap);
+    C2_HIP(hipMemcpyAsync(d_map, map, 4 * (size_t)nmap, hipMemcpyHostToDevice, st));
+    C2_LAUNCH(ctx, k_yt_remap, dim3(grid_for(Y.nias, YB)), dim3(YB), 0, Y.nias, Y.woff, Y.wc,
+              Y.wb, d_map, (uint32_t)nmap);
+    C2_HIP(hipStreamSynchronize(st));
+  }
+  Y.nattr = nattr_new;
 }
 
 }  // namespace c2

@@ -80,7 +80,9 @@ contains
        end do
     end do
 
-    ! attractor options as intgrid_driver sets them (integration@proc.f90:236-248)
+    ! attractor options as intgrid_driver sets them (integration@proc.f90:236-248);
+    ! the known attractors are the field's critical points, atoms first
+    ! (bader@proc.f90:109-117), and only the atoms are identify_atom's
     allocate(xat(3,max(s%f(iref)%ncpcel,1)))
     xat = 0d0
     do i = 1, s%f(iref)%ncpcel
@@ -95,6 +97,7 @@ contains
     opts%ratom = bas%ratom
     opts%x2c = reshape(s%c%m_x2c,(/9/))
     opts%nat = s%f(iref)%ncpcel
+    opts%nat_atoms = s%c%ncel
     opts%xat = c_loc(xat)
     if (len_trim(bas%expr) > 0) then
        disc_sys => s

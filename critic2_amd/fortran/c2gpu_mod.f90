@@ -28,6 +28,9 @@ module c2gpu
      ! nonzero to discard the new maximum; c_null_funptr for none
      type(c_funptr) :: discard = c_null_funptr
      type(c_ptr) :: discard_user = c_null_ptr
+     ! leading entries of xat that are atoms (s%c%ncel); the rest are the
+     ! field's other critical points (cpcel order); 0: all are atoms
+     integer(c_int) :: nat_atoms = 0
   end type c2gpu_attractor_opts
 
   type, bind(c), public :: c2gpu_bader_stats
@@ -51,7 +54,7 @@ module c2gpu
   public :: c2gpu_ctx_create, c2gpu_ctx_destroy, c2gpu_bader_metrics, c2gpu_bader
   public :: c2gpu_basin_sums, c2gpu_fft_deriv, c2gpu_relabel
   public :: c2gpu_cube_read_header, c2gpu_cube_read, c2gpu_bincube_read, c2gpu_cube_write
-  public :: c2gpu_nci_cutoff, c2gpu_multipoles, c2gpu_yt_multipoles
+  public :: c2gpu_nci_cutoff, c2gpu_multipoles, c2gpu_yt_multipoles, c2gpu_yt_remap
   public :: c2gpu_nci, c2gpu_nci_promol_nodes, c2gpu_nci_fragments, c2gpu_nci_cutoff_frag
 
   integer(c_int), parameter, public :: C2GPU_FP64 = 0, C2GPU_FP32 = 1
@@ -317,6 +320,14 @@ module c2gpu
        real(c_double), intent(out) :: mpole_out(*)
        integer(c_int) :: c2gpu_yt_multipoles
      end function c2gpu_yt_multipoles
+
+     function c2gpu_yt_remap(ctx, map, nmap, nattr_new) bind(c, name="c2gpu_yt_remap")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: map(*)
+       integer(c_int), value :: nmap, nattr_new
+       integer(c_int) :: c2gpu_yt_remap
+     end function c2gpu_yt_remap
 
      function c2gpu_ctx_error(ctx) bind(c, name="c2gpu_ctx_error")
        import :: c_ptr

@@ -11,28 +11,22 @@ import numpy as np
 
 from . import _lib
 from ._lib import Context
-from .bader import _mode
+from .bader import _attractor_opts, _mode
 
 
 def yt_integrate(ctx: Context, rho, n, x2c, xat, fields=(), nnm=False, ratom=None, noatoms=False,
-                 labels=True) -> dict:
+                 labels=True, cps=None) -> dict:
     """Basins and weights of the FP64 grid rho (x fastest; numpy array or
     DeviceArray, as are the fields).  Returns nattr, xattr (fractional),
     labels (basin per point in grid order, 0 on the interatomic surfaces),
     vol (sum of each basin's weights) and sums[q, a] (sum of weights times
-    fields[q]); vol and sums are unscaled (critic2 multiplies by Omega/N)."""
+    fields[q]); vol and sums are unscaled (critic2 multiplies by Omega/N).
+    cps: known critical points other than the atoms, as for
+    bader.bader_integrate."""
     n = tuple(int(v) for v in n)
     npts = n[0] * n[1] * n[2]
     mode, ra = _mode(nnm, ratom, noatoms)
-    xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1, 3)
-    opts = _lib.AttractorOpts()
-    opts.mode = mode
-    opts.ratom = ra
-    for q, v in enumerate(_lib.mat_f(x2c)):
-        opts.x2c[q] = v
-    opts.nat = xat.shape[0]
-    xat_flat = np.ascontiguousarray(xat.reshape(-1))
-    opts.xat = xat_flat.ctypes.data
+    opts, xat_flat = _attractor_opts(mode, ra, x2c, xat, cps)
     dev = hasattr(rho, "data_ptr")
     flags = _lib.PTR_DEVICE if dev else 0
     if not dev:

@@ -122,14 +122,24 @@ enum {
  * It may be called once or more per maximum and must depend on x only. */
 typedef int (*c2gpu_discard_fn)(const double x[3], void *user);
 
+/* The attractors known before the sweep are the field's critical points,
+ * cpcel(1:ncpcel) (bader@proc.f90:109-117): the atoms of the complete cell
+ * list first, then any other critical points an earlier run added (known
+ * non-nuclear maxima, or the bond/ring/cage points of a critical-point
+ * search).  A new maximum takes the nearest ATOM within ratom
+ * (identify_atom: only the first nat_atoms entries), else the first entry or
+ * earlier new attractor within ratom (are_lclose over all of them). */
 typedef struct {
   int mode;           /* C2GPU_ATTR_*                                      */
   double ratom;       /* bohr (ignored for NEAREST_ATOM)                   */
   double x2c[9];      /* lattice vectors as columns (bohr), crystal m_x2c  */
-  int nat;            /* atoms of the complete cell list (cpcel order)     */
-  const double *xat;  /* fractional atom coordinates, 3*nat, host memory   */
+  int nat;            /* entries of xat: ncpcel (the atoms when no other
+                         critical points are known)                        */
+  const double *xat;  /* fractional cpcel coordinates, 3*nat, host memory  */
   c2gpu_discard_fn discard; /* DISCARD filter, or NULL (none)              */
   void *discard_user;       /* passed to discard                           */
+  int nat_atoms;      /* leading entries of xat that are atoms (ncel);
+                         <= 0: all nat entries are atoms                   */
 } c2gpu_attractor_opts;
 
 typedef struct {
@@ -442,6 +452,15 @@ int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
              int32_t *label_out, int *nattr_out, double *xattr_out, int xattr_cap, double *vol_out,
              double *sums_out, c2gpu_yt_stats *stats);
 
+/* The YT checkpoint update of int_reorder_gridout (integration@proc.f90:
+ * 1125-1158): the surface weights kept by the last c2gpu_yt call change
+ * basin ids through map (ibasin2 = assigned(ibasin)); nmap must be that
+ * call's nattr ("inconsistent number of attractors in yt checkpoint"
+ * otherwise, C2GPU_ERR_ARG) and every map entry lie in 1..nattr_new.
+ * Weights of merged basins add, as yt_weights over the remapped checkpoint
+ * gives them.  The labels are remapped by c2gpu_relabel with the same map. */
+int c2gpu_yt_remap(c2gpu_ctx *ctx, const int32_t *map, int nmap, int nattr_new);
+
 /* Basin multipoles of a YT partition (INTEGRABLE f MULTIPOLE lmax,
  * integration@proc.f90:1314-1336, the imtype_yt branch):
  * mpole_out[a*(lmax+1)^2 + q] = sum over the points p with |w_a(p)| >= 1e-15
@@ -449,8 +468,12 @@ int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c
  * interior points, the surface weight elsewhere), rrlm as in
  * c2gpu_multipoles.  The surface weights are those of the last c2gpu_yt
  * call on this context, which keeps them as critic2 keeps its YT checkpoint
- * for later yt_weights calls; label is that call's label_out, and n and
- * nattr must be that call's (else C2GPU_ERR_ARG).  Every basin is computed:
+ * for later yt_weights calls (after c2gpu_yt_remap, with their new ids);
+ * label is that call's label_out (relabelled the same way), and n and nattr
+ * must be that call's, or c2gpu_yt_remap's nattr_new (else C2GPU_ERR_ARG).
+ * A surface point's entries are skipped one by one below 1e-15, where the
+ * reference tests their sum for merged basins (a difference below 1e-15 of
+ * the point's contribution).  Every basin is computed:
  * critic2 leaves those whose attractor it does not integrate
  * (docelatom) at zero.  Flags, layout and units as c2gpu_multipoles. */
 int c2gpu_yt_multipoles(c2gpu_ctx *ctx, const int32_t *label, const double *f, const int n[3],

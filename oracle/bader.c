@@ -395,7 +395,8 @@ static double lattice_dist(const double x2c[9], const double xd[3]) {
 static int identify_atom(const orc_geom *g, const double x[3], double distmax) {
   int best = 0;
   double bd = DBL_MAX;
-  for (int a = 0; a < g->nat; a++) {
+  const int natoms = g->nat_atoms > 0 && g->nat_atoms < g->nat ? g->nat_atoms : g->nat;
+  for (int a = 0; a < natoms; a++) {
     double xd[3] = {x[0] - g->xat[3 * a], x[1] - g->xat[3 * a + 1], x[2] - g->xat[3 * a + 2]};
     double d = lattice_dist(g->x2c, xd);
     if (d < bd) { bd = d; best = a + 1; }
@@ -949,6 +950,25 @@ void oracle_tosphere(const double v[3], double *r, double tp[2]) {
 /* The Cartesian shortest lattice translate of p/n - xattr (crystal%shortest,
  * crystalmod@proc.f90:1054-1085), by the nint fold and a search over the
  * neighbouring translates. */
+/* crystal%isortho for the cells used here: the lattice vectors are
+ * mutually orthogonal (then the Wigner-Seitz cell is the box of 6 facets
+ * across +-a, +-b, +-c that tools@proc.f90:842-849 tests for), to 1e-12 of
+ * the product of their lengths. */
+static int cell_is_ortho(const double x2c[9]) {
+  const double *a = x2c, *b = x2c + 3, *c = x2c + 6;
+  const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  const double lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  const double lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  const double ac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+  const double bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+  return fabs(ab) <= 1e-12 * la * lb && fabs(ac) <= 1e-12 * la * lc && fabs(bc) <= 1e-12 * lb * lc;
+}
+
+/* crystal%shortest (crystalmod@proc.f90:1054-1085) of p/n - xa, Cartesian:
+ * the nint fold; for a non-orthogonal cell then the first strictly shorter
+ * of the translates x0 + lattice vector (Cartesian addition, as the
+ * reference adds ws_ineighc), over the translates -2..2. */
 static void shortest_offset(const int n[3], const double x2c[9], int64_t i, int64_t j, int64_t k,
                             const double *xa, double bv[3]) {
   double dv[3] = {(double)i / n[0] - xa[0], (double)j / n[1] - xa[1], (double)k / n[2] - xa[2]};
@@ -957,11 +977,12 @@ static void shortest_offset(const int n[3], const double x2c[9], int64_t i, int6
   x2c_apply(x2c, dv, v);
   double best = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   bv[0] = v[0]; bv[1] = v[1]; bv[2] = v[2];
+  if (cell_is_ortho(x2c)) return;
   for (int a = -2; a <= 2; a++)
     for (int b = -2; b <= 2; b++)
       for (int c = -2; c <= 2; c++) {
-        double xt[3] = {dv[0] + a, dv[1] + b, dv[2] + c}, t[3];
-        x2c_apply(x2c, xt, t);
+        double t[3];
+        for (int e = 0; e < 3; e++) t[e] = v[e] + x2c[e] * a + x2c[e + 3] * b + x2c[e + 6] * c;
         const double d = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
         if (d < best) {
           best = d;

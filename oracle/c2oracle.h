@@ -37,6 +37,10 @@ typedef struct {
    * Cartesian x; NULL: no expression */
   int (*discard)(const double x[3], void *user);
   void *discard_user;
+  /* with atexist the known attractors are the field's critical points
+   * cpcel(1:nat) (bader@proc.f90:109-117); the first nat_atoms are the
+   * atoms identify_atom looks at (<= 0: all nat) */
+  int nat_atoms;
 } orc_geom;
 
 /* Mutation switches of the Bader restatement (tests only; 0 = the

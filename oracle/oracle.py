@@ -48,7 +48,8 @@ DISCARD_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.c_void_p)
 
 class OrcGeom(C.Structure):
     _fields_ = [("x2c", C.c_double * 9), ("nat", C.c_int), ("xat", C.c_void_p), ("mode", C.c_int),
-                ("ratom", C.c_double), ("discard", DISCARD_FN), ("discard_user", C.c_void_p)]
+                ("ratom", C.c_double), ("discard", DISCARD_FN), ("discard_user", C.c_void_p),
+                ("nat_atoms", C.c_int)]
 
 
 class OrcInfo(C.Structure):
@@ -138,11 +139,13 @@ def metrics(x2c, n):
     return l2c, c2l, lid
 
 
-def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, lat_i_dist=None, discard=None):
+def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, lat_i_dist=None, discard=None,
+          cps=None):
     """Serial bader_integrate; returns (labels int32 x-fastest, info dict,
     xattr (nattr,3), known or None).  discard: None or a function of the
     Cartesian position (bohr) of a new maximum, true to discard it (BADER's
-    DISCARD expression, bader@proc.f90:185-191)."""
+    DISCARD expression, bader@proc.f90:185-191).  cps: the field's other
+    known critical points (fractional, after the atoms in cpcel order)."""
     lib = load()
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
     nn = np.asarray(n, dtype=np.int32)
@@ -152,6 +155,9 @@ def bader(f, n, x2c, xat, mode=0, ratom=1e40, sweep_only=False, car2lat=None, la
     lid = np.ascontiguousarray(lat_i_dist, dtype=np.float64)
     xa = np.ascontiguousarray(np.asarray(xat, dtype=np.float64).reshape(-1))
     g = OrcGeom()
+    g.nat_atoms = len(xa) // 3
+    if cps is not None and len(cps):
+        xa = np.ascontiguousarray(np.concatenate([xa, np.asarray(cps, dtype=np.float64).reshape(-1)]))
     for q, v in enumerate(_matf(x2c)):
         g.x2c[q] = v
     g.nat = len(xa) // 3
@@ -499,7 +505,7 @@ def nci_cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e
     return dat, cg, sums
 
 
-def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40, weights=False):
+def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40, weights=False, cps=None):
     """yt_integrate + yt_weights + the imtype_yt sums (oracle_yt).  Returns
     (labels in grid order, nattr, xattr, vol, sums[q, a]), and with
     weights=True also w[a, p], every basin's yt_weights in grid order;
@@ -513,6 +519,9 @@ def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40, weights=False):
     nn = np.asarray(n, dtype=np.int32)
     xat = np.ascontiguousarray(xat, dtype=np.float64).reshape(-1)
     g = OrcGeom()
+    g.nat_atoms = xat.size // 3
+    if cps is not None and len(cps):
+        xat = np.ascontiguousarray(np.concatenate([xat, np.asarray(cps, dtype=np.float64).reshape(-1)]))
     for q, v in enumerate(_matf(x2c)):
         g.x2c[q] = v
     g.nat = xat.size // 3
@@ -591,3 +600,78 @@ def multipoles(idg, f, n, x2c, xattr, lmax=5):
     lib.oracle_multipoles(idg.ctypes.data, f.ctypes.data, nn.ctypes.data, _matf(x2c).ctypes.data, xattr.shape[0],
                           xattr.ctypes.data, lmax, out.ctypes.data)
     return out.reshape(xattr.shape[0], -1)
+
+
+# ---------------------------------------------------------------- attractor reordering
+
+def _shortest(x2c, xd):
+    """crystal%shortest length of a fractional difference (search over
+    translations -2..2 after the nint fold)."""
+    x0 = np.asarray(xd, dtype=np.float64) - np.round(xd)
+    best = np.inf
+    for a in range(-2, 3):
+        for b in range(-2, 3):
+            for c in range(-2, 3):
+                v = np.asarray(x2c, dtype=np.float64) @ (x0 + np.array([a, b, c], dtype=np.float64))
+                best = min(best, float(np.sqrt(v @ v)))
+    return best
+
+
+def reorder_gridout(xattr, x2c, cpcel, cptyp, nat_atoms, ratom, atexist=True):
+    """int_reorder_gridout (integration@proc.f90:1027-1122), Bader/YT branch,
+    line by line: returns (assigned, nattr, icp, xattr_new).  assigned[i] is
+    the new label of attractor i+1; an attractor matched to critical point k
+    keeps label k (the reference does not compact those ids)."""
+    xattr = np.asarray(xattr, dtype=np.float64).reshape(-1, 3)
+    cpcel = np.asarray(cpcel, dtype=np.float64).reshape(-1, 3)
+    nattr0 = len(xattr)
+    assigned = [0] * nattr0
+    if atexist:                                              # :1069-1082
+        for i in range(nattr0):
+            nid, bd = 0, np.inf
+            for a in range(nat_atoms):                       # identify_atom, distmax = ratom
+                d = _shortest(x2c, xattr[i] - cpcel[a])
+                if d < bd:
+                    nid, bd = a + 1, d
+            if nid and bd <= ratom:
+                assigned[i] = nid
+            else:
+                nid, dist = 0, 1e30                          # nearest_cp(..., type=typnuc)
+                for j in range(len(cpcel)):
+                    if cptyp[j] != -3:
+                        continue
+                    d = _shortest(x2c, cpcel[j] - xattr[i])
+                    if d < dist:
+                        nid, dist = j + 1, d
+                if dist < ratom:
+                    assigned[i] = nid
+    nattr, icp, xnew = 0, [], []
+    for i in range(1, len(cpcel) + 1):                       # :1084-1091
+        if i in assigned:
+            nattr += 1
+            icp.append(i)
+            xnew.append(cpcel[i - 1])
+    for i in range(nattr0):                                  # :1094-1110
+        if assigned[i] > 0:
+            continue
+        nattr += 1
+        assigned[i] = nattr
+        icp.append(0)
+        xnew.append(xattr[i])
+        for j in range(i + 1, nattr0):
+            if assigned[j] > 0:
+                continue
+            if _shortest(x2c, xattr[i] - xattr[j]) < ratom:
+                assigned[j] = nattr
+    return np.array(assigned, dtype=np.int32), nattr, np.array(icp, dtype=np.int32), np.array(xnew).reshape(-1, 3)
+
+
+def relabel(idg, assigned):
+    """The `where` loop of int_reorder_gridout (:1113-1122): every i with
+    assigned(i) /= i maps the labels of the ORIGINAL idg."""
+    idg = np.asarray(idg)
+    out = idg.copy()
+    for i, a in enumerate(assigned, start=1):
+        if a != i:
+            out[idg == i] = a
+    return out
