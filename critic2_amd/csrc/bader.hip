@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -47,20 +48,56 @@ struct NoVisit {
 // Free trajectory (no known==2 stops) of every start, walked once into the
 // thread's scratch (through an LDS ring) and appended to a pool through a
 // bump allocator; pos[s] / len[s] locate it.  A pool overflow only counts
-// the needed size.  Launched with TRAJ_B threads per workgroup.  (The lanes
-// of a wavefront walk neighbouring starts side by side; letting each lane
-// take its next start as soon as its own path ends was measured 2.4x
-// slower: the lanes drift apart on the grid and the path copies interleave.)
+// the needed size.  Persistent, TRAJ_B threads per workgroup.
+//
+// Walk order: a wavefront takes a row of 64 neighbouring starts along x
+// from a queue, and the lanes walk side by side.  The rows are ordered in
+// blocks of 64 x 64 x 64 points (blocks x fastest), within a block in
+// 4 x 4 groups of rows, so the walks in flight start in a region about
+// 128 x 64 x 64 points wide whose ascents re-read the direction records of
+// one neighbourhood.  At 512^3 this takes the walk from 389 ms (each thread
+// striding over the grid in grid order) to 261 ms, at 768^3 from 1627 ms
+// to 1228 ms; block edges of 16 to 128 points and row groups of 2 to 16
+// measured the same.  (Letting each lane take its next start as soon as
+// its own path ends was measured 2.4x slower: the lanes drift apart on the
+// grid and the path copies interleave.)
 constexpr int TRAJ_B = 256;
-__global__ void __launch_bounds__(TRAJ_B) k_traj(BParams P, uint32_t N, uint32_t *scratch, int maxlen,
-                                                 uint32_t *__restrict__ pool, unsigned long long pool_cap,
-                                                 unsigned long long *pool_top, uint64_t *__restrict__ pos,
-                                                 uint32_t *__restrict__ len, int *err) {
+struct TrajOrder {
+  uint32_t nbi, nbj, nbk;  // blocks along x, y, z
+  uint32_t bj, bk;         // block edge along y and z (points); along x 64
+  uint32_t g;              // rows per group edge (g x g rows a group)
+  uint32_t nrows;          // nbi * nbj * nbk * bj * bk
+};
+__device__ __forceinline__ bool traj_row(const BParams &P, const TrajOrder &T, uint32_t r, int &i0, int &j,
+                                         int &k) {
+  const uint32_t per = T.bj * T.bk;
+  const uint32_t b = r / per, q = r % per;
+  const uint32_t bi = b % T.nbi, bj = (b / T.nbi) % T.nbj, bk = b / (T.nbi * T.nbj);
+  const uint32_t gg = T.g * T.g, t = q / gg, u = q % gg, gj = T.bj / T.g;
+  i0 = (int)(bi * 64);
+  j = (int)(bj * T.bj + (t % gj) * T.g + u % T.g);
+  k = (int)(bk * T.bk + (t / gj) * T.g + u / T.g);
+  return j < P.n2 && k < P.n3;
+}
+__global__ void __launch_bounds__(TRAJ_B) k_traj(BParams P, TrajOrder T, unsigned *row_next, uint32_t *scratch,
+                                                       int maxlen, uint32_t *__restrict__ pool,
+                                                       unsigned long long pool_cap, unsigned long long *pool_top,
+                                                       uint64_t *__restrict__ pos, uint32_t *__restrict__ len,
+                                                       int *err) {
   __shared__ uint32_t ring[RING * TRAJ_B];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nth = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
   const RingPath path{ring + threadIdx.x, scratch + (size_t)tid * maxlen, TRAJ_B};
-  for (uint32_t s = tid; s < N; s += nth) {
+  for (;;) {
+    unsigned r = 0;
+    if (lane == 0) r = atomicAdd(row_next, 1u);
+    r = __shfl(r, 0, 64);
+    if (r >= T.nrows) break;  // every lane of the wavefront leaves together
+    int i0, j, k;
+    if (!traj_row(P, T, r, i0, j, k)) continue;
+    const int i = i0 + lane;
+    if (i >= P.n1) continue;
+    const uint32_t s = lin3(P, i, j, k);
     int L = walk_path(P, s, path, maxlen, Never{}, NoVisit{});
     if (L < 0) {
       atomicOr(err, 1);
@@ -270,8 +307,19 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     uint32_t *pool = buf<uint32_t>(ctx, "bader_arena", cap);
     cap = ctx->bufs["bader_arena"].bytes / 4;
     C2_HIP(hipMemsetAsync(d_top, 0, 8, st));
-    C2_LAUNCH(ctx, k_traj, dim3(nth / TRAJ_B), dim3(TRAJ_B), 0, P, N, scratch, MAX_PATH, pool, (unsigned long long)cap,
-              d_top, d_pos, d_len, d_err);
+    TrajOrder T;
+    T.bj = T.bk = 64;
+    T.g = 4;
+    T.nbi = (uint32_t)((n[0] + 63) / 64);
+    T.nbj = (uint32_t)((n[1] + T.bj - 1) / T.bj);
+    T.nbk = (uint32_t)((n[2] + T.bk - 1) / T.bk);
+    const uint64_t rows = (uint64_t)T.nbi * T.nbj * T.nbk * T.bj * T.bk;
+    if (rows >= (1ull << 32)) fail(C2GPU_ERR_ARG, "c2gpu_bader: grid too large for the walk order");
+    T.nrows = (uint32_t)rows;
+    unsigned *d_row = buf<unsigned>(ctx, "bader_rownext", 1);
+    C2_HIP(hipMemsetAsync(d_row, 0, sizeof(unsigned), st));
+    C2_LAUNCH(ctx, k_traj, dim3(nth / TRAJ_B), dim3(TRAJ_B), 0, P, T, d_row, scratch, MAX_PATH, pool,
+              (unsigned long long)cap, d_top, d_pos, d_len, d_err);
     ntraj = read_scalar(ctx, d_top);
     if (ntraj <= cap) break;
     cap = ntraj;  // exact size now known: walk again into a pool that fits
