@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(TRAJ_B) k_traj(BParams P, TrajOrder T, unsigne
                                                        int maxlen, uint32_t *__restrict__ pool,
                                                        unsigned long long pool_cap, unsigned long long *pool_top,
                                                        uint64_t *__restrict__ pos, uint32_t *__restrict__ len,
-                                                       int *err) {
+                                                       uint32_t *__restrict__ tpk, int *err) {
   __shared__ uint32_t ring[RING * TRAJ_B];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -108,6 +108,8 @@ __global__ void __launch_bounds__(TRAJ_B) k_traj(BParams P, TrajOrder T, unsigne
     pos[s] = at;
     if (at + (unsigned long long)L <= pool_cap)
       for (int q = 0; q < L; q++) pool[at + q] = path.get(q, L);
+    if (tpk)
+      for (int q = 1; q <= TPK && q < L; q++) tpk[(size_t)(q - 1) * P.npts + s] = path.get(q, L);
   }
 }
 
@@ -216,6 +218,8 @@ static BParams make_params(const double *d_rho, const int n[3], const double car
   for (int q = 0; q < 9; q++) P.c[q] = car2lat[q];
   for (int q = 0; q < 27; q++) P.lid[q] = lat_i_dist[q];
   P.G = nullptr;
+  P.tpk = nullptr;
+  P.npts = (uint32_t)((uint64_t)n[0] * n[1] * n[2]);
   return P;
 }
 
@@ -302,6 +306,19 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     const uint64_t want = afford < most ? afford : most;
     if (want > cap) cap = want;
   }
+  // the first TPK positions of every trajectory, position-major, when the
+  // device has room for them beside the largest pool and CSR copy planned
+  // and the sweep's state (4 * TPK bytes a point; large cells go without)
+  uint32_t *d_tpk = nullptr;
+  {
+    auto tk = ctx->bufs.find("bader_tpk");
+    const uint64_t need = 4ull * TPK * N, own = (tk != ctx->bufs.end() && tk->second.ptr) ? tk->second.bytes : 0;
+    const uint64_t arena = poolbuf.ptr ? poolbuf.bytes : 0;
+    size_t fr = 0, tot = 0;
+    C2_HIP(hipMemGetInfo(&fr, &tot));
+    const uint64_t rest = 8ull * cap + (8ull << 30) + 112ull * N;  // pool + CSR, spare, sweep state
+    if (own >= need || fr + own + arena > need + rest) d_tpk = buf<uint32_t>(ctx, "bader_tpk", (size_t)TPK * N);
+  }
   uint64_t ntraj = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     uint32_t *pool = buf<uint32_t>(ctx, "bader_arena", cap);
@@ -319,12 +336,13 @@ void bader_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double c
     unsigned *d_row = buf<unsigned>(ctx, "bader_rownext", 1);
     C2_HIP(hipMemsetAsync(d_row, 0, sizeof(unsigned), st));
     C2_LAUNCH(ctx, k_traj, dim3(nth / TRAJ_B), dim3(TRAJ_B), 0, P, T, d_row, scratch, MAX_PATH, pool,
-              (unsigned long long)cap, d_top, d_pos, d_len, d_err);
+              (unsigned long long)cap, d_top, d_pos, d_len, d_tpk, d_err);
     ntraj = read_scalar(ctx, d_top);
     if (ntraj <= cap) break;
     cap = ntraj;  // exact size now known: walk again into a pool that fits
   }
   if (read_scalar(ctx, d_err) & 1) fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: ascent path longer than 2048 points");
+  P.tpk = d_tpk;
   exclusive_scan(ctx, "len", d_len, d_toff, (int64_t)N + 1);
   uint32_t *d_tp = buf<uint32_t>(ctx, "bader_tp", ntraj);
   const unsigned gR = (unsigned)std::min<int64_t>(((int64_t)N * REORDER_LANES + B - 1) / B, int64_t(1) << 20);

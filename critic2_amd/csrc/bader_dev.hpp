@@ -18,7 +18,13 @@ struct BParams {
   // when set: each point's normalised step direction (x, y, z; w = 1 where
   // max|g| < 1e-30), so a step reads one record instead of seven values
   const double4 *__restrict__ G;
+  // when set: positions 1..TPK of every free trajectory, position-major
+  // (tpk[(q-1)*npts + s]), so the lanes of a wavefront (neighbouring starts)
+  // read a path position from one contiguous run; valid for q < its length
+  const uint32_t *__restrict__ tpk;
+  uint32_t npts;
 };
+constexpr int TPK = 8;
 
 // x / d and x % d for x < 2^32 from the FP64 reciprocal: the product is
 // within 2^-20 of x/d, so truncation is off by at most one, which the

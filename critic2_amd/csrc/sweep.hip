@@ -200,6 +200,36 @@ __global__ void k_story_init(uint32_t N, const uint64_t *__restrict__ toff, cons
   lam[s] = maxlab[tp[toff[s + 1] - 1]];
 }
 
+// Path positions examined per batch of independent loads.
+constexpr int PRE_CHUNK = 4;
+
+// Length of a committed path: the trajectory tp[o .. o+len) up to and
+// including its first point (from position 1) whose known == 2 time is
+// before ts, or all of it.  PRE_CHUNK positions are loaded at a time, so the
+// loads of a batch are in flight together instead of one after another.
+// Position q (>= 1) of start s's trajectory: from the position-major
+// prefix table when it holds it, else from the CSR copy.
+__device__ __forceinline__ uint32_t traj_at(const BParams &P, uint32_t s, const uint32_t *__restrict__ tp, uint64_t o,
+                                            int q) {
+  return (P.tpk && q <= TPK) ? P.tpk[(size_t)(q - 1) * P.npts + s] : tp[o + q];
+}
+
+__device__ __forceinline__ int stop_position(const BParams &P, uint32_t s, const uint32_t *__restrict__ tp,
+                                             uint64_t o, int len, const uint64_t *__restrict__ Kt, uint64_t ts) {
+  for (int q0 = 1; q0 < len; q0 += PRE_CHUNK) {
+    uint32_t x[PRE_CHUNK];
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
+    uint64_t kt[PRE_CHUNK];
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : KT_INF;
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++)
+      if (kt[u] < ts) return q0 + u + 1;
+  }
+  return len;
+}
+
 // A closer first story from a few cheap rounds without event lists.  Every
 // point's label is taken as its free trajectory's attractor (flab), so a
 // point enters known == 2 once it and its six face neighbours are covered
@@ -244,7 +274,7 @@ __global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ C, 
 
 __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
                             const uint64_t *__restrict__ Cold, const uint64_t *__restrict__ Kt,
-                            unsigned long long *__restrict__ Cnew) {
+                            unsigned long long *__restrict__ Cnew, int cap) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const uint64_t ts = sweep_index(P, s) << POS_BITS;
@@ -253,12 +283,26 @@ __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ 
   // update; a covering time only decreases, so a value already at or below
   // this path's needs no atomic either (most points lie on many paths)
   const uint64_t o = toff[s];
-  const int len = (int)(toff[s + 1] - o);
-  for (int q = 1; q < len; q++) {
-    const uint32_t x = tp[o + q];
-    const unsigned long long c = ts | (uint64_t)q;
-    if (c < __hip_atomic_load(&Cnew[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&Cnew[x], c);
-    if (Kt[x] < ts) break;
+  int len = (int)(toff[s + 1] - o);
+  if (len > cap) len = cap;
+  // PRE_CHUNK path points at a time: their indices and Kt are loaded
+  // together (independent loads), then the stop is found among them
+  for (int q0 = 1; q0 < len; q0 += PRE_CHUNK) {
+    uint32_t x[PRE_CHUNK];
+    uint64_t kt[PRE_CHUNK];
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : 0ull;
+    bool stop = false;
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) {
+      if (stop || q0 + u >= len) break;
+      const unsigned long long c = ts | (uint64_t)(q0 + u);
+      if (c < __hip_atomic_load(&Cnew[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&Cnew[x[u]], c);
+      stop = kt[u] < ts;
+    }
+    if (stop) break;
   }
 }
 
@@ -275,14 +319,9 @@ __global__ void k_pre_story(BParams P, uint32_t N, const uint64_t *__restrict__ 
   }
   const uint64_t o = toff[s];
   const int len = (int)(toff[s + 1] - o);
-  int e = len;
-  for (int q = 1; q < len; q++)
-    if (Kt[tp[o + q]] < ts) {
-      e = q + 1;
-      break;
-    }
+  const int e = stop_position(P, s, tp, o, len, Kt, ts);
   ell[s] = e;
-  lam[s] = flab[tp[o + e - 1]];
+  lam[s] = flab[e > 1 ? traj_at(P, s, tp, o, e - 1) : s];
 }
 
 __global__ void k_ev_count(uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
@@ -554,14 +593,9 @@ __device__ void decide_start(const BParams &P, uint32_t s, const uint64_t *__res
   if (v_at(L, s, ts, false) != 0) return;  // volnum /= 0: not a start
   const uint64_t o = toff[s];
   const int len = (int)(toff[s + 1] - o);
-  e = len;
-  for (int q = 1; q < len; q++)
-    if (Kt[tp[o + q]] < ts) {  // known == 2 when the path reaches it
-      e = q + 1;
-      break;
-    }
+  e = stop_position(P, s, tp, o, len, Kt, ts);  // stops at a point known == 2 when the path reaches it
   // (e positions of the path were read: tp and Kt)
-  const uint32_t end = tp[o + e - 1];
+  const uint32_t end = e > 1 ? traj_at(P, s, tp, o, e - 1) : s;
   la = v_at(L, end, ts, false);
   if (la == 0) {
     if (e != len) atomicOr(err, 2);  // an unlabeled known==2 point cannot exist
@@ -1055,6 +1089,11 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   const int nmax = std::max(P.n1, std::max(P.n2, P.n3));
   int pre_rounds = std::min(32, std::max(8, nmax / 16));
   if (const char *pr = std::getenv("C2GPU_SWEEP_PRE")) pre_rounds = std::atoi(pr);
+  // the rounds follow each start's path for at most 32 points: longer ones
+  // (rare, and mostly in the early rounds when the known == 2 times are
+  // late) cost more than their coverage helps (512^3: 925 ms against 950 ms
+  // uncapped; 8 points: 1289 ms)
+  const int pre_cap = 32;
   if (pre_rounds > 0) {
     // buffers the first full build overwrites: flab in cur, C in offN / aoff
     int32_t *flab = reinterpret_cast<int32_t *>(cur);
@@ -1066,7 +1105,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     const bool count = trace;  // covering times changed per round (diagnostics)
     for (int r = 1; r < pre_rounds; r++) {
       if (count) C2_HIP(hipMemsetAsync(d_pc, 0, sizeof(unsigned long long), st));
-      C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, (unsigned long long *)Cb);
+      C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, (unsigned long long *)Cb, pre_cap);
       // Cb now holds this round's covering times; Ca is reset for the next
       C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, flab, Kt, Ca, (const uint64_t *)Ca,
                 count ? d_pc : (unsigned long long *)nullptr);
