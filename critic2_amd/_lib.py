@@ -73,7 +73,7 @@ class CubeHeader(C.Structure):
 
 class YtStats(C.Structure):
     _fields_ = [("nattr", C.c_int), ("nmaxima", C.c_int), ("nias", C.c_int64), ("label_passes", C.c_int),
-                ("weight_passes", C.c_int), ("wide", C.c_int64), ("ms_total", C.c_double)]
+                ("weight_passes", C.c_int), ("wide", C.c_int64), ("ms_total", C.c_double), ("ranked", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

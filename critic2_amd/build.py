@@ -28,6 +28,7 @@ SOURCES = [
     ("yt.hip", ["-ffp-contract=off"]),
     ("mpole.hip", ["-ffp-contract=off"]),
     ("wscell.cpp", ["-ffp-contract=off"]),
+    ("qcksort.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
 ]

@@ -8,8 +8,9 @@
 // joins it; the others lie on an interatomic surface (basin 0) and pass on
 // weight to the basins of their higher neighbours, fractions chi_k / sum chi
 // with chi_k = area_k * (rho_j - rho_i).  None of this needs the ranking
-// itself, only the comparisons between neighbours (equal neighbours are
-// refused: their order would be the reference sort's):
+// itself, only the comparisons between neighbours; where equal values meet,
+// the comparisons take the ranks of the reference's quicksort instead
+// (qcksort.cpp, on the host):
 //   k_yt_class    higher-neighbour mask per point, maxima listed;
 //   host          maxima in decreasing rho, attractor rules (as for Bader);
 //   k_yt_lab_step per point the least and greatest basin reachable upwards,
@@ -44,6 +45,7 @@ namespace c2 {
 int attractor_rule(const c2gpu_attractor_opts *o, uint32_t nmax, const uint32_t *order, const double *dv,
                    int32_t *lab, std::vector<double> &xattr, std::vector<int8_t> &discarded);
 int ws_facets(const double m[9], int vec[3 * 14], double area[14]);
+void qcksort_order(const double *arr, uint32_t *iord, int64_t first, int64_t last);
 void basin_sums_run(c2gpu_ctx *, const int32_t *, int64_t, int, int, const double *const *, int, double *,
                     int64_t *);
 
@@ -58,7 +60,20 @@ struct YtGeo {
   int n1, n2, n3, nvec;
   int di[14], dj[14], dk[14];
   double al[14];
+  // the reference sort's rank of every point (qcksort_order), set when the
+  // grid has equal values where the order matters; else null and values
+  // decide (they are distinct wherever compared)
+  const uint32_t *rk;
 };
+
+// Whether point a is ranked above point b (yt@proc.f90:117: jj > ii).
+__device__ __forceinline__ bool yt_above(const YtGeo &G, const double *__restrict__ g, uint32_t a, uint32_t b) {
+  return G.rk ? G.rk[a] > G.rk[b] : g[a] > g[b];
+}
+// A key that orders points as their ranks do.
+__device__ __forceinline__ double yt_key(const YtGeo &G, const double *__restrict__ g, uint32_t a) {
+  return G.rk ? (double)G.rk[a] : g[a];
+}
 
 __device__ __forceinline__ int wrapn(int x, int n) {
   x %= n;
@@ -98,11 +113,21 @@ __global__ void __launch_bounds__(YB) k_yt_class(YtGeo G, uint32_t N, const doub
     coords3(G, p, i, j, k);
     const double gp = g[p];
     bool tie = false;
+    double hv[14];
+    int nh = 0;
     for (int q = 0; q < G.nvec; q++) {
-      const double gj = g[nbr(G, i, j, k, q)];
-      if (gj > gp) mask |= 1u << q;
-      else if (gj == gp) tie = true;
+      const uint32_t jn = nbr(G, i, j, k, q);
+      if (yt_above(G, g, jn, p)) {
+        mask |= 1u << q;
+        hv[nh++] = g[jn];
+      }
+      if (!G.rk && g[jn] == gp) tie = true;
     }
+    // without ranks, equal values among the higher neighbours would also be
+    // ordered by the reference sort (the order their weights add in)
+    if (!G.rk)
+      for (int a = 1; a < nh; a++)
+        for (int b = 0; b < a; b++) tie = tie || hv[a] == hv[b];
     hmask[p] = (uint16_t)mask;
     pend[p] = (uint32_t)__popc(mask);
     if (tie) atomicOr(err, 1);
@@ -163,10 +188,9 @@ __global__ void __launch_bounds__(YB) k_yt_lab_step(YtGeo G, const uint32_t *nfp
       const uint32_t p = front[t];
       int i, j, k;
       coords3(G, p, i, j, k);
-      const double gp = g[p];
       for (int q = 0; q < G.nvec; q++) {
         const uint32_t jn = nbr(G, i, j, k, q);
-        if (g[jn] < gp && atomicSub(&pend[jn], 1u) == 1u) {
+        if (yt_above(G, g, p, jn) && atomicSub(&pend[jn], 1u) == 1u) {
           lohi[jn] = yt_lohi_of(G, jn, hmask, lohi);
           done[nd++] = jn;
         }
@@ -222,7 +246,7 @@ __device__ __forceinline__ int yt_fracs(const YtGeo &G, uint32_t p, const double
     const double gj = g[jn];
     const double c = fmax(G.al[q] * (gj - gp), VSMALL);
     q0[nh] = jn;
-    hg[nh] = gj;
+    hg[nh] = yt_key(G, g, jn);
     chi[nh] = c;
     csum = csum + c;
     nh++;
@@ -273,10 +297,11 @@ __device__ __forceinline__ void yt_narrow_point(const YtGeo &G, uint32_t p, int 
       c++;
     }
   uint32_t jn = 0;
-  double gj = 0.0, chi = 0.0;
+  double gj = 0.0, chi = 0.0, kj = 0.0;
   if (q >= 0) {
     jn = nbr(G, i, j, k, q);
     gj = g[jn];
+    kj = yt_key(G, g, jn);
     chi = fmax(G.al[q] * (gj - gp), VSMALL);
   }
   double csum = 0.0;
@@ -284,8 +309,8 @@ __device__ __forceinline__ void yt_narrow_point(const YtGeo &G, uint32_t p, int 
   const double fr = chi / fmax(csum, VSMALL);
   int rank = 0;
   for (int u = 0; u < nh; u++) {
-    const double gu = __shfl(gj, u, GL);
-    rank += (gu > gj || (gu == gj && u < r)) ? 1 : 0;
+    const double ku = __shfl(kj, u, GL);
+    rank += (ku > kj || (ku == kj && u < r)) ? 1 : 0;
   }
   // the lane of each rank, four bits each
   uint64_t src = 0;
@@ -487,10 +512,9 @@ __global__ void __launch_bounds__(YB) k_yt_w_wide_step(YtGeo G, const uint32_t *
       const uint32_t p = front[t];
       int i, j, k;
       coords3(G, p, i, j, k);
-      const double gp = g[p];
       for (int q = 0; q < G.nvec; q++) {
         const uint32_t jn = nbr(G, i, j, k, q);
-        if (g[jn] < gp && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
+        if (yt_above(G, g, p, jn) && lab[jn] == 0 && atomicSub(&pend[jn], 1u) == 1u) done[nd++] = jn;
       }
     }
     for (int q = 0; q < 14; q++) yt_append(q < nd, q < nd ? done[q] : 0u, next, nnext);
@@ -673,11 +697,35 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   uint32_t *pend = buf<uint32_t>(ctx, "yt_pend", N);
   C2_LAUNCH(ctx, k_yt_class, dim3(gN), dim3(YB), 0, G, N, g, hmask, pend, maxl, d_cnt, d_err);
   add_kernel_bytes(ctx, "k_yt_class", (double)N * (8.0 * (G.nvec + 1) + 2.0));
-  const uint32_t nmax = read_scalar(ctx, d_cnt);
-  if (read_scalar(ctx, d_err) & 1)
-    fail(C2GPU_ERR_DEGENERATE, "c2gpu_yt: neighbouring grid points with equal values (their order is the "
-                               "reference sort's)");
-  // maxima in decreasing rho; the attractor rules
+  uint32_t nmax = read_scalar(ctx, d_cnt);
+  // Equal values where the order matters (neighbours, the higher neighbours
+  // of one point, or maxima): every point is ranked as the reference's
+  // quicksort ranks it (yt@proc.f90:85-93), on the host, and the ranks
+  // decide every order from here on.
+  std::vector<uint32_t> h_rk;
+  auto rank_points = [&]() {
+    std::vector<double> h_g(N);
+    copy_to_host(ctx, h_g.data(), g, 8 * (size_t)N);
+    C2_HIP(hipStreamSynchronize(st));
+    std::vector<uint32_t> io(N);
+    for (uint32_t q = 0; q < N; q++) io[q] = q;
+    qcksort_order(h_g.data(), io.data(), 0, (int64_t)N - 1);
+    h_rk.assign(N, 0);
+    for (uint32_t q = 0; q < N; q++) h_rk[io[q]] = q;
+    uint32_t *d_rk = buf<uint32_t>(ctx, "yt_rank", N);
+    C2_HIP(hipMemcpyAsync(d_rk, h_rk.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st));
+    C2_HIP(hipStreamSynchronize(st));
+    G.rk = d_rk;
+    S.ranked = 1;
+  };
+  if (read_scalar(ctx, d_err) & 1) {
+    rank_points();
+    C2_HIP(hipMemsetAsync(d_cnt, 0, 8, st));
+    C2_HIP(hipMemsetAsync(d_err, 0, 8, st));
+    C2_LAUNCH(ctx, k_yt_class, dim3(gN), dim3(YB), 0, G, N, g, hmask, pend, maxl, d_cnt, d_err);
+    nmax = read_scalar(ctx, d_cnt);
+  }
+  // maxima in decreasing rho (rank); the attractor rules
   std::vector<uint32_t> h_max(nmax);
   std::vector<double> h_gmax(nmax);
   if (nmax) {
@@ -689,12 +737,17 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   }
   std::vector<uint32_t> order(nmax);
   for (uint32_t q = 0; q < nmax; q++) order[q] = q;
-  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+  auto by_value = [&](uint32_t a, uint32_t b) {
     return h_gmax[a] != h_gmax[b] ? h_gmax[a] > h_gmax[b] : h_max[a] < h_max[b];
-  });
-  for (uint32_t q = 1; q < nmax; q++)
-    if (h_gmax[order[q]] == h_gmax[order[q - 1]])
-      fail(C2GPU_ERR_DEGENERATE, "c2gpu_yt: maxima with equal values (their numbering is the reference sort's)");
+  };
+  if (h_rk.empty()) {
+    std::sort(order.begin(), order.end(), by_value);
+    bool tied = false;
+    for (uint32_t q = 1; q < nmax; q++) tied = tied || h_gmax[order[q]] == h_gmax[order[q - 1]];
+    if (tied) rank_points();  // maxima of equal value: their order is the reference sort's
+  }
+  if (!h_rk.empty())
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return h_rk[h_max[a]] > h_rk[h_max[b]]; });
   std::vector<double> dv(3 * (size_t)nmax);
   for (uint32_t q = 0; q < nmax; q++) {
     const uint32_t l = h_max[q];

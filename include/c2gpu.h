@@ -431,6 +431,8 @@ typedef struct {
   int weight_passes;  /* propagation passes for the surface weights        */
   int64_t wide;       /* surface points where more than 32 basins meet     */
   double ms_total;
+  int ranked;         /* 1: equal values met, points ranked by the
+                         reference's quicksort on the host                 */
 } c2gpu_yt_stats;
 
 /* yt_integrate (yt@proc.f90:38-224) with the weights of yt_weights
@@ -444,9 +446,12 @@ typedef struct {
  * label_out: basin per point in grid order (x fastest; critic2 keeps it in
  * rank order, yt@proc.f90:218).  vol_out[a] = sum of basin a+1's weights,
  * sums_out[q*nattr + a] = sum of weights times fields[q] (both unscaled:
- * critic2 multiplies by Omega/N).  Equal values at neighbouring points give
- * C2GPU_ERR_DEGENERATE (their order is the reference sort's).  rho,
- * fields and label_out are device memory with C2GPU_PTR_DEVICE. */
+ * critic2 multiplies by Omega/N).  Where equal values meet (neighbouring
+ * points, the higher neighbours of one point, or maxima), their order is the
+ * one critic2's qcksort gives them (tools@proc.f90:83-168): the points are
+ * then ranked by a restatement of that quicksort on the host (seconds for
+ * 16 M points) and the ranks decide every comparison.  rho, fields and
+ * label_out are device memory with C2GPU_PTR_DEVICE. */
 int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9],
              const c2gpu_attractor_opts *opts, int nfields, const double *const *fields, int flags,
              int32_t *label_out, int *nattr_out, double *xattr_out, int xattr_cap, double *vol_out,

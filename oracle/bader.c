@@ -36,6 +36,11 @@
  * maximum gives one basin, counts sum to N and populations to the grid
  * sum, the attractor rules follow hand-worked cases.  The reference's own
  * Bader golden outputs need input grids its checkout does not hold.
+ *
+ * qcksort_r8 (the YT ranking): pinned by two hand-traced arrays (insertion
+ * sort; one partition) and sortedness of large tied arrays
+ * (tests/test_yt_cpu.py); its pivot sequence past the first partition is
+ * parity unpinned (the reference's tools module does not build here).
  */
 #include "c2oracle.h"
 #include <math.h>
@@ -714,25 +719,101 @@ void oracle_step_pointers(const double *f, const int n[3], const double car2lat[
 /* (integration@proc.f90:1208-1218, :1289-1300) for imtype_yt.          */
 /* ==================================================================== */
 
-static const double *yt_sort_g;
-static int yt_cmp(const void *a, const void *b) {
-  const int64_t ia = *(const int64_t *)a, ib = *(const int64_t *)b;
-  const double fa = yt_sort_g[ia], fb = yt_sort_g[ib];
-  if (fa < fb) return -1;
-  if (fa > fb) return 1;
-  return ia < ib ? -1 : (ia > ib ? 1 : 0);
-}
-
 /* f: the grid (x fastest).  vec (3 per vector, grid steps) and area: the
  * Voronoi-relevant vectors of the grid lattice and their facet areas
- * (grid3%vec, grid3%area).  Returns 0, or 1 if two neighbouring points hold
- * equal values (the order qcksort gives such a pair decides the result; the
- * restatement sorts ties by index and refuses them instead), 2 if the
+ * (grid3%vec, grid3%area).  Points are ranked by oracle_qcksort_r8, the
+ * reference's sort, so equal values take its order.  Returns 0, or 2 if the
  * attractor list overflows xcap.  idg: the basin of each point in grid
  * order (0 on the interatomic surface; critic2 stores ibasin in sorted
  * order, yt@proc.f90:218).  vol[a] = sum of the weights of basin a+1,
  * sums[q*nattr + a] = sum of w*fields[q]; both unscaled (critic2
  * multiplies by Omega/N). */
+/* qcksort_r8 (tools@proc.f90:83-168), the reference's quicksort: iord(first:last)
+ * (1-based positions in the reference; 0-based here, as are the element
+ * indices in iord) sorted by arr ascending.  The pivot position comes from a
+ * linear congruential sequence (fx) advanced once per partition; the
+ * partition moves elements through a hole, right scan then left scan;
+ * partitions below m = 7 elements are insertion-sorted.  The arrangement of
+ * equal keys is this sequence's, which is what yt_integrate's ranks depend
+ * on. */
+void oracle_qcksort_r8(const double *arr, int64_t *iord, int64_t first, int64_t last) {
+  const int64_t m = 7;
+  const int nstack = 50;
+  const double fm = 7875.0, fa = 211.0, fc = 1663.0, fmi = 1.2698413e-4;
+  int64_t istack[50];
+  int jstack = 0;
+  int64_t l = first, ir = last, i, j, na, iq;
+  double fx = 0.0, a;
+L10:
+  if (ir - l < m) {
+    for (j = l + 1; j <= ir; j++) {
+      na = iord[j];
+      a = arr[na];
+      for (i = j - 1; i >= first; i--) {
+        if (arr[iord[i]] <= a) goto L12;
+        iord[i + 1] = iord[i];
+      }
+      i = first - 1;
+    L12:
+      iord[i + 1] = na;
+    }
+    if (jstack == 0) return;
+    ir = istack[jstack - 1];
+    l = istack[jstack - 2];
+    jstack -= 2;
+  } else {
+    i = l;
+    j = ir;
+    fx = fmod(fx * fa + fc, fm);
+    iq = l + (int64_t)((double)(ir - l + 1) * (fx * fmi));
+    na = iord[iq];
+    a = arr[na];
+    iord[iq] = iord[l];
+  L20:
+  L21:
+    if (j >= first) {
+      if (a < arr[iord[j]]) {
+        j--;
+        goto L21;
+      }
+    }
+    if (j <= i) {
+      iord[i] = na;
+      goto L30;
+    }
+    iord[i] = iord[j];
+    i++;
+  L22:
+    if (i <= last) {
+      if (a > arr[iord[i]]) {
+        i++;
+        goto L22;
+      }
+    }
+    if (j <= i) {
+      iord[j] = na;
+      i = j;
+      goto L30;
+    }
+    iord[j] = iord[i];
+    j--;
+    goto L20;
+  L30:
+    jstack += 2;
+    if (jstack > nstack) return; /* the reference stops here ("Increase nstack") */
+    if (ir - i >= i - l) {
+      istack[jstack - 1] = ir;
+      istack[jstack - 2] = i + 1;
+      ir = i - 1;
+    } else {
+      istack[jstack - 1] = i - 1;
+      istack[jstack - 2] = l;
+      l = i + 1;
+    }
+  }
+  goto L10;
+}
+
 int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, const int *vec, const double *area,
               int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
               double *vol, double *sums, double *wout, int wcap) {
@@ -744,17 +825,12 @@ int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, cons
   double *fnear = calloc(nn * (size_t)nvec, sizeof(double));
   int rc = 0;
   for (int64_t i = 0; i < nn; i++) io[i] = i;
-  yt_sort_g = f;
-  qsort(io, nn, sizeof(int64_t), yt_cmp); /* from smaller to larger (qcksort) */
+  oracle_qcksort_r8(f, io, 0, nn - 1); /* from smaller to larger (yt@proc.f90:85-93) */
   for (int64_t i = 0; i < nn; i++) iio[io[i]] = i;
 #define YT_NB(i, k)                                                                           \
   (wrap((int)((i) % n[0]) + vec[3 * (k)], n[0]) +                                             \
    n[0] * (wrap((int)(((i) / n[0]) % n[1]) + vec[3 * (k) + 1], n[1]) +                        \
            (int64_t)n[1] * wrap((int)((i) / ((int64_t)n[0] * n[1])) + vec[3 * (k) + 2], n[2])))
-  /* equal neighbours: the order between them would be qcksort's */
-  for (int64_t i = 0; i < nn && !rc; i++)
-    for (int k = 0; k < nvec; k++)
-      if (f[YT_NB(i, k)] == f[i]) rc = 1;
   int nattr = g->mode != ORC_ATTR_NOATOMS ? g->nat : 0;
   if (nattr > xcap) rc = 2;
   for (int a = 0; a < nattr && !rc; a++)

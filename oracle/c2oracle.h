@@ -180,8 +180,14 @@ void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
 /* Yu-Trinkle integration (yt@proc.f90:38-224, :399-531; the imtype_yt sums
  * of intgrid_fields).  vec: 3*nvec grid steps, area: nvec facet areas of the
  * grid lattice's Wigner-Seitz cell.  idg in grid order; vol and sums
- * unscaled; sums[q*nattr + a].  Returns 0, 1 (equal neighbouring values:
- * refused), 2 (more than xcap attractors). */
+ * unscaled; sums[q*nattr + a].  Points are ranked by oracle_qcksort_r8, so
+ * equal values take the reference sort's order.  Returns 0, or 2 (more than
+ * xcap attractors). */
+/* qcksort_r8 (tools@proc.f90:83-168): iord[first..last] (0-based positions,
+ * element indices) sorted by arr ascending, equal keys in that quicksort's
+ * arrangement. */
+void oracle_qcksort_r8(const double *arr, int64_t *iord, int64_t first, int64_t last);
+
 int oracle_yt(const double *f, const int n[3], const orc_geom *g, int nvec, const int *vec, const double *area,
               int nfields, const double *const *fields, int32_t *idg, int *nattr_out, double *xattr, int xcap,
               double *vol, double *sums, double *wout, int wcap);

@@ -508,8 +508,9 @@ def nci_cutoffs(crho, cgrad, rhocut=0.2, dimcut=2.0, rhoplot=0.1, srhorange=(-1e
 def yt(f, n, x2c, xat, vec, area, fields=(), mode=0, ratom=1e40, weights=False, cps=None):
     """yt_integrate + yt_weights + the imtype_yt sums (oracle_yt).  Returns
     (labels in grid order, nattr, xattr, vol, sums[q, a]), and with
-    weights=True also w[a, p], every basin's yt_weights in grid order;
-    raises on equal neighbouring values."""
+    weights=True also w[a, p], every basin's yt_weights in grid order.
+    Equal values are ranked in the reference quicksort's order
+    (qcksort_order)."""
     lib = load()
     lib.oracle_yt.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcGeom), C.c_int, C.c_void_p, C.c_void_p,
                               C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
@@ -675,3 +676,16 @@ def relabel(idg, assigned):
         if a != i:
             out[idg == i] = a
     return out
+
+
+def qcksort_order(arr):
+    """qcksort_r8 over the whole array (tools@proc.f90:83-168): the
+    permutation (0-based element indices) that sorts arr ascending, equal
+    keys in that quicksort's arrangement."""
+    lib = load()
+    lib.oracle_qcksort_r8.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64]
+    arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+    io = np.arange(arr.size, dtype=np.int64)
+    if arr.size:
+        lib.oracle_qcksort_r8(arr.ctypes.data, io.ctypes.data, 0, arr.size - 1)
+    return io

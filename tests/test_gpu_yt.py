@@ -66,13 +66,53 @@ def test_yt_many_basins_meet_wide_path(ctx):
     assert got["stats"]["wide"] > 0
 
 
-def test_yt_refuses_equal_neighbours(ctx):
-    cell = random_cell(65, (8.0, 8.0, 8.0), (90, 90, 90), 2, (1, 6), (16, 16, 16))
+@pytest.mark.parametrize("digits,seed,n,ranked", [(3, 65, (24, 26, 28), 1), (6, 67, (32, 34, 36), None),
+                                                   (2, 68, (20, 22, 24), 1)])
+def test_yt_equal_values_follow_the_reference_sort(ctx, digits, seed, n, ranked):
+    """Grids rounded to a few significant digits, as cube files print them
+    (ties between neighbours, between the higher neighbours of one point and
+    between maxima): the points are ranked by the reference's quicksort and
+    the labels, attractors and weight sums equal the oracle's, which ranks
+    them with its own restatement of that sort."""
+    cell = random_cell(seed, (8.0, 8.5, 9.0), (90, 95, 100), 4, (1, 6, 7, 8), n)
+    print(f"[case] yt ties digits={digits} n={cell.n}", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    f = np.array([float(f"{v:.{digits}g}") for v in f])
+    ref, got = _run_both(ctx, cell, f, {})
+    idg, nattr, xattr, vol, sums = ref
+    print(f"  ranked {got['stats']['ranked']} nattr {nattr}", flush=True)
+    if ranked is not None:  # 6 digits: ties only where they may not matter
+        assert got["stats"]["ranked"] == ranked
+    assert got["nattr"] == nattr
+    np.testing.assert_array_equal(got["xattr"], xattr)
+    assert int(np.count_nonzero(got["labels"] != idg)) == 0
+    np.testing.assert_allclose(got["vol"], vol, rtol=1e-10)
+    np.testing.assert_allclose(got["sums"], sums, rtol=1e-10)
+
+
+def test_yt_plateau_and_tied_maxima(ctx):
+    """A flat plateau with two equal maxima: every order decision is a tie,
+    decided by the quicksort's ranks on both sides."""
+    cell = random_cell(69, (8.0, 8.0, 8.0), (90, 90, 90), 2, (1, 6), (16, 16, 16))
     f = np.ones(16 ** 3)
     f[100] = 2.0
-    with pytest.raises(_lib.C2GpuError) as e:
-        gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat)
-    assert e.value.code == 8
+    f[2000] = 2.0
+    f[3000] = 1.5
+    ref, got = _run_both(ctx, cell, f, {"noatoms": True, "ratom": 0.5})
+    idg, nattr, xattr, vol, sums = ref
+    assert got["stats"]["ranked"] == 1
+    assert got["nattr"] == nattr
+    np.testing.assert_array_equal(got["xattr"], xattr)
+    assert int(np.count_nonzero(got["labels"] != idg)) == 0
+    np.testing.assert_allclose(got["vol"], vol, rtol=1e-10)
+    np.testing.assert_allclose(got["sums"], sums, rtol=1e-10)
+
+
+def test_yt_distinct_values_need_no_ranking(ctx):
+    cell = random_cell(61, (8.0, 8.5, 9.0), (90, 90, 90), 4, (1, 6, 7, 8), (24, 26, 28))
+    f = gb.synth_slater(ctx, cell)
+    got = gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat)
+    assert got["stats"]["ranked"] == 0
 
 
 YT_EDGE = [  # name, seed, cell edges (A), angles, atoms, grid

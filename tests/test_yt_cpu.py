@@ -108,3 +108,60 @@ def test_oracle_yt_weights_and_multipoles():
     lab = np.where(surf, 0, idg).astype(np.int32)
     np.testing.assert_allclose(oracle.yt_multipoles(ind, f, cell.n, cell.x2c, xa, lmax),
                                oracle.multipoles(lab, f, cell.n, cell.x2c, xa, lmax), rtol=1e-12, atol=1e-12)
+
+
+def test_qcksort_hand_traced():
+    """The oracle's qcksort_r8 (tools@proc.f90:83-168) on arrays traced by
+    hand.  Five elements: insertion sort only (keeps the index order of
+    equal keys).  Eight elements: one partition, fx = 1663, pivot position
+    int(8 * 1663 * 1.2698413e-4) = 1 (value 1), the hole scheme moves index 5
+    to the front and leaves [5, 1 | 2, 3, 4, 0, 6, 7]; the right part is then
+    insertion-sorted to [3, 6, 2, 4, 0, 7]: the tie order of the 1s and the
+    3s is not the index order."""
+    from oracle import oracle
+    assert oracle.qcksort_order([2, 1, 2, 1, 2]).tolist() == [1, 3, 0, 2, 4]
+    assert oracle.qcksort_order([3, 1, 3, 2, 3, 1, 2, 3]).tolist() == [5, 1, 3, 6, 2, 4, 0, 7]
+
+
+@pytest.mark.parametrize("n,levels", [(1000, 7), (20000, 50), (200000, 1000)])
+def test_qcksort_sorts_with_ties(n, levels):
+    """Larger arrays with many ties (several partitions, the pivot sequence
+    advancing): a permutation that sorts ascending."""
+    from oracle import oracle
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, levels, n).astype(np.float64)
+    io = oracle.qcksort_order(a)
+    assert sorted(io.tolist()) == list(range(n))
+    assert np.all(np.diff(a[io]) >= 0)
+    # equal keys are not left in index order (the sort is not stable)
+    assert not all(np.all(np.diff(io[a[io] == v]) > 0) for v in range(levels))
+
+
+def test_oracle_yt_ranks_equal_values_by_qcksort():
+    """A grid quantised to 3 significant digits (ties everywhere): the YT
+    oracle runs (it no longer refuses), its weights still form a partition
+    of unity, and its labels follow the quicksort's ranks: a point is a
+    maximum exactly when no Voronoi neighbour is ranked above it."""
+    from oracle import oracle
+    from tests.helpers import small_cell
+    from critic2_amd.synth import BOHR_PER_ANG
+    cell = small_cell(7, (14, 16, 18), nat=3, ang=(90, 90, 90))
+    f = oracle.slater_grid(cell.x2c, cell.n, cell.xat, cell.zat, 8.0 * BOHR_PER_ANG)
+    f = np.array([float(f"{v:.3g}") for v in f])
+    vec, area = _lib.ws_facets(np.asarray(cell.x2c) / np.array(cell.n)[None, :])
+    idg, nattr, xattr, vol, sums = oracle.yt(f, cell.n, cell.x2c, cell.xat, vec, area, fields=[f])
+    assert vol.sum() == pytest.approx(f.size, rel=1e-12)
+    io = oracle.qcksort_order(f)
+    rk = np.empty(f.size, dtype=np.int64)
+    rk[io] = np.arange(f.size)
+    n = np.array(cell.n)
+    ijk = np.stack(np.unravel_index(np.arange(f.size), cell.n, order="F"), axis=1)
+    above = np.zeros(f.size, dtype=bool)
+    for v in np.asarray(vec).reshape(-1, 3):
+        nb = (ijk + v) % n
+        jn = nb[:, 0] + n[0] * (nb[:, 1] + n[1] * nb[:, 2])
+        above |= rk[jn] > rk
+    maxima = np.nonzero(~above)[0]
+    assert len(maxima) >= 1 and all(idg[m] > 0 for m in maxima)
+    ties = sum(int(np.count_nonzero(f == v)) > 1 for v in np.unique(f))
+    assert ties > 10
