@@ -179,8 +179,14 @@ __global__ void k_reassign(BParams P, uint32_t N, const int32_t *__restrict__ V,
     const uint32_t e = path[L - 1];
     const int32_t pv = V[e];
     if (pv < 0 || pv > nbasin) {
-      // the serial sweep would stop (:400-402), or the end is itself a start
-      // of this sweep (plateau point): flag it
+      // the serial sweep stops here (:400-402).  An end e != s that is
+      // negative cannot occur: a walk ends only at a grid maximum (a point
+      // that is not is_max always has a strictly higher neighbour, so
+      // step_ongrid moves, and a near-grid step back onto the path falls
+      // back to step_ongrid) or at a known == 2 point, and the marking never
+      // makes a maximum negative nor leaves a negative point known == 2.
+      // Such an end would make the result depend on the serial order, so it
+      // is reported as an internal inconsistency rather than followed.
       atomicOr(err, (pv < 0 && e != s) ? 8 : 4);
       Vout[s] = b;
       continue;
@@ -236,7 +242,7 @@ RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin
     if (e & 1)
       fail(C2GPU_ERR_PATHLEN, "c2gpu_bader: refinement path longer than " + std::to_string(maxlen) + " points");
     if (e & 4) fail(C2GPU_ERR_NEWMAX, "refine_edge: should be no new maxima in edge refinement");
-    if (e & 8) fail(C2GPU_ERR_DEGENERATE, "c2gpu_bader: refinement path ends on a point being reassigned");
+    if (e & 8) fail(C2GPU_ERR_DEGENERATE, "c2gpu_bader: internal inconsistency: a refinement path ends on a point being reassigned");
     if (nre == 0) break;
   }
   R.iters = iter;

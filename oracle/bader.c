@@ -82,6 +82,15 @@ int oracle_refine_counts(int64_t *out, int max) {
 }
 int oracle_get_mutation(void) { return orc_mut; }
 
+/* Reassign paths (all passes of the last run) that ended on another point
+ * which was negative when its pass's reassign loop began; tests only. */
+static int64_t orc_negend = 0;
+int64_t oracle_refine_negend(void) {
+  const int64_t v = orc_negend;
+  orc_negend = 0;
+  return v;
+}
+
 static inline int wrap(int p, int n) {
   /* pbc: repeated +n / -n until 1..n (0..n-1 here) */
   while (p < 0) p += n;
@@ -342,6 +351,10 @@ static int64_t refine_edge(bstate *s, int ref_itrs) {
     if (ref_itrs <= ORC_MAXPASS) orc_reascend[ref_itrs - 1] = neg;
     orc_npass = ref_itrs;
   }
+  /* the signs at the start of the reassign loop, for the count of paths
+   * that end on another point that was negative then (tests only) */
+  uint8_t *neg0 = (uint8_t *)malloc((size_t)s->N);
+  for (int64_t q = 0; q < s->N; q++) neg0[q] = s->volnum[q] < 0;
   for (int i = 0; i < n1; i++)
     for (int j = 0; j < n2; j++)
       for (int k = 0; k < n3; k++) {
@@ -350,8 +363,13 @@ static int64_t refine_edge(bstate *s, int ref_itrs) {
         if (bvolnum < 0) {
           p[0] = i; p[1] = j; p[2] = k;
           max_neargrid(s, p);
-          int32_t pv = s->volnum[lin(s, p[0], p[1], p[2])];
-          if (pv < 0 || pv > s->nbasin) return -1;
+          const int64_t ie = lin(s, p[0], p[1], p[2]);
+          if (ie != id && neg0[ie]) orc_negend++;
+          int32_t pv = s->volnum[ie];
+          if (pv < 0 || pv > s->nbasin) {
+            free(neg0);
+            return -1;
+          }
           s->volnum[id] = pv;
           if (abs(bvolnum) != pv) {
             num_reassign++;
@@ -363,6 +381,7 @@ static int64_t refine_edge(bstate *s, int ref_itrs) {
           }
         }
       }
+  free(neg0);
   return num_reassign;
 }
 

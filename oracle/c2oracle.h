@@ -183,6 +183,10 @@ void oracle_slater_grid(const double x2c[9], const int n[3], int nat,
  * unscaled; sums[q*nattr + a].  Points are ranked by oracle_qcksort_r8, so
  * equal values take the reference sort's order.  Returns 0, or 2 (more than
  * xcap attractors). */
+/* Reassign paths of the last run's refinement that ended on another point
+ * negative at the start of its pass (resets the count; tests only). */
+int64_t oracle_refine_negend(void);
+
 /* qcksort_r8 (tools@proc.f90:83-168): iord[first..last] (0-based positions,
  * element indices) sorted by arr ascending, equal keys in that quicksort's
  * arrangement. */

@@ -90,3 +90,26 @@ def test_oracle_sweep_matches_only_atoms_by_identify_atom():
     # the nearest-atom rule would give label 2
     lab, _, _, _ = oracle.bader(f, n, X2C, xat + cps, mode=0)
     assert np.all(lab == 2)
+
+
+def test_refine_paths_never_end_on_a_point_being_reassigned():
+    """The refinement's reassign paths end at a maximum or a known == 2
+    point, never on another point that is being reassigned (negative): the
+    oracle counts such ends on plateau grids of random cells (the case the
+    GPU reports as an internal inconsistency)."""
+    import ctypes as C
+    from critic2_amd.synth import random_cell, BOHR_PER_ANG
+    lib = oracle.load()
+    lib.oracle_refine_negend.restype = C.c_int64
+    lib.oracle_refine_negend()
+    for seed in range(24):
+        rng = np.random.default_rng(seed)
+        n = tuple(int(x) for x in rng.integers(10, 18, 3))
+        ang = tuple(float(a) for a in rng.integers(70, 111, 3)) if seed % 2 else (90, 90, 90)
+        cell = random_cell(1000 + seed, (7.0, 7.5, 8.0), ang, int(rng.integers(2, 6)), (1, 6, 7, 8), n)
+        f = oracle.slater_grid(cell.x2c, cell.n, cell.xat, cell.zat, 8.0 * BOHR_PER_ANG)
+        m = f.max()
+        f = np.round(f / m, int(rng.integers(1, 3))) * m
+        _, info, _, _ = oracle.bader(f, cell.n, cell.x2c, cell.xat)
+        assert info["status"] == 0
+    assert lib.oracle_refine_negend() == 0
