@@ -247,6 +247,18 @@ __global__ void k_pre_flab(uint32_t N, const uint64_t *__restrict__ toff, const 
   C[s] = sweep_index(P, s) << POS_BITS;
 }
 
+// Covering by every point's first step (the plain first story's paths of
+// two points), a middle ground to start the rounds from.
+__global__ void k_pre_seed(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
+                           unsigned long long *__restrict__ C) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const uint64_t o = toff[s];
+  if (toff[s + 1] - o < 2) return;
+  const uint64_t ts = sweep_index(P, s) << POS_BITS;
+  atomicMin(&C[traj_at(P, s, tp, o, 1)], (unsigned long long)(ts | 1u));
+}
+
 __global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ C, const int32_t *__restrict__ flab,
                          uint64_t *__restrict__ Kt, uint64_t *__restrict__ Cnext, const uint64_t *__restrict__ Cold,
                          unsigned long long *nchanged) {
@@ -1099,6 +1111,9 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     int32_t *flab = reinterpret_cast<int32_t *>(cur);
     uint64_t *Ca = offN, *Cb = aoff;
     C2_LAUNCH(ctx, k_pre_flab, dim3(gN), dim3(B), 0, N, toff, tp, maxlab, flab, Ca, P);
+    // start from every point's first step covered (512^3: 900 ms of
+    // fixpoint and rounds against 923 ms from own times only)
+    C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, (unsigned long long *)Ca);
     C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, flab, Kt, Cb, (const uint64_t *)nullptr,
               (unsigned long long *)nullptr);
     unsigned long long *d_pc = d_stats + 4;
