@@ -16,6 +16,8 @@
 
 namespace c2 {
 
+constexpr int REF_SLOTS = 64;
+
 __global__ void k_ref1_mark(BParams P, uint32_t N, const int32_t *__restrict__ V,
                             uint8_t *__restrict__ E, unsigned long long *nedge) {
   const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
@@ -35,7 +37,11 @@ __global__ void k_ref1_mark(BParams P, uint32_t N, const int32_t *__restrict__ V
   }
   uint64_t m = mark;
   for (int off = 32; off > 0; off >>= 1) m += __shfl_down(m, off, 64);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(nedge, (unsigned long long)m);
+  // spread over REF_SLOTS counters (one address taking an atomic from most
+  // wavefronts of the grid serialises them); the host sums them
+  if ((threadIdx.x & 63) == 0 && m)
+    atomicAdd(nedge + ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (REF_SLOTS - 1)),
+              (unsigned long long)m);
 }
 
 // volnum = -volnum on marked points; known = 0 on their 27-neighbourhoods
@@ -217,6 +223,7 @@ RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin
   const unsigned gN = grid_for(N, B);
   uint8_t *E = buf<uint8_t>(ctx, "bader_edge", N);
   unsigned long long *d_cnt = buf<unsigned long long>(ctx, "refine_counters", 4);
+  unsigned long long *d_edge = buf<unsigned long long>(ctx, "refine_edges", REF_SLOTS);
   int *d_err = buf<int>(ctx, "refine_err", 1);
   C2_HIP(hipMemsetAsync(d_err, 0, sizeof(int), ctx->stream));
   RefineResult R{0, 0};
@@ -225,9 +232,13 @@ RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin
     if (iter > 100000) fail(C2GPU_ERR_NOCONVERGE, "c2gpu_bader: refinement did not converge");
     C2_HIP(hipMemsetAsync(d_cnt, 0, 4 * sizeof(unsigned long long), ctx->stream));
     if (iter == 1) {
-      C2_LAUNCH(ctx, k_ref1_mark, dim3(gN), dim3(B), 0, P, N, V, E, d_cnt + 2);
+      C2_HIP(hipMemsetAsync(d_edge, 0, REF_SLOTS * sizeof(unsigned long long), ctx->stream));
+      C2_LAUNCH(ctx, k_ref1_mark, dim3(gN), dim3(B), 0, P, N, V, E, d_edge);
       C2_LAUNCH(ctx, k_ref1_apply, dim3(gN), dim3(B), 0, P, N, E, V, known);
-      R.edge_points = (int64_t)read_scalar(ctx, d_cnt + 2);
+      unsigned long long h[REF_SLOTS];
+      copy_to_host(ctx, h, d_edge, sizeof(h));
+      R.edge_points = 0;
+      for (int q = 0; q < REF_SLOTS; q++) R.edge_points += (int64_t)h[q];
     } else {
       C2_LAUNCH(ctx, k_refn_mark, dim3(gN), dim3(B), 0, P, N, V, known, V2, known2);
       std::swap(V, V2);

@@ -111,10 +111,17 @@ __device__ __forceinline__ uint64_t first_at_or_after(const uint64_t *__restrict
   return lo;
 }
 
+// Counters that many wavefronts add to are spread over CSLOTS slots (slot
+// q of a counter at dst[q * CSTRIDE]; the host sums them): one address
+// taking an atomic from every wavefront of a full-grid launch serialises
+// them (k_final_state took 25 ms for its count of starts, 1.4 ms without).
+constexpr int CSLOTS = 64, CSTRIDE = 16;
 __device__ __forceinline__ void wave_add(unsigned long long *dst, uint64_t v) {
   if (!dst) return;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+  if ((threadIdx.x & 63) == 0 && v)
+    atomicAdd(dst + CSTRIDE * ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (CSLOTS - 1)),
+              (unsigned long long)v);
 }
 
 // Appends val to list where pred holds: one atomic per wavefront.  Every lane
@@ -924,6 +931,16 @@ __global__ void k_final_state(uint32_t N, Lists L, const uint64_t *__restrict__ 
 
 // ---------------------------------------------------------------- host driver
 
+// Counter q of a slotted counter array (wave_add), summed over its slots.
+static unsigned long long sum_slots(c2gpu_ctx *ctx, const unsigned long long *d, int q) {
+  unsigned long long h[CSLOTS * CSTRIDE];
+  copy_to_host(ctx, h, d, sizeof(h));
+  C2_HIP(hipStreamSynchronize(ctx->stream));
+  unsigned long long t = 0;
+  for (int s = 0; s < CSLOTS; s++) t += h[s * CSTRIDE + q];
+  return t;
+}
+
 constexpr uint64_t HUGE_BATCH = 1ull << 27;
 
 // Sorts the lists of the points in `list` (all points when null).
@@ -1032,10 +1049,10 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   uint32_t *longq = buf<uint32_t>(ctx, "sweep_longq", N);
   const unsigned gLong = (unsigned)persistent_threads(ctx->device, 256) / 256 / 2;  // 4 workgroups per CU
   int *d_err = buf<int>(ctx, "sweep_err", 1);
-  unsigned long long *d_stats = buf<unsigned long long>(ctx, "sweep_stats", 5);
-  unsigned long long *d_work = buf<unsigned long long>(ctx, "sweep_work", 16);  // algorithmic work counters
+  unsigned long long *d_stats = buf<unsigned long long>(ctx, "sweep_stats", CSLOTS * CSTRIDE);
+  unsigned long long *d_work = buf<unsigned long long>(ctx, "sweep_work", CSLOTS * CSTRIDE);  // algorithmic work counters
   C2_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
-  C2_HIP(hipMemsetAsync(d_work, 0, 16 * sizeof(unsigned long long), st));
+  C2_HIP(hipMemsetAsync(d_work, 0, CSLOTS * CSTRIDE * sizeof(unsigned long long), st));
   // host-side totals for the algorithmic-bytes report
   double s_nc = 0, s_nd = 0, s_nq = 0, s_ns = 0, s_coll = 0, n_coll = 0, n_full = 0, n_decall = 0, s_nev = 0;
   C2_HIP(hipMemsetAsync(delta, 0, sizeof(int32_t) * N, st));
@@ -1116,10 +1133,10 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, (unsigned long long *)Ca);
     C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, flab, Kt, Cb, (const uint64_t *)nullptr,
               (unsigned long long *)nullptr);
-    unsigned long long *d_pc = d_stats + 4;
+    unsigned long long *d_pc = buf<unsigned long long>(ctx, "sweep_precount", CSLOTS * CSTRIDE);
     const bool count = trace;  // covering times changed per round (diagnostics)
     for (int r = 1; r < pre_rounds; r++) {
-      if (count) C2_HIP(hipMemsetAsync(d_pc, 0, sizeof(unsigned long long), st));
+      if (count) C2_HIP(hipMemsetAsync(d_pc, 0, CSLOTS * CSTRIDE * sizeof(unsigned long long), st));
       C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, (unsigned long long *)Cb, pre_cap);
       // Cb now holds this round's covering times; Ca is reset for the next
       C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, flab, Kt, Ca, (const uint64_t *)Ca,
@@ -1127,7 +1144,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       std::swap(Ca, Cb);
       if (count)
         std::fprintf(stderr, "[sweep] pre round %d: %llu covering times changed\n", r,
-                     (unsigned long long)read_scalar(ctx, d_pc));
+                     (unsigned long long)sum_slots(ctx, d_pc, 0));
     }
     C2_LAUNCH(ctx, k_pre_story, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, flab, ell, lam);
   } else {
@@ -1157,7 +1174,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       continue;
     }
     // 1. points whose lists change, and their new sizes
-    if (trace) C2_HIP(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), st));
+    if (trace) C2_HIP(hipMemsetAsync(d_stats, 0, CSLOTS * CSTRIDE * sizeof(unsigned long long), st));
     C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, lam, lam2, delta, dflag, cfrom,
               trace ? d_stats : nullptr, d_work + 5);
     s_nc += nc;
@@ -1213,7 +1230,8 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     R.events_total += (int64_t)added;
     if (trace) {
       unsigned long long h[5];
-      copy_to_host(ctx, h, d_stats, sizeof(h));
+      for (int q = 0; q < 4; q++) h[q] = sum_slots(ctx, d_stats, q);
+      h[4] = read_scalar(ctx, d_stats + 4);  // slot 0 only: an atomic minimum
       const auto now = std::chrono::steady_clock::now();
       std::fprintf(stderr, "[sweep] it %d %.3f ms\n", it,
                    std::chrono::duration<double, std::milli>(now - t_last).count());
@@ -1244,7 +1262,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     // once (read-modify-write twice), from the host-side launch sizes and the
     // device work counters
     unsigned long long w[16];
-    copy_to_host(ctx, w, d_work, sizeof(w));
+    for (int q = 0; q < 16; q++) w[q] = sum_slots(ctx, d_work, q);
     const double dN = (double)N;
     add_kernel_bytes(ctx, "k_patch_keep", 56.0 * s_nd + 9.0 * w[0] + 8.0 * w[1]);
     add_kernel_bytes(ctx, "k_mark_decide", 6.0 * s_nq + 13.0 * w[3] + 9.0 * w[2]);
@@ -1256,10 +1274,10 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     add_kernel_bytes(ctx, "k_ev_sort", 17.0 * (s_nd + dN * n_full) + 16.0 * w[8]);
     add_kernel_bytes(ctx, "k_ev_count", n_full * 12.0 * dN + 12.0 * s_nev);
   }
-  unsigned long long *d_ns = buf<unsigned long long>(ctx, "sweep_nstarts", 1);
-  C2_HIP(hipMemsetAsync(d_ns, 0, 8, st));
+  unsigned long long *d_ns = buf<unsigned long long>(ctx, "sweep_nstarts", CSLOTS * CSTRIDE);
+  C2_HIP(hipMemsetAsync(d_ns, 0, CSLOTS * CSTRIDE * sizeof(unsigned long long), st));
   C2_LAUNCH(ctx, k_final_state, dim3(gN), dim3(B), 0, N, L, Kt, V, known, d_ns, ell);
-  R.nstarts = (int64_t)read_scalar(ctx, d_ns);
+  R.nstarts = (int64_t)sum_slots(ctx, d_ns, 0);
   R.iters = it;
   R.ell = ell;
   return R;
