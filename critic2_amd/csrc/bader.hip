@@ -94,17 +94,31 @@ __global__ void __launch_bounds__(TRAJ_B) k_traj(BParams P, TrajOrder T, unsigne
     r = __shfl(r, 0, 64);
     if (r >= T.nrows) break;  // every lane of the wavefront leaves together
     int i0, j, k;
-    if (!traj_row(P, T, r, i0, j, k)) continue;
+    if (!traj_row(P, T, r, i0, j, k)) continue;  // uniform over the wavefront
     const int i = i0 + lane;
-    if (i >= P.n1) continue;
-    const uint32_t s = lin3(P, i, j, k);
-    int L = walk_path(P, s, path, maxlen, Never{}, NoVisit{});
-    if (L < 0) {
-      atomicOr(err, 1);
-      L = 1;
+    const bool live = i < P.n1;
+    const uint32_t s = live ? lin3(P, i, j, k) : 0u;
+    int L = 0;
+    if (live) {
+      L = walk_path(P, s, path, maxlen, Never{}, NoVisit{});
+      if (L < 0) {
+        atomicOr(err, 1);
+        L = 1;
+      }
     }
+    // the wavefront's paths side by side in the pool: one reservation per
+    // wavefront instead of one per lane on the shared counter
+    uint32_t incl = (uint32_t)L;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    unsigned long long base = 0;
+    if (lane == 63) base = atomicAdd(pool_top, (unsigned long long)incl);
+    base = __shfl(base, 63, 64);
+    if (!live) continue;
+    const unsigned long long at = base + (incl - (uint32_t)L);
     len[s] = (uint32_t)L;
-    const unsigned long long at = atomicAdd(pool_top, (unsigned long long)L);
     pos[s] = at;
     if (at + (unsigned long long)L <= pool_cap)
       for (int q = 0; q < L; q++) pool[at + q] = path.get(q, L);
