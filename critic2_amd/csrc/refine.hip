@@ -158,7 +158,14 @@ __global__ void k_refn_clear(BParams P, uint32_t N, const int32_t *__restrict__ 
 // Reassign sweep (bader@proc.f90:391-417): every negative point ascends
 // against the frozen known==2 set; the label read at the path end is frozen
 // during the sweep, so the points are independent.
-__global__ void k_reassign(BParams P, uint32_t N, const int32_t *__restrict__ V,
+// The points to reassign (negative) listed in grid order, so that every
+// lane of the reassign sweep walks.
+__global__ void k_neg_flags(uint32_t N, const int32_t *__restrict__ V, uint8_t *__restrict__ flag) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) flag[s] = V[s] < 0;
+}
+
+__global__ void k_reassign(BParams P, uint32_t nneg, const uint32_t *__restrict__ neg, const int32_t *__restrict__ V,
                            int32_t *__restrict__ Vout, int8_t *__restrict__ known,
                            uint32_t *scratch, int maxlen, int nbasin,
                            unsigned long long *nre, int *err) {
@@ -166,12 +173,9 @@ __global__ void k_reassign(BParams P, uint32_t N, const int32_t *__restrict__ V,
   const uint32_t nth = gridDim.x * blockDim.x;
   uint32_t *path = scratch + (size_t)tid * maxlen;
   uint64_t mine = 0;
-  for (uint32_t s = tid; s < N; s += nth) {
+  for (uint32_t t = tid; t < nneg; t += nth) {
+    const uint32_t s = neg[t];
     const int32_t b = V[s];
-    if (b >= 0) {
-      Vout[s] = b;
-      continue;
-    }
     int L = walk_path(
         P, s, FlatPath{path}, maxlen, [&](uint32_t x) { return known[x] == 2; },
         [&](uint32_t x) {
@@ -224,6 +228,8 @@ RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin
   uint8_t *E = buf<uint8_t>(ctx, "bader_edge", N);
   unsigned long long *d_cnt = buf<unsigned long long>(ctx, "refine_counters", 4);
   unsigned long long *d_edge = buf<unsigned long long>(ctx, "refine_edges", REF_SLOTS);
+  uint8_t *negf = buf<uint8_t>(ctx, "refine_negf", N);
+  uint32_t *negl = buf<uint32_t>(ctx, "refine_negl", N);
   int *d_err = buf<int>(ctx, "refine_err", 1);
   C2_HIP(hipMemsetAsync(d_err, 0, sizeof(int), ctx->stream));
   RefineResult R{0, 0};
@@ -245,8 +251,13 @@ RefineResult refine_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, int nbasin
       std::swap(known, known2);
       C2_LAUNCH(ctx, k_refn_clear, dim3(gN), dim3(B), 0, P, N, V, known);
     }
-    C2_LAUNCH(ctx, k_reassign, dim3(nth / B), dim3(B), 0, P, N, V, V2, known, scratch, maxlen, nbasin, d_cnt,
-              d_err);
+    // every point keeps its label but the negative ones, which the sweep
+    // below reassigns from the listed points only
+    C2_LAUNCH(ctx, k_neg_flags, dim3(gN), dim3(B), 0, N, V, negf);
+    const uint32_t nneg = compact_flags(ctx, "refneg", negf, N, negl);
+    C2_HIP(hipMemcpyAsync(V2, V, 4 * (size_t)N, hipMemcpyDeviceToDevice, ctx->stream));
+    C2_LAUNCH(ctx, k_reassign, dim3(nth / B), dim3(B), 0, P, nneg, negl, V, V2, known, scratch, maxlen, nbasin,
+              d_cnt, d_err);
     std::swap(V, V2);
     const unsigned long long nre = read_scalar(ctx, d_cnt);
     const int e = read_scalar(ctx, d_err);
