@@ -266,7 +266,22 @@ __global__ void k_pre_seed(BParams P, uint32_t N, const uint64_t *__restrict__ t
   atomicMin(&C[traj_at(P, s, tp, o, 1)], (unsigned long long)(ts | 1u));
 }
 
-__global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ C, const int32_t *__restrict__ flab,
+// Whether a point and its six face neighbours carry one positive free
+// label: fixed for the rounds, computed once.
+__global__ void k_pre_agree(BParams P, uint32_t N, const int32_t *__restrict__ flab, uint8_t *__restrict__ agree) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int i, j, k;
+  coords(P, s, i, j, k);
+  const int32_t la = flab[s];
+  bool same = la > 0;
+  same = same && flab[lin3(P, wrap1(i + 1, P.n1), j, k)] == la && flab[lin3(P, wrap1(i - 1, P.n1), j, k)] == la;
+  same = same && flab[lin3(P, i, wrap1(j + 1, P.n2), k)] == la && flab[lin3(P, i, wrap1(j - 1, P.n2), k)] == la;
+  same = same && flab[lin3(P, i, j, wrap1(k + 1, P.n3))] == la && flab[lin3(P, i, j, wrap1(k - 1, P.n3))] == la;
+  agree[s] = same;
+}
+
+__global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ C, const uint8_t *__restrict__ agree,
                          uint64_t *__restrict__ Kt, uint64_t *__restrict__ Cnext, const uint64_t *__restrict__ Cold,
                          unsigned long long *nchanged) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -275,20 +290,22 @@ __global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ C, 
   // the next round's covering times start at each point's own time (a point
   // no path covers is a start and covers itself)
   Cnext[s] = sweep_index(P, s) << POS_BITS;
+  if (!agree[s]) {
+    Kt[s] = KT_INF;
+    return;
+  }
   int i, j, k;
   coords(P, s, i, j, k);
-  const int32_t la = flab[s];
   uint64_t m = C[s];
-  bool ok = la > 0;
-  const int d[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  const uint32_t z[6] = {lin3(P, wrap1(i + 1, P.n1), j, k), lin3(P, wrap1(i - 1, P.n1), j, k),
+                         lin3(P, i, wrap1(j + 1, P.n2), k), lin3(P, i, wrap1(j - 1, P.n2), k),
+                         lin3(P, i, j, wrap1(k + 1, P.n3)), lin3(P, i, j, wrap1(k - 1, P.n3))};
 #pragma unroll
   for (int q = 0; q < 6; q++) {
-    const uint32_t z = lin3(P, wrap1(i + d[q][0], P.n1), wrap1(j + d[q][1], P.n2), wrap1(k + d[q][2], P.n3));
-    ok = ok && flab[z] == la;
-    const uint64_t c = C[z];
+    const uint64_t c = C[z[q]];
     m = c > m ? c : m;
   }
-  Kt[s] = ok ? m : KT_INF;
+  Kt[s] = m;
 }
 
 __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
@@ -1131,7 +1148,9 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     // start from every point's first step covered (512^3: 900 ms of
     // fixpoint and rounds against 923 ms from own times only)
     C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, (unsigned long long *)Ca);
-    C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, flab, Kt, Cb, (const uint64_t *)nullptr,
+    uint8_t *agree = uni;  // rewritten by the first full build
+    C2_LAUNCH(ctx, k_pre_agree, dim3(gN), dim3(B), 0, P, N, flab, agree);
+    C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, agree, Kt, Cb, (const uint64_t *)nullptr,
               (unsigned long long *)nullptr);
     unsigned long long *d_pc = buf<unsigned long long>(ctx, "sweep_precount", CSLOTS * CSTRIDE);
     const bool count = trace;  // covering times changed per round (diagnostics)
@@ -1142,7 +1161,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       const int cap_r = r < 8 ? 8 : pre_cap;
       C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, (unsigned long long *)Cb, cap_r);
       // Cb now holds this round's covering times; Ca is reset for the next
-      C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, flab, Kt, Ca, (const uint64_t *)Ca,
+      C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, agree, Kt, Ca, (const uint64_t *)Ca,
                 count ? d_pc : (unsigned long long *)nullptr);
       std::swap(Ca, Cb);
       if (count)
