@@ -245,25 +245,34 @@ __device__ __forceinline__ int stop_position(const BParams &P, uint32_t s, const
 // not covered before its own turn; its path stops at the first point whose
 // Kt is earlier); rounds alternate C and Kt from C = each point's own time.
 // Labels that later paths change are what the fixpoint then corrects.
+//
+// The rounds compare covering times only with a start's own time, whose
+// path position is 0, so they keep sweep indices alone (32 bits; the
+// position on the covering path never decides a comparison).
+constexpr uint32_t PRE_INF = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t sweep_index32(const BParams &P, uint32_t l) {
+  return (uint32_t)sweep_index(P, l);
+}
+
 __global__ void k_pre_flab(uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
                            const int32_t *__restrict__ maxlab, int32_t *__restrict__ flab,
-                           uint64_t *__restrict__ C, BParams P) {
+                           uint32_t *__restrict__ C, BParams P) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   flab[s] = maxlab[tp[toff[s + 1] - 1]];
-  C[s] = sweep_index(P, s) << POS_BITS;
+  C[s] = sweep_index32(P, s);
 }
 
 // Covering by every point's first step (the plain first story's paths of
 // two points), a middle ground to start the rounds from.
 __global__ void k_pre_seed(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                           unsigned long long *__restrict__ C) {
+                           uint32_t *__restrict__ C) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const uint64_t o = toff[s];
   if (toff[s + 1] - o < 2) return;
-  const uint64_t ts = sweep_index(P, s) << POS_BITS;
-  atomicMin(&C[traj_at(P, s, tp, o, 1)], (unsigned long long)(ts | 1u));
+  atomicMin(&C[traj_at(P, s, tp, o, 1)], sweep_index32(P, s));
 }
 
 // Whether a point and its six face neighbours carry one positive free
@@ -281,39 +290,39 @@ __global__ void k_pre_agree(BParams P, uint32_t N, const int32_t *__restrict__ f
   agree[s] = same;
 }
 
-__global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ C, const uint8_t *__restrict__ agree,
-                         uint64_t *__restrict__ Kt, uint64_t *__restrict__ Cnext, const uint64_t *__restrict__ Cold,
+__global__ void k_pre_kt(BParams P, uint32_t N, const uint32_t *__restrict__ C, const uint8_t *__restrict__ agree,
+                         uint32_t *__restrict__ Kt, uint32_t *__restrict__ Cnext, const uint32_t *__restrict__ Cold,
                          unsigned long long *nchanged) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (nchanged) wave_add(nchanged, s < N && C[s] != Cold[s] ? 1 : 0);
   if (s >= N) return;
   // the next round's covering times start at each point's own time (a point
   // no path covers is a start and covers itself)
-  Cnext[s] = sweep_index(P, s) << POS_BITS;
+  Cnext[s] = sweep_index32(P, s);
   if (!agree[s]) {
-    Kt[s] = KT_INF;
+    Kt[s] = PRE_INF;
     return;
   }
   int i, j, k;
   coords(P, s, i, j, k);
-  uint64_t m = C[s];
+  uint32_t m = C[s];
   const uint32_t z[6] = {lin3(P, wrap1(i + 1, P.n1), j, k), lin3(P, wrap1(i - 1, P.n1), j, k),
                          lin3(P, i, wrap1(j + 1, P.n2), k), lin3(P, i, wrap1(j - 1, P.n2), k),
                          lin3(P, i, j, wrap1(k + 1, P.n3)), lin3(P, i, j, wrap1(k - 1, P.n3))};
 #pragma unroll
   for (int q = 0; q < 6; q++) {
-    const uint64_t c = C[z[q]];
+    const uint32_t c = C[z[q]];
     m = c > m ? c : m;
   }
   Kt[s] = m;
 }
 
 __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                            const uint64_t *__restrict__ Cold, const uint64_t *__restrict__ Kt,
-                            unsigned long long *__restrict__ Cnew, int cap) {
+                            const uint32_t *__restrict__ Cold, const uint32_t *__restrict__ Kt,
+                            uint32_t *__restrict__ Cnew, int cap) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  const uint64_t ts = sweep_index(P, s) << POS_BITS;
+  const uint32_t ts = sweep_index32(P, s);
   if (Cold[s] < ts) return;  // covered before its turn: not a start
   // Cnew starts at each point's own time, so the start itself needs no
   // update; a covering time only decreases, so a value already at or below
@@ -324,18 +333,16 @@ __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ 
   // PRE_CHUNK path points at a time: their indices and Kt are loaded
   // together (independent loads), then the stop is found among them
   for (int q0 = 1; q0 < len; q0 += PRE_CHUNK) {
-    uint32_t x[PRE_CHUNK];
-    uint64_t kt[PRE_CHUNK];
+    uint32_t x[PRE_CHUNK], kt[PRE_CHUNK];
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : 0ull;
+    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : 0u;
     bool stop = false;
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) {
       if (stop || q0 + u >= len) break;
-      const unsigned long long c = ts | (uint64_t)(q0 + u);
-      if (c < __hip_atomic_load(&Cnew[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&Cnew[x[u]], c);
+      if (ts < __hip_atomic_load(&Cnew[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&Cnew[x[u]], ts);
       stop = kt[u] < ts;
     }
     if (stop) break;
@@ -343,11 +350,11 @@ __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ 
 }
 
 __global__ void k_pre_story(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                            const uint64_t *__restrict__ C, const uint64_t *__restrict__ Kt,
+                            const uint32_t *__restrict__ C, const uint32_t *__restrict__ Kt,
                             const int32_t *__restrict__ flab, int32_t *__restrict__ ell, int32_t *__restrict__ lam) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  const uint64_t ts = sweep_index(P, s) << POS_BITS;
+  const uint32_t ts = sweep_index32(P, s);
   if (C[s] < ts) {
     ell[s] = 0;
     lam[s] = 0;
@@ -355,7 +362,17 @@ __global__ void k_pre_story(BParams P, uint32_t N, const uint64_t *__restrict__ 
   }
   const uint64_t o = toff[s];
   const int len = (int)(toff[s + 1] - o);
-  const int e = stop_position(P, s, tp, o, len, Kt, ts);
+  int e = len;
+  for (int q0 = 1; q0 < len && e == len; q0 += PRE_CHUNK) {
+    uint32_t x[PRE_CHUNK], kt[PRE_CHUNK];
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : PRE_INF;
+#pragma unroll
+    for (int u = 0; u < PRE_CHUNK; u++)
+      if (e == len && kt[u] < ts) e = q0 + u + 1;
+  }
   ell[s] = e;
   lam[s] = flab[e > 1 ? traj_at(P, s, tp, o, e - 1) : s];
 }
@@ -1143,14 +1160,15 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   if (pre_rounds > 0) {
     // buffers the first full build overwrites: flab in cur, C in offN / aoff
     int32_t *flab = reinterpret_cast<int32_t *>(cur);
-    uint64_t *Ca = offN, *Cb = aoff;
+    uint32_t *Ca = reinterpret_cast<uint32_t *>(offN), *Cb = reinterpret_cast<uint32_t *>(aoff);
+    uint32_t *Kp = alloc;  // the rounds' known == 2 times (sweep indices)
     C2_LAUNCH(ctx, k_pre_flab, dim3(gN), dim3(B), 0, N, toff, tp, maxlab, flab, Ca, P);
     // start from every point's first step covered (512^3: 900 ms of
     // fixpoint and rounds against 923 ms from own times only)
-    C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, (unsigned long long *)Ca);
+    C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca);
     uint8_t *agree = uni;  // rewritten by the first full build
     C2_LAUNCH(ctx, k_pre_agree, dim3(gN), dim3(B), 0, P, N, flab, agree);
-    C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, agree, Kt, Cb, (const uint64_t *)nullptr,
+    C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, agree, Kp, Cb, (const uint32_t *)nullptr,
               (unsigned long long *)nullptr);
     unsigned long long *d_pc = buf<unsigned long long>(ctx, "sweep_precount", CSLOTS * CSTRIDE);
     const bool count = trace;  // covering times changed per round (diagnostics)
@@ -1159,16 +1177,16 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       // the first rounds, whose known == 2 times are still late, follow 8
       // points only (512^3: 773 ms of rounds and fixpoint against 813 ms)
       const int cap_r = r < 8 ? 8 : pre_cap;
-      C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, (unsigned long long *)Cb, cap_r);
+      C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kp, Cb, cap_r);
       // Cb now holds this round's covering times; Ca is reset for the next
-      C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, agree, Kt, Ca, (const uint64_t *)Ca,
+      C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, agree, Kp, Ca, (const uint32_t *)Ca,
                 count ? d_pc : (unsigned long long *)nullptr);
       std::swap(Ca, Cb);
       if (count)
         std::fprintf(stderr, "[sweep] pre round %d: %llu covering times changed\n", r,
                      (unsigned long long)sum_slots(ctx, d_pc, 0));
     }
-    C2_LAUNCH(ctx, k_pre_story, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kt, flab, ell, lam);
+    C2_LAUNCH(ctx, k_pre_story, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kp, flab, ell, lam);
   } else {
     C2_LAUNCH(ctx, k_story_init, dim3(gN), dim3(B), 0, N, toff, tp, maxlab, ell, lam);
   }
