@@ -148,13 +148,28 @@ __global__ void __launch_bounds__(256) k_collect(uint32_t N, uint8_t *__restrict
   const uint32_t nq = (N + 15) >> 4;
   uint4 *fq = reinterpret_cast<uint4 *>(flag);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < nq; q0 += gridDim.x * blockDim.x) {
-    const uint32_t q = q0 + threadIdx.x;
-    const uint4 v = (q < nq) ? fq[q] : make_uint4(0u, 0u, 0u, 0u);
-    const bool any = (v.x | v.y | v.z | v.w) != 0u;
-    if (clear && any) fq[q] = make_uint4(0u, 0u, 0u, 0u);
-    const int c = __popc(v.x & 0x01010101u) + __popc(v.y & 0x01010101u) + __popc(v.z & 0x01010101u) +
-                  __popc(v.w & 0x01010101u);
+  // COLLECT_Q groups of 16 flags per thread and pass: the loads of a pass
+  // are in flight together, and the scan and barriers are paid once per
+  // COLLECT_Q groups
+  constexpr int COLLECT_Q = 4;
+  const uint32_t span = blockDim.x * COLLECT_Q;
+  for (uint32_t q0 = blockIdx.x * span; q0 < nq; q0 += gridDim.x * span) {
+    uint4 vv[COLLECT_Q];
+    int cc[COLLECT_Q];
+    int c = 0;
+#pragma unroll
+    for (int g = 0; g < COLLECT_Q; g++) {
+      const uint32_t q = q0 + g * blockDim.x + threadIdx.x;
+      vv[g] = (q < nq) ? fq[q] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int g = 0; g < COLLECT_Q; g++) {
+      const uint4 v = vv[g];
+      if (clear && (v.x | v.y | v.z | v.w) != 0u) fq[q0 + g * blockDim.x + threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+      cc[g] = __popc(v.x & 0x01010101u) + __popc(v.y & 0x01010101u) + __popc(v.z & 0x01010101u) +
+              __popc(v.w & 0x01010101u);
+      c += cc[g];
+    }
     int incl = c;
     for (int o = 1; o < 64; o <<= 1) {
       const int t = __shfl_up(incl, o, 64);
@@ -169,12 +184,15 @@ __global__ void __launch_bounds__(256) k_collect(uint32_t N, uint8_t *__restrict
     __syncthreads();
     uint32_t o = bbase + (uint32_t)(incl - c);
     for (int w = 0; w < wv; w++) o += wsum[w];
-    if (c) {
-      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-      for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 4; b++)
-          if ((wd[a] >> (8 * b)) & 1u) list[o++] = 16 * q + 4 * a + b;
-    }
+    if (c)
+      for (int g = 0; g < COLLECT_Q; g++) {
+        if (!cc[g]) continue;
+        const uint32_t q = q0 + g * blockDim.x + threadIdx.x;
+        const uint32_t wd[4] = {vv[g].x, vv[g].y, vv[g].z, vv[g].w};
+        for (int a = 0; a < 4; a++)
+          for (int b = 0; b < 4; b++)
+            if ((wd[a] >> (8 * b)) & 1u) list[o++] = 16 * q + 4 * a + b;
+      }
     __syncthreads();  // wsum and bbase are rewritten by the next pass
   }
 }
