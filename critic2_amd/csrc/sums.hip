@@ -23,24 +23,29 @@ struct FieldPtrs {
   const double *p[4];
 };
 
-// Bins layout in LDS and in the partials: [q][a], q = 0 count, q>0 field q-1.
-__global__ void __launch_bounds__(64 * SUM_WAVES)
-    k_basin_partials(const int32_t *__restrict__ lab, uint64_t npts, int nattr, int nf,
-                     FieldPtrs F, uint64_t chunk, double *__restrict__ partial) {
-  extern __shared__ double bins[];  // SUM_WAVES * (nf+1) * nattr
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nb = (nf + 1) * nattr;
-  for (int q = threadIdx.x; q < nb * SUM_WAVES; q += blockDim.x) bins[q] = 0.0;
-  __syncthreads();
-  double *wb = bins + (size_t)wave * nb;
-  const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
-  uint64_t b1 = b0 + chunk;
-  if (b1 > npts) b1 = npts;
-  // each wave handles 64-point tiles, tiles interleaved over the waves
-  for (uint64_t t = b0 + (uint64_t)wave * 64; t < b1; t += 64 * SUM_WAVES) {
-    const uint64_t p = t + lane;
-    const bool in = p < b1;
-    const int32_t l = in ? lab[p] : -1;
+// One wavefront's 64-point tile added into its bins wb[q*nattr + a]
+// (q = 0 count, q > 0 field q-1): runs of equal labels found by a segmented
+// scan over the lanes, each run's total added by its tail in lane order; a
+// tile of one label (the common case, labels being spatially coherent) is a
+// plain reduction.  The order of the additions depends only on the grid.
+__device__ __forceinline__ void tile_sums(const int32_t *__restrict__ lab, uint64_t t, uint64_t b1, int nattr, int nf,
+                                          const FieldPtrs &F, double *wb) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t p = t + lane;
+  const bool in = p < b1;
+  const int32_t l = in ? lab[p] : -1;
+  const int32_t l0 = __shfl(l, 0, 64);
+  if (__all(in && l == l0)) {
+    if (l0 < 1 || l0 > nattr) return;
+    if (lane == 0) wb[l0 - 1] += 64.0;
+    for (int q = 1; q <= nf; q++) {
+      double v = F.p[q - 1][p];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) wb[(size_t)q * nattr + (l0 - 1)] += v;
+    }
+    return;
+  }
+  {
     const int32_t lprev = __shfl_up(l, 1, 64);
     const int32_t lnext = __shfl_down(l, 1, 64);
     const bool head = (lane == 0) || (l != lprev);
@@ -69,6 +74,23 @@ __global__ void __launch_bounds__(64 * SUM_WAVES)
       }
     }
   }
+}
+
+// Bins layout in LDS and in the partials: [q][a], q = 0 count, q>0 field q-1.
+__global__ void __launch_bounds__(64 * SUM_WAVES)
+    k_basin_partials(const int32_t *__restrict__ lab, uint64_t npts, int nattr, int nf,
+                     FieldPtrs F, uint64_t chunk, double *__restrict__ partial) {
+  extern __shared__ double bins[];  // SUM_WAVES * (nf+1) * nattr
+  const int wave = threadIdx.x >> 6;
+  const int nb = (nf + 1) * nattr;
+  for (int q = threadIdx.x; q < nb * SUM_WAVES; q += blockDim.x) bins[q] = 0.0;
+  __syncthreads();
+  double *wb = bins + (size_t)wave * nb;
+  const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+  uint64_t b1 = b0 + chunk;
+  if (b1 > npts) b1 = npts;
+  // each wave handles 64-point tiles, tiles interleaved over the waves
+  for (uint64_t t = b0 + (uint64_t)wave * 64; t < b1; t += 64 * SUM_WAVES) tile_sums(lab, t, b1, nattr, nf, F, wb);
   __syncthreads();
   // fold the waves in wave order and write the block partial
   for (int q = threadIdx.x; q < nb; q += blockDim.x) {
@@ -76,6 +98,21 @@ __global__ void __launch_bounds__(64 * SUM_WAVES)
     for (int w = 0; w < SUM_WAVES; w++) s += bins[(size_t)w * nb + q];
     partial[(size_t)blockIdx.x * nb + q] = s;
   }
+}
+
+// The same with each wavefront's bins in its own row of global memory
+// (zeroed beforehand; rows in block-then-wave order): for more bins than
+// the LDS holds.
+__global__ void __launch_bounds__(64 * SUM_WAVES)
+    k_basin_partials_rows(const int32_t *__restrict__ lab, uint64_t npts, int nattr, int nf, FieldPtrs F,
+                          uint64_t chunk, double *__restrict__ rows) {
+  const int wave = threadIdx.x >> 6;
+  const int nb = (nf + 1) * nattr;
+  double *wb = rows + ((size_t)blockIdx.x * SUM_WAVES + wave) * nb;
+  const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+  uint64_t b1 = b0 + chunk;
+  if (b1 > npts) b1 = npts;
+  for (uint64_t t = b0 + (uint64_t)wave * 64; t < b1; t += 64 * SUM_WAVES) tile_sums(lab, t, b1, nattr, nf, F, wb);
 }
 
 __global__ void k_basin_reduce(const double *__restrict__ partial, int nblocks, int nb,
@@ -100,8 +137,8 @@ void basin_sums_run(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int natt
                     int64_t *counts_out) {
   if (npts <= 0 || nattr <= 0 || nfields < 0 || nfields > 4)
     fail(C2GPU_ERR_ARG, "c2gpu_basin_sums: need npts > 0, nattr > 0, 0 <= nfields <= 4");
-  if ((int64_t)nattr * (nfields + 1) * SUM_WAVES * 8 > 65536)
-    fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_basin_sums: nattr*(nfields+1) too large for LDS bins");
+  // more bins than the LDS holds go to per-wavefront rows in global memory
+  const bool rows = (int64_t)nattr * (nfields + 1) * SUM_WAVES * 8 > 65536;
   hipStream_t st = ctx->stream;
   const bool dev = flags & C2GPU_PTR_DEVICE;
   const int32_t *d_lab = label;
@@ -124,14 +161,31 @@ void basin_sums_run(c2gpu_ctx *ctx, const int32_t *label, int64_t npts, int natt
   uint64_t chunk = ((uint64_t)npts + 2047) / 2048;
   chunk = ((chunk + tile - 1) / tile) * tile;
   if (chunk < tile) chunk = tile;
-  const int nblocks = (int)(((uint64_t)npts + chunk - 1) / chunk);
-  double *partial = buf<double>(ctx, "sum_partial", (size_t)nblocks * nb);
+  int nblocks = (int)(((uint64_t)npts + chunk - 1) / chunk);
   double *d_out = buf<double>(ctx, "sum_out", nb);
-  const size_t lds = sizeof(double) * (size_t)nb * SUM_WAVES;
-  C2_LAUNCH(ctx, k_basin_partials, dim3(nblocks), dim3(64 * SUM_WAVES), lds, d_lab,
-                     (uint64_t)npts, nattr, nfields, F, chunk, partial);
-  C2_LAUNCH(ctx, k_basin_reduce, dim3(grid_for(nb, 256)), dim3(256), 0, partial, nblocks,
-                     nb, d_out);
+  if (!rows) {
+    double *partial = buf<double>(ctx, "sum_partial", (size_t)nblocks * nb);
+    const size_t lds = sizeof(double) * (size_t)nb * SUM_WAVES;
+    C2_LAUNCH(ctx, k_basin_partials, dim3(nblocks), dim3(64 * SUM_WAVES), lds, d_lab, (uint64_t)npts, nattr, nfields,
+              F, chunk, partial);
+    C2_LAUNCH(ctx, k_basin_reduce, dim3(grid_for(nb, 256)), dim3(256), 0, partial, nblocks, nb, d_out);
+  } else {
+    // fewer, longer chunks keep the rows (8 bytes x bins x 4 per block) to
+    // at most about 1 GiB
+    const uint64_t row_bytes = 8ull * nb * SUM_WAVES;
+    uint64_t maxb = ((uint64_t)1 << 30) / row_bytes;
+    if (maxb < 1) maxb = 1;
+    if ((uint64_t)nblocks > maxb) {
+      chunk = ((uint64_t)npts + maxb - 1) / maxb;
+      chunk = ((chunk + tile - 1) / tile) * tile;
+      nblocks = (int)(((uint64_t)npts + chunk - 1) / chunk);
+    }
+    double *rowsb = buf<double>(ctx, "sum_rows", (size_t)nblocks * SUM_WAVES * nb);
+    C2_HIP(hipMemsetAsync(rowsb, 0, sizeof(double) * (size_t)nblocks * SUM_WAVES * nb, st));
+    C2_LAUNCH(ctx, k_basin_partials_rows, dim3(nblocks), dim3(64 * SUM_WAVES), 0, d_lab, (uint64_t)npts, nattr,
+              nfields, F, chunk, rowsb);
+    C2_LAUNCH(ctx, k_basin_reduce, dim3(grid_for(nb, 256)), dim3(256), 0, rowsb, nblocks * SUM_WAVES, nb, d_out);
+  }
   std::vector<double> h(nb);
   copy_to_host(ctx, h.data(), d_out, 8 * (size_t)nb);
   C2_HIP(hipStreamSynchronize(st));

@@ -143,6 +143,40 @@ def test_basin_sums_deterministic_and_ragged(ctx):
         np.testing.assert_allclose(s1, s_ref, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(fld).sum()))
 
 
+@pytest.mark.parametrize("nattr,nf", [(1500, 1), (6000, 2), (40000, 0)])
+def test_basin_sums_many_attractors(ctx, nattr, nf):
+    """More bins than the LDS holds (attractors x (fields + 1) x 4 waves x 8
+    bytes > 64 KiB): per-wavefront rows in global memory, same results;
+    labels in runs (basins) and scattered (a ragged tail)."""
+    rng = np.random.default_rng(nattr)
+    npts = 300007
+    lab = np.sort(rng.integers(0, nattr + 1, size=npts)).astype(np.int32)
+    lab[-5000:] = rng.integers(0, nattr + 1, size=5000)
+    flds = [rng.standard_normal(npts) for _ in range(nf)]
+    c1, s1 = gb.basin_sums(ctx, lab, nattr, flds)
+    c2, s2 = gb.basin_sums(ctx, lab, nattr, flds)
+    assert np.array_equal(s1, s2) and np.array_equal(c1, c2)
+    c_ref, s_ref = oracle.basin_sums(lab, nattr, flds)
+    np.testing.assert_array_equal(c1, c_ref)
+    if nf:
+        np.testing.assert_allclose(s1, s_ref, rtol=1e-12, atol=1e-12 * max(1.0, float(np.abs(flds[0]).sum())))
+
+
+def test_basin_sums_single_label_tiles(ctx):
+    """Long runs of one label (whole 64-point tiles) take the plain
+    reduction; the result still matches the oracle and repeats bit for bit."""
+    rng = np.random.default_rng(11)
+    npts = 64 * 5000 + 17
+    lab = np.repeat(np.arange(1, 11, dtype=np.int32), npts // 10 + 1)[:npts]
+    fld = rng.random(npts)
+    c1, s1 = gb.basin_sums(ctx, lab, 10, [fld])
+    _, s2 = gb.basin_sums(ctx, lab, 10, [fld])
+    assert np.array_equal(s1, s2)
+    c_ref, s_ref = oracle.basin_sums(lab, 10, [fld])
+    np.testing.assert_array_equal(c1, c_ref)
+    np.testing.assert_allclose(s1, s_ref, rtol=1e-12)
+
+
 def test_step_pointers_match_oracle(ctx):
     cell = CASES["triclinic-40x44x48-6atoms"]()
     f = _grid(ctx, cell)
