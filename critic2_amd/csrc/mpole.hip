@@ -20,6 +20,7 @@
 // c2gpu_yt call), the stored surface weight at its surface points (label 0).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -115,12 +116,15 @@ __device__ __forceinline__ void shortest_offset(const MpGeo &G, int i, int j, in
   }
 }
 
+// Bins of a workgroup: in LDS, or with gbins (more bins than the LDS
+// holds) in the workgroup's own row of global memory.
 __global__ void __launch_bounds__(MB) k_mpole(MpGeo G, uint64_t npts, const int32_t *__restrict__ lab,
                                                const double *__restrict__ f, const double *__restrict__ xattr,
-                                               double *__restrict__ out) {
-  extern __shared__ double bins[];  // nattr * nlm
+                                               double *__restrict__ out, double *__restrict__ gbins) {
+  extern __shared__ double lbins[];  // nattr * nlm
   const int nlm = (G.lmax + 1) * (G.lmax + 1);
   const int nb = G.nattr * nlm;
+  double *bins = gbins ? gbins + (size_t)blockIdx.x * nb : lbins;
   for (int q = threadIdx.x; q < nb; q += MB) bins[q] = 0.0;
   __syncthreads();
   double R[NLM_MAX];
@@ -152,10 +156,12 @@ __global__ void __launch_bounds__(MB) k_mpole_yt_surface(MpGeo G, uint32_t nias,
                                                           const uint16_t *__restrict__ wc,
                                                           const uint32_t *__restrict__ wb,
                                                           const double *__restrict__ ww, const double *__restrict__ f,
-                                                          const double *__restrict__ xattr, double *__restrict__ out) {
-  extern __shared__ double bins[];  // nattr * nlm
+                                                          const double *__restrict__ xattr, double *__restrict__ out,
+                                                          double *__restrict__ gbins) {
+  extern __shared__ double lbins[];  // nattr * nlm
   const int nlm = (G.lmax + 1) * (G.lmax + 1);
   const int nb = G.nattr * nlm;
+  double *bins = gbins ? gbins + (size_t)blockIdx.x * nb : lbins;
   for (int q = threadIdx.x; q < nb; q += MB) bins[q] = 0.0;
   __syncthreads();
   double R[NLM_MAX];
@@ -207,8 +213,11 @@ void mpole_impl(c2gpu_ctx *ctx, const char *who, bool yt, const int32_t *label, 
       fail(C2GPU_ERR_ARG, w + ": the grid and nattr must be those of the last c2gpu_yt call");
   }
   const int nlm = (lmax + 1) * (lmax + 1);
-  const size_t lds = 8 * (size_t)nattr * nlm;
-  if (lds > 120 * 1024) fail(C2GPU_ERR_UNSUPPORTED, w + ": nattr * (lmax+1)^2 too large for LDS bins");
+  const size_t nbins = (size_t)nattr * nlm;
+  // bins per workgroup in LDS, or in rows of global memory when they do
+  // not fit (fewer workgroups then, the rows at most about 1 GiB)
+  const bool rows = 8 * nbins > 120 * 1024;
+  const size_t lds = rows ? 0 : 8 * nbins;
   const uint64_t npts = (uint64_t)n[0] * n[1] * n[2];
   hipStream_t st = ctx->stream;
   MpGeo G{};
@@ -253,15 +262,21 @@ void mpole_impl(c2gpu_ctx *ctx, const char *who, bool yt, const int32_t *label, 
   C2_HIP(hipMemcpyAsync(d_x, xattr, 24 * (size_t)nattr, hipMemcpyHostToDevice, st));
   double *d_out = buf<double>(ctx, "mp_out", (size_t)nattr * nlm);
   C2_HIP(hipMemsetAsync(d_out, 0, 8 * (size_t)nattr * nlm, st));
-  const unsigned gp = (unsigned)persistent_threads(ctx->device, MB) / MB;
+  unsigned gp = (unsigned)persistent_threads(ctx->device, MB) / MB;
+  double *gbins = nullptr;
+  if (rows) {
+    const uint64_t fit = ((uint64_t)1 << 30) / (8 * nbins);
+    gp = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(gp, fit));
+    gbins = buf<double>(ctx, "mp_rows", (size_t)gp * nbins);
+  }
   const unsigned g = std::min<unsigned>(grid_for((int64_t)npts, MB), gp);
-  C2_LAUNCH(ctx, k_mpole, dim3(g), dim3(MB), lds, G, npts, d_lab, d_f, d_x, d_out);
+  C2_LAUNCH(ctx, k_mpole, dim3(g), dim3(MB), lds, G, npts, d_lab, d_f, d_x, d_out, gbins);
   add_kernel_bytes(ctx, "k_mpole", 12.0 * (double)npts);
   if (yt && ctx->yt.nias) {
     const auto &Y = ctx->yt;
     const unsigned gs = std::min<unsigned>(grid_for((int64_t)Y.nias, MB), gp);
     C2_LAUNCH(ctx, k_mpole_yt_surface, dim3(gs), dim3(MB), lds, G, Y.nias, Y.ias, Y.woff, Y.wc, Y.wb, Y.ww, d_f,
-              d_x, d_out);
+              d_x, d_out, gbins);
     add_kernel_bytes(ctx, "k_mpole_yt_surface", 22.0 * (double)Y.nias + 12.0 * (double)Y.nent);
   }
   copy_to_host(ctx, mpole_out, d_out, 8 * (size_t)nattr * nlm);

@@ -552,12 +552,19 @@ struct YtFields {
 
 // Surface points' weighted sums: acc[q*nattr + a], q = 0 the weights;
 // binned per workgroup in LDS, one global addition per bin and workgroup.
+// With direct (more bins than the LDS holds) every entry adds straight
+// into acc; there are a few entries per surface point, spread over nattr
+// basins.
 __global__ void __launch_bounds__(YB) k_yt_surface_sums(uint32_t nias, const uint32_t *__restrict__ ias, WBuf W,
-                                                         int nattr, int nf, YtFields F, double *__restrict__ acc) {
-  extern __shared__ double bins[];  // (nf + 1) * nattr
+                                                         int nattr, int nf, YtFields F, double *__restrict__ acc,
+                                                         bool direct) {
+  extern __shared__ double lbins[];  // (nf + 1) * nattr
   const int nb = (nf + 1) * nattr;
-  for (int q = threadIdx.x; q < nb; q += YB) bins[q] = 0.0;
-  __syncthreads();
+  double *bins = direct ? acc : lbins;
+  if (!direct) {
+    for (int q = threadIdx.x; q < nb; q += YB) bins[q] = 0.0;
+    __syncthreads();
+  }
   for (uint32_t t = blockIdx.x * YB + threadIdx.x; t < nias; t += gridDim.x * YB) {
     const uint32_t p = ias[t];
     const int ce = W.c[t];
@@ -572,6 +579,7 @@ __global__ void __launch_bounds__(YB) k_yt_surface_sums(uint32_t nias, const uin
       for (int q = 0; q < nf; q++) atomicAdd(&bins[(q + 1) * nattr + a], w * fv[q]);
     }
   }
+  if (direct) return;
   __syncthreads();
   for (int q = threadIdx.x; q < nb; q += YB)
     if (bins[q] != 0.0) atomicAdd(&acc[q], bins[q]);
@@ -855,12 +863,13 @@ void yt_run(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[
   basin_sums_run(ctx, lab, N, nattr, nfields, F.p, C2GPU_PTR_DEVICE, nfields ? isums.data() : nullptr,
                  counts.data());
   const size_t nb = (size_t)(nfields + 1) * nattr;
-  if (8 * nb > 60000) fail(C2GPU_ERR_UNSUPPORTED, "c2gpu_yt: too many attractors x fields for the LDS sums");
+  const bool direct = 8 * nb > 60000;  // bins past the LDS budget: add into acc itself
   double *acc = buf<double>(ctx, "yt_acc", nb);
   C2_HIP(hipMemsetAsync(acc, 0, 8 * nb, st));
   if (nias) {
     const unsigned gs = std::min<unsigned>(grid_for(nias, YB), (unsigned)persistent_threads(ctx->device, YB) / YB);
-    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(gs), dim3(YB), 8 * nb, nias, ias, W, nattr, nfields, F, acc);
+    C2_LAUNCH(ctx, k_yt_surface_sums, dim3(gs), dim3(YB), direct ? 0 : 8 * nb, nias, ias, W, nattr, nfields, F,
+              acc, direct);
     add_kernel_bytes(ctx, "k_yt_surface_sums", (double)nias * (14.0 + 8.0 * nfields) +
                                                    12.0 * (double)read_scalar(ctx, W.top));
   }

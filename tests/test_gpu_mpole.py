@@ -105,3 +105,62 @@ def test_yt_multipoles_need_a_yt_call(ctx):
                              r["xattr"], 2)
     finally:
         fresh.close()
+
+
+@pytest.mark.parametrize("nattr,lmax,device", [
+    (600, 5, False),    # 600 * 36 bins: 169 KiB, more than the LDS holds
+    (130, 10, True),    # 130 * 121 bins: 123 KiB, just past it
+    (106, 10, False),   # 100 KiB: the LDS bins, for the same labels
+])
+def test_multipoles_many_basins(ctx, nattr, lmax, device):
+    """More basin bins than fit in the LDS: each workgroup accumulates into
+    a row of global memory. Labels are 4^3 bricks dealt to nattr basins
+    (some basins get no point), attractors at random positions."""
+    rng = np.random.default_rng(nattr + lmax)
+    n = (40, 36, 44)
+    x2c = np.array([[9.0, 0.4, -0.3], [0.0, 8.5, 0.6], [0.0, 0.0, 9.5]])
+    i, j, k = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    brick = (i // 4) + 10 * (j // 4) + 100 * (k // 4)
+    deal = rng.integers(1, nattr + 1, size=int(brick.max()) + 1)
+    idg = deal[brick].transpose(2, 1, 0).reshape(-1).astype(np.int32)   # x fastest
+    f = rng.random(idg.size) + 0.1
+    xa = rng.random((nattr, 3))
+    print(f"[case] multipoles bricks nattr={nattr} lmax={lmax} device={device}", flush=True)
+    ref = oracle.multipoles(idg, f, n, x2c, xa, lmax)
+    if device:
+        got = gb.multipoles(ctx, ctx.to_device(idg), ctx.to_device(f), n, x2c, xa, lmax)
+    else:
+        got = gb.multipoles(ctx, idg, f, n, x2c, xa, lmax)
+    assert got.shape == ref.shape
+    for l in range(lmax + 1):
+        s = slice(l * l, (l + 1) * (l + 1))
+        scale = np.maximum(np.max(np.abs(ref[:, s]), axis=1, keepdims=True), 1e-300)
+        err = np.abs(got[:, s] - ref[:, s]) / scale
+        assert err.max() < 1e-10, (l, err.max())
+    empty = np.setdiff1d(np.arange(1, nattr + 1), idg)
+    assert np.all(got[empty - 1] == 0.0)
+
+
+def test_yt_multipoles_many_basins(ctx):
+    """YT multipoles of 140 basins at lmax 10 (138 KiB of bins, past the
+    LDS): interior and surface kernels both accumulate in global rows."""
+    from critic2_amd import _lib
+    from critic2_amd import yt as gy
+    cell = random_cell(91, (13.0, 13.5, 14.0), (85, 95, 100), 140, (1, 6, 7, 8), (56, 56, 60))
+    print(f"[case] yt multipoles seed=91 n={cell.n} atoms=140 lmax=10", flush=True)
+    f = gb.synth_slater(ctx, cell)
+    vec, area = _lib.ws_facets(np.asarray(cell.x2c) / np.array(cell.n)[None, :])
+    idg, nattr, xattr, vol, sums, w = oracle.yt(f, cell.n, cell.x2c, cell.xat, vec, area, fields=[f],
+                                                weights=True)
+    assert nattr * 121 * 8 > 120 * 1024
+    xa = np.asarray(xattr).reshape(-1, 3)[:nattr]
+    ref = oracle.yt_multipoles(w, f, cell.n, cell.x2c, xa, 10)
+    got_yt = gy.yt_integrate(ctx, f, cell.n, cell.x2c, cell.xat, fields=[f])
+    assert got_yt["nattr"] == nattr
+    got = gy.yt_multipoles(ctx, got_yt["labels"], f, cell.n, cell.x2c, got_yt["xattr"], 10)
+    for l in range(11):
+        s = slice(l * l, (l + 1) * (l + 1))
+        scale = np.maximum(np.max(np.abs(ref[:, s]), axis=1, keepdims=True), 1e-300)
+        err = np.abs(got[:, s] - ref[:, s]) / scale
+        assert err.max() < 1e-10, (l, err.max())
+    np.testing.assert_allclose(got[:, 0], got_yt["sums"][0], rtol=1e-12)

@@ -66,6 +66,32 @@ def test_yt_many_basins_meet_wide_path(ctx):
     assert got["stats"]["wide"] > 0
 
 
+def test_yt_thousands_of_basins(ctx):
+    """Smoothed noise with a tiny ratom: 1858 maxima, each its own basin,
+    and four fields, so the surface bins (5 x 1858 doubles) pass the LDS
+    budget and add into the global accumulator directly."""
+    rng = np.random.default_rng(5)
+    n = (36, 36, 36)
+    x2c = np.diag([12.0, 12.0, 12.0])
+    g = rng.random(n)
+    for ax in range(3):
+        g = (g + np.roll(g, 1, ax) + np.roll(g, -1, ax)) / 3
+    f = np.exp(3 * g).transpose(2, 1, 0).reshape(-1)
+    fl = [f, f * f, np.sqrt(f), np.ones_like(f)]
+    xat = np.zeros((1, 3))
+    print(f"[case] yt noise n={n} ratom=0.01 fields=4", flush=True)
+    vec, area = _lib.ws_facets(x2c / np.array(n)[None, :])
+    idg, nattr, xattr, vol, sums = oracle.yt(f, n, x2c, xat, vec, area, fields=fl, mode=_lib.ATTR_RATOM,
+                                             ratom=0.01)
+    assert 8 * 5 * nattr > 60000
+    got = gy.yt_integrate(ctx, f, n, x2c, xat, fields=fl, ratom=0.01)
+    assert got["nattr"] == nattr
+    np.testing.assert_array_equal(got["xattr"], xattr)
+    assert int(np.count_nonzero(got["labels"] != idg)) == 0
+    np.testing.assert_allclose(got["vol"], vol, rtol=1e-10)
+    np.testing.assert_allclose(got["sums"], sums, rtol=1e-10)
+
+
 @pytest.mark.parametrize("digits,seed,n,ranked", [(3, 65, (24, 26, 28), 1), (6, 67, (32, 34, 36), None),
                                                    (2, 68, (20, 22, 24), 1)])
 def test_yt_equal_values_follow_the_reference_sort(ctx, digits, seed, n, ranked):
