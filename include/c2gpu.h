@@ -450,8 +450,9 @@ typedef struct {
  * points, the higher neighbours of one point, or maxima), their order is the
  * one critic2's qcksort gives them (tools@proc.f90:83-168): the points are
  * then ranked by a restatement of that quicksort on the host (seconds for
- * 16 M points) and the ranks decide every comparison.  rho, fields and
- * label_out are device memory with C2GPU_PTR_DEVICE. */
+ * 16 M points) and the ranks decide every comparison.  At most 13653
+ * attractors (C2GPU_ERR_UNSUPPORTED beyond).  rho, fields and label_out are
+ * device memory with C2GPU_PTR_DEVICE. */
 int c2gpu_yt(c2gpu_ctx *ctx, const double *rho, const int n[3], const double x2c[9],
              const c2gpu_attractor_opts *opts, int nfields, const double *const *fields, int flags,
              int32_t *label_out, int *nattr_out, double *xattr_out, int xattr_cap, double *vol_out,
