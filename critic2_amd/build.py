@@ -98,10 +98,10 @@ def build_fortran(verbose: bool = False) -> str | None:
     steps = [
         [fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "c2gpu_mod.f90"), "-o",
          os.path.join(objdir, "c2gpu_mod.o"), "-module-dir", objdir],
-        [fc, "-O2", "-I", objdir, os.path.join(FORTRAN_DIR, "c2gpu_host.f90"),
+        [fc, "-O2", "-I", objdir, "-module-dir", objdir, os.path.join(FORTRAN_DIR, "c2gpu_host.f90"),
          os.path.join(objdir, "c2gpu_mod.o"), "-o", FORTRAN_HOST, "-L", HERE, "-lc2gpu",
          "-Wl,-rpath,$ORIGIN/.."],
-        [fc, "-O2", "-I", objdir, os.path.join(FORTRAN_DIR, "c2gpu_nci_host.f90"),
+        [fc, "-O2", "-I", objdir, "-module-dir", objdir, os.path.join(FORTRAN_DIR, "c2gpu_nci_host.f90"),
          os.path.join(objdir, "c2gpu_mod.o"), "-o", FORTRAN_NCI_HOST, "-L", HERE, "-lc2gpu",
          "-Wl,-rpath,$ORIGIN/.."],
     ]
