@@ -561,18 +561,71 @@ __global__ void k_huge_copyback(const uint32_t *__restrict__ hugelist, Lists L, 
 
 // ---------------------------------------------------------------- known == 2
 
-// Time at which q enters known == 2 (assign_surrounding_pts and
-// known_volnum_ongrid, bader@proc.f90:665-726).
-__device__ uint64_t kentry_point(const BParams &P, uint32_t q, const Lists &L, const uint8_t *__restrict__ uni) {
+// The six face neighbours of q (periodic).
+__device__ __forceinline__ void face_neighbours(const BParams &P, uint32_t q, uint32_t nb[6]) {
   int i, j, k;
   coords(P, q, i, j, k);
-  uint32_t nb[6];
   nb[0] = lin3(P, wrap1(i + 1, P.n1), j, k);
   nb[1] = lin3(P, wrap1(i - 1, P.n1), j, k);
   nb[2] = lin3(P, i, wrap1(j + 1, P.n2), k);
   nb[3] = lin3(P, i, wrap1(j - 1, P.n2), k);
   nb[4] = lin3(P, i, j, wrap1(k + 1, P.n3));
   nb[5] = lin3(P, i, j, wrap1(k - 1, P.n3));
+}
+
+// Candidate events above which a point's merge goes to a wavefront
+// (k_kentry_long) instead of one thread.
+constexpr uint64_t KT_LONG = 256;
+
+// The merge of the general case over the candidates with times in
+// [tlo, thi): the six neighbour lists' events in time order, the labels of
+// the seven points followed with forward-only cursors; the first time at
+// which all seven labels agree (and are > 0), or KT_INF.  At most `steps`
+// candidates are taken per call; cpos/vpos carry the state between calls
+// and *more says whether candidates remain.
+__device__ __forceinline__ uint64_t kt_merge(const uint64_t *__restrict__ ev, uint64_t cpos[6],
+                                             const uint64_t cend[6], uint64_t vpos[7], const uint64_t vbeg[7],
+                                             const uint64_t vend[7], int steps, bool *more) {
+  for (int it = 0; it < steps; it++) {
+    uint64_t c = KT_INF;
+    int tc = -1;
+    for (int t = 0; t < 6; t++)
+      if (cpos[t] < cend[t]) {
+        const uint64_t tt = ev_time(ev[cpos[t]]);
+        if (tt < c) {
+          c = tt;
+          tc = t;
+        }
+      }
+    if (tc < 0) {
+      *more = false;
+      return KT_INF;
+    }
+    cpos[tc]++;
+    int32_t lab[7];
+    for (int u = 0; u < 7; u++) {
+      while (vpos[u] < vend[u] && ev_time(ev[vpos[u]]) <= c) vpos[u]++;
+      lab[u] = (vpos[u] > vbeg[u]) ? ev_lab(ev[vpos[u] - 1]) : 0;
+    }
+    bool ok = lab[0] > 0;
+    for (int u = 1; u < 7 && ok; u++) ok = (lab[u] == lab[0]);
+    if (ok) {
+      *more = false;
+      return c;
+    }
+  }
+  *more = true;
+  return KT_INF;
+}
+
+// Time at which q enters known == 2 (assign_surrounding_pts and
+// known_volnum_ongrid, bader@proc.f90:665-726).  A general-case merge of
+// more than KT_LONG candidates is not done here: *defer is set instead.
+__device__ uint64_t kentry_point(const BParams &P, uint32_t q, const Lists &L, const uint8_t *__restrict__ uni,
+                                 bool *defer) {
+  *defer = false;
+  uint32_t nb[6];
+  face_neighbours(P, q, nb);
   const uint64_t *ev = L.ev;
   // all seven points must carry a label
   if (L.cnt[q] == 0) return KT_INF;
@@ -601,12 +654,16 @@ __device__ uint64_t kentry_point(const BParams &P, uint32_t q, const Lists &L, c
     }
     return kt;
   }
-  // general case: merge the six neighbour lists in time order; the labels of
-  // the seven points are followed with forward-only cursors
-  uint64_t cpos[6], cend[6];
+  // general case: merge the six neighbour lists in time order from tmin
+  uint64_t cpos[6], cend[6], total = 0;
   for (int t = 0; t < 6; t++) {
     cend[t] = L.e(nb[t]);
     cpos[t] = first_at_or_after(ev, L.b(nb[t]), cend[t], tmin);
+    total += cend[t] - cpos[t];
+  }
+  if (total > KT_LONG) {
+    *defer = true;
+    return KT_INF;
   }
   const uint32_t y7[7] = {q, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5]};
   uint64_t vpos[7], vend[7], vbeg[7];
@@ -615,40 +672,93 @@ __device__ uint64_t kentry_point(const BParams &P, uint32_t q, const Lists &L, c
     vend[u] = L.e(y7[u]);
     vpos[u] = first_at_or_after(ev, vbeg[u], vend[u], tmin);
   }
-  for (;;) {
-    uint64_t c = KT_INF;
-    int tc = -1;
-    for (int t = 0; t < 6; t++)
-      if (cpos[t] < cend[t]) {
-        const uint64_t tt = ev_time(ev[cpos[t]]);
-        if (tt < c) {
-          c = tt;
-          tc = t;
-        }
-      }
-    if (tc < 0) return KT_INF;
-    cpos[tc]++;
-    int32_t lab[7];
-    for (int u = 0; u < 7; u++) {
-      while (vpos[u] < vend[u] && ev_time(ev[vpos[u]]) <= c) vpos[u]++;
-      lab[u] = (vpos[u] > vbeg[u]) ? ev_lab(ev[vpos[u] - 1]) : 0;
-    }
-    bool ok = lab[0] > 0;
-    for (int u = 1; u < 7 && ok; u++) ok = (lab[u] == lab[0]);
-    if (ok) return c;
-  }
+  bool more;
+  return kt_merge(ev, cpos, cend, vpos, vbeg, vend, (int)KT_LONG + 1, &more);
 }
 
 // Kt for the points in `list` (every point when null); changed entries are
-// flagged in ktchg when ktchg != null.
+// flagged in ktchg when ktchg != null.  Points with long merges are queued
+// in longq for k_kentry_long.
 __global__ void k_kentry(BParams P, uint32_t n, const uint32_t *__restrict__ list, Lists L,
-                         const uint8_t *__restrict__ uni, uint64_t *__restrict__ Kt, uint8_t *__restrict__ ktchg) {
+                         const uint8_t *__restrict__ uni, uint64_t *__restrict__ Kt, uint8_t *__restrict__ ktchg,
+                         uint32_t *__restrict__ longq, uint32_t *nlong) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t q = list ? list[t] : t;
-  const uint64_t kt = kentry_point(P, q, L, uni);
-  if (ktchg && kt != Kt[q]) ktchg[q] = 1;
-  Kt[q] = kt;
+  bool defer;
+  const uint64_t kt = kentry_point(P, q, L, uni, &defer);
+  if (!defer) {
+    if (ktchg && kt != Kt[q]) ktchg[q] = 1;
+    Kt[q] = kt;
+  }
+  wave_append(defer, q, longq, nlong);
+}
+
+// The long merges, one wavefront per point.  The candidates (events of the
+// six neighbour lists from tmin on) are split by time into 64 consecutive
+// ranges, their bounds sampled evenly from the longest neighbour list; lane
+// i merges range i from cursors placed by binary search, which is the state
+// the serial merge has on reaching that range (a cursor only depends on the
+// time reached).  The ranges increase with the lane, so the lowest lane that
+// finds an agreement holds the first one; lanes above it stop early.
+__global__ void __launch_bounds__(256) k_kentry_long(BParams P, const uint32_t *__restrict__ longq,
+                                                     const uint32_t *nlong, Lists L, uint64_t *__restrict__ Kt,
+                                                     uint8_t *__restrict__ ktchg) {
+  const uint32_t nl = *nlong;
+  const int lane = threadIdx.x & 63;
+  const uint32_t wpb = blockDim.x / 64;
+  const uint64_t *ev = L.ev;
+  for (uint32_t w = blockIdx.x * wpb + threadIdx.x / 64; w < nl; w += gridDim.x * wpb) {
+    const uint32_t q = longq[w];
+    uint32_t nb[6];
+    face_neighbours(P, q, nb);
+    // deferred points have labels at all seven points (counts > 0)
+    uint64_t tmin = ev_time(ev[L.b(q)]);
+    for (int t = 0; t < 6; t++) {
+      const uint64_t ft = ev_time(ev[L.b(nb[t])]);
+      tmin = ft > tmin ? ft : tmin;
+    }
+    uint64_t cpos[6], cend[6], lb[6];
+    uint64_t best = 0;
+    int tl = 0;
+    for (int t = 0; t < 6; t++) {
+      cend[t] = L.e(nb[t]);
+      lb[t] = first_at_or_after(ev, L.b(nb[t]), cend[t], tmin);
+      if (cend[t] - lb[t] > best) {
+        best = cend[t] - lb[t];
+        tl = t;
+      }
+    }
+    // this lane's range [tlo, thi): lane 0 from tmin, lane 63 to the end
+    const uint64_t tlo = lane == 0 ? tmin : ev_time(ev[lb[tl] + (best * (uint64_t)lane) / 64]);
+    const uint64_t thi = lane == 63 ? KT_INF : ev_time(ev[lb[tl] + (best * (uint64_t)(lane + 1)) / 64]);
+    for (int t = 0; t < 6; t++) {
+      cpos[t] = first_at_or_after(ev, lb[t], cend[t], tlo);
+      if (lane != 63) cend[t] = first_at_or_after(ev, cpos[t], cend[t], thi);
+    }
+    const uint32_t y7[7] = {q, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5]};
+    uint64_t vpos[7], vend[7], vbeg[7];
+    for (int u = 0; u < 7; u++) {
+      vbeg[u] = L.b(y7[u]);
+      vend[u] = L.e(y7[u]);
+      vpos[u] = first_at_or_after(ev, vbeg[u], vend[u], tlo);
+    }
+    uint64_t res = KT_INF;
+    bool more = true;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (;;) {
+      if (more) res = kt_merge(ev, cpos, cend, vpos, vbeg, vend, 32, &more);
+      const uint64_t found = __ballot(res != KT_INF);
+      if (found & below) more = false;  // an earlier range holds the answer
+      if (__ballot(more) == 0) break;
+    }
+    const uint64_t found = __ballot(res != KT_INF);
+    const uint64_t kt = found ? __shfl(res, __ffsll((unsigned long long)found) - 1, 64) : KT_INF;
+    if (lane == 0) {
+      if (ktchg && kt != Kt[q]) ktchg[q] = 1;
+      Kt[q] = kt;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- decisions
@@ -1097,7 +1207,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   uint32_t *ncnt = buf<uint32_t>(ctx, "sweep_ncnt", N);
   uint32_t *alloc = buf<uint32_t>(ctx, "sweep_alloc", (size_t)N + 1);
   uint64_t *aoff = buf<uint64_t>(ctx, "sweep_aoff", (size_t)N + 1);
-  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 6);  // nch, ndl, nql, nsl, long lists x2
+  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 7);  // nch, ndl, nql, nsl, long lists x3
   uint32_t *longq = buf<uint32_t>(ctx, "sweep_longq", N);
   const unsigned gLong = (unsigned)persistent_threads(ctx->device, 256) / 256 / 2;  // 4 workgroups per CU
   int *d_err = buf<int>(ctx, "sweep_err", 1);
@@ -1126,6 +1236,14 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     return c;
   };
 
+  // known==2 times of the points in list (every point when null), the long
+  // merges on wavefronts
+  auto kentry = [&](uint32_t n, const uint32_t *list, uint8_t *flags) {
+    C2_HIP(hipMemsetAsync(d_n + 6, 0, sizeof(uint32_t), st));
+    C2_LAUNCH(ctx, k_kentry, dim3(grid_for(n, B)), dim3(B), 0, P, n, list, L, uni, Kt, flags, longq, d_n + 6);
+    C2_LAUNCH(ctx, k_kentry_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 6, L, Kt, flags);
+  };
+
   // every list from the story: counts, reserved blocks, events, order,
   // known==2 times
   auto full_build = [&]() {
@@ -1149,8 +1267,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     C2_LAUNCH(ctx, k_ev_fill, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, toff, tp, ell, lam,
               (const uint16_t *)nullptr, L, cur, d_err, d_work + 4);
     sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8);
-    C2_LAUNCH(ctx, k_kentry, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, L, uni, Kt,
-              (uint8_t *)nullptr);
+    kentry(N, nullptr, nullptr);
     R.events_total += (int64_t)nev;
     n_full += 1;
     s_nev += (double)nev;
@@ -1267,7 +1384,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     C2_LAUNCH(ctx, k_mark_kt, dim3(gD), dim3(B), 0, P, nd, dl, qflag);
     const uint32_t nq = collect(qflag, 1, ql, d_n + 2);
     const unsigned gQ = grid_for(nq, B);
-    C2_LAUNCH(ctx, k_kentry, dim3(gQ), dim3(B), 0, P, nq, ql, L, uni, Kt, ktchg);
+    if (nq) kentry(nq, ql, ktchg);
     // 4. starts whose inputs changed decide again
     C2_LAUNCH(ctx, k_mark_decide, dim3(gQ), dim3(B), 0, P, nq, ql, L, dflag, ktchg, sflag, longq, d_n + 5,
               d_work + 2);
