@@ -111,6 +111,34 @@ __device__ __forceinline__ uint64_t first_at_or_after(const uint64_t *__restrict
   return lo;
 }
 
+// first_at_or_after over K ranges at once ([lo[k], hi[k]) -> pos[k]): the
+// searches advance in lockstep, so their loads overlap and the dependent
+// chain is that of the longest range alone.
+template <int K>
+__device__ __forceinline__ void first_at_or_after_k(const uint64_t *__restrict__ ev, const uint64_t *lo,
+                                                    const uint64_t *hi, uint64_t t, uint64_t *pos) {
+  uint64_t n[K];
+  for (int k = 0; k < K; k++) {
+    pos[k] = lo[k];
+    n[k] = hi[k] - lo[k];
+  }
+  for (;;) {
+    bool any = false;
+    for (int k = 0; k < K; k++)
+      if (n[k] > 0) {
+        const uint64_t half = n[k] >> 1;
+        if (ev_time(ev[pos[k] + half]) < t) {
+          pos[k] += half + 1;
+          n[k] -= half + 1;
+        } else {
+          n[k] = half;
+        }
+        any = true;
+      }
+    if (!any) break;
+  }
+}
+
 // Counters that many wavefronts add to are spread over CSLOTS slots (slot
 // q of a counter at dst[q * CSTRIDE]; the host sums them): one address
 // taking an atomic from every wavefront of a full-grid launch serialises
@@ -573,9 +601,39 @@ __device__ __forceinline__ void face_neighbours(const BParams &P, uint32_t q, ui
   nb[5] = lin3(P, i, j, wrap1(k - 1, P.n3));
 }
 
-// Candidate events above which a point's merge goes to a wavefront
-// (k_kentry_long) instead of one thread.
-constexpr uint64_t KT_LONG = 256;
+// Candidate events above which a point's merge is cut into work items
+// (k_kt_items) instead of running on one thread.
+constexpr uint64_t KT_LONG = 8;
+
+// First index at or after pos (in [pos, end)) whose event time exceeds c:
+// a gallop from pos, then a binary search (cursors that jump far over a
+// long list cost a logarithm, not the jump).
+__device__ __forceinline__ uint64_t advance_past(const uint64_t *__restrict__ ev, uint64_t pos, uint64_t end,
+                                                 uint64_t c) {
+  if (pos >= end || ev_time(ev[pos]) > c) return pos;
+  uint64_t lo = pos, step = 1;  // ev[lo] <= c
+  while (lo + step < end && ev_time(ev[lo + step]) <= c) {
+    lo += step;
+    step <<= 1;
+  }
+  const uint64_t hi = lo + step < end ? lo + step : end;
+  return first_at_or_after(ev, lo + 1, hi, c + 1);
+}
+
+// Whether the labels of the seven points agree (and are > 0) at time c:
+// each cursor moves to the first event after c, whose predecessor carries
+// the label at c (none: 0).
+__device__ __forceinline__ bool labels_agree(const uint64_t *__restrict__ ev, uint64_t c, uint64_t vpos[7],
+                                             const uint64_t vbeg[7], const uint64_t vend[7]) {
+  int32_t lab[7];
+  for (int u = 0; u < 7; u++) {
+    vpos[u] = advance_past(ev, vpos[u], vend[u], c);
+    lab[u] = (vpos[u] > vbeg[u]) ? ev_lab(ev[vpos[u] - 1]) : 0;
+  }
+  bool ok = lab[0] > 0;
+  for (int u = 1; u < 7 && ok; u++) ok = (lab[u] == lab[0]);
+  return ok;
+}
 
 // The merge of the general case over the candidates with times in
 // [tlo, thi): the six neighbour lists' events in time order, the labels of
@@ -602,14 +660,7 @@ __device__ __forceinline__ uint64_t kt_merge(const uint64_t *__restrict__ ev, ui
       return KT_INF;
     }
     cpos[tc]++;
-    int32_t lab[7];
-    for (int u = 0; u < 7; u++) {
-      while (vpos[u] < vend[u] && ev_time(ev[vpos[u]]) <= c) vpos[u]++;
-      lab[u] = (vpos[u] > vbeg[u]) ? ev_lab(ev[vpos[u] - 1]) : 0;
-    }
-    bool ok = lab[0] > 0;
-    for (int u = 1; u < 7 && ok; u++) ok = (lab[u] == lab[0]);
-    if (ok) {
+    if (labels_agree(ev, c, vpos, vbeg, vend)) {
       *more = false;
       return c;
     }
@@ -643,34 +694,38 @@ __device__ uint64_t kentry_point(const BParams &P, uint32_t q, const Lists &L, c
     if (l0 <= 0) return KT_INF;
     for (int t = 0; t < 6; t++)
       if (ev_lab(ev[L.b(nb[t])]) != l0) return KT_INF;
-    uint64_t kt = KT_INF;
+    uint64_t b6[6], e6[6], x6[6];
     for (int t = 0; t < 6; t++) {
-      const uint64_t e = L.e(nb[t]);
-      const uint64_t x = first_at_or_after(ev, L.b(nb[t]), e, tmin);
-      if (x < e) {
-        const uint64_t tt = ev_time(ev[x]);
+      b6[t] = L.b(nb[t]);
+      e6[t] = L.e(nb[t]);
+    }
+    first_at_or_after_k<6>(ev, b6, e6, tmin, x6);
+    uint64_t kt = KT_INF;
+    for (int t = 0; t < 6; t++)
+      if (x6[t] < e6[t]) {
+        const uint64_t tt = ev_time(ev[x6[t]]);
         kt = tt < kt ? tt : kt;
       }
-    }
     return kt;
   }
-  // general case: merge the six neighbour lists in time order from tmin
-  uint64_t cpos[6], cend[6], total = 0;
-  for (int t = 0; t < 6; t++) {
-    cend[t] = L.e(nb[t]);
-    cpos[t] = first_at_or_after(ev, L.b(nb[t]), cend[t], tmin);
-    total += cend[t] - cpos[t];
-  }
-  if (total > KT_LONG) {
-    *defer = true;
-    return KT_INF;
-  }
+  // general case: merge the six neighbour lists in time order from tmin;
+  // the label cursors of the seven points start where the candidates do
   const uint32_t y7[7] = {q, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5]};
   uint64_t vpos[7], vend[7], vbeg[7];
   for (int u = 0; u < 7; u++) {
     vbeg[u] = L.b(y7[u]);
     vend[u] = L.e(y7[u]);
-    vpos[u] = first_at_or_after(ev, vbeg[u], vend[u], tmin);
+  }
+  first_at_or_after_k<7>(ev, vbeg, vend, tmin, vpos);
+  uint64_t cpos[6], cend[6], total = 0;
+  for (int t = 0; t < 6; t++) {
+    cpos[t] = vpos[t + 1];
+    cend[t] = vend[t + 1];
+    total += cend[t] - cpos[t];
+  }
+  if (total > KT_LONG) {
+    *defer = true;
+    return KT_INF;
   }
   bool more;
   return kt_merge(ev, cpos, cend, vpos, vbeg, vend, (int)KT_LONG + 1, &more);
@@ -678,7 +733,7 @@ __device__ uint64_t kentry_point(const BParams &P, uint32_t q, const Lists &L, c
 
 // Kt for the points in `list` (every point when null); changed entries are
 // flagged in ktchg when ktchg != null.  Points with long merges are queued
-// in longq for k_kentry_long.
+// in longq for k_kt_plan / k_kt_items / k_kt_commit.
 __global__ void k_kentry(BParams P, uint32_t n, const uint32_t *__restrict__ list, Lists L,
                          const uint8_t *__restrict__ uni, uint64_t *__restrict__ Kt, uint8_t *__restrict__ ktchg,
                          uint32_t *__restrict__ longq, uint32_t *nlong) {
@@ -694,71 +749,107 @@ __global__ void k_kentry(BParams P, uint32_t n, const uint32_t *__restrict__ lis
   wave_append(defer, q, longq, nlong);
 }
 
-// The long merges, one wavefront per point.  The candidates (events of the
-// six neighbour lists from tmin on) are split by time into 64 consecutive
-// ranges, their bounds sampled evenly from the longest neighbour list; lane
-// i merges range i from cursors placed by binary search, which is the state
-// the serial merge has on reaching that range (a cursor only depends on the
-// time reached).  The ranges increase with the lane, so the lowest lane that
-// finds an agreement holds the first one; lanes above it stop early.
-__global__ void __launch_bounds__(256) k_kentry_long(BParams P, const uint32_t *__restrict__ longq,
-                                                     const uint32_t *nlong, Lists L, uint64_t *__restrict__ Kt,
-                                                     uint8_t *__restrict__ ktchg) {
-  const uint32_t nl = *nlong;
-  const int lane = threadIdx.x & 63;
-  const uint32_t wpb = blockDim.x / 64;
+// The long merges.  Every candidate of such a point can be checked on its
+// own (labels_agree from cursors placed by binary search), so the
+// candidates are cut into work items of KT_CHUNK consecutive events of one
+// neighbour list; an item checks its candidates in time order and lowers
+// the point's time to its first agreement.  The minimum over the items is
+// the first agreement in the merged order, the serial merge's answer.
+
+constexpr uint64_t KT_CHUNK = 8;
+
+// Per deferred point w: its lists' first candidates (lb) and the prefix
+// of its item counts over the six lists (cpre, 7 entries); cnt[w] = items.
+__global__ void k_kt_plan(BParams P, uint32_t nl, const uint32_t *__restrict__ longq, Lists L,
+                          uint64_t *__restrict__ lbw, uint32_t *__restrict__ cpre, uint32_t *__restrict__ cnt,
+                          uint64_t *__restrict__ ktmp) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > nl) return;
+  if (w == nl) {  // scan sentinel
+    cnt[nl] = 0;
+    return;
+  }
+  const uint32_t q = longq[w];
+  uint32_t nb[6];
+  face_neighbours(P, q, nb);
   const uint64_t *ev = L.ev;
-  for (uint32_t w = blockIdx.x * wpb + threadIdx.x / 64; w < nl; w += gridDim.x * wpb) {
+  // deferred points have events at all seven points
+  uint64_t tmin = ev_time(ev[L.b(q)]);
+  for (int t = 0; t < 6; t++) {
+    const uint64_t ft = ev_time(ev[L.b(nb[t])]);
+    tmin = ft > tmin ? ft : tmin;
+  }
+  uint64_t b6[6], e6[6], lb[6];
+  for (int t = 0; t < 6; t++) {
+    b6[t] = L.b(nb[t]);
+    e6[t] = L.e(nb[t]);
+  }
+  first_at_or_after_k<6>(ev, b6, e6, tmin, lb);
+  uint32_t acc = 0;
+  for (int t = 0; t < 6; t++) {
+    lbw[6 * (size_t)w + t] = lb[t];
+    cpre[7 * (size_t)w + t] = acc;
+    acc += (uint32_t)((e6[t] - lb[t] + KT_CHUNK - 1) / KT_CHUNK);
+  }
+  cpre[7 * (size_t)w + 6] = acc;
+  cnt[w] = acc;
+  ktmp[w] = KT_INF;
+}
+
+__global__ void __launch_bounds__(256) k_kt_items(BParams P, uint32_t nl, const uint32_t *__restrict__ longq,
+                                                  Lists L, const uint64_t *__restrict__ lbw,
+                                                  const uint32_t *__restrict__ cpre,
+                                                  const uint64_t *__restrict__ off, uint64_t *__restrict__ ktmp) {
+  const uint64_t total = off[nl];
+  const uint64_t *ev = L.ev;
+  for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total;
+       x += (uint64_t)gridDim.x * blockDim.x) {
+    // the point: the last w with off[w] <= x
+    uint32_t lo = 0, hi = nl;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (off[mid] <= x)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t w = lo;
+    const uint32_t r = (uint32_t)(x - off[w]);
+    int t = 0;
+    while (t < 5 && cpre[7 * (size_t)w + t + 1] <= r) t++;
     const uint32_t q = longq[w];
     uint32_t nb[6];
     face_neighbours(P, q, nb);
-    // deferred points have labels at all seven points (counts > 0)
-    uint64_t tmin = ev_time(ev[L.b(q)]);
-    for (int t = 0; t < 6; t++) {
-      const uint64_t ft = ev_time(ev[L.b(nb[t])]);
-      tmin = ft > tmin ? ft : tmin;
-    }
-    uint64_t cpos[6], cend[6], lb[6];
-    uint64_t best = 0;
-    int tl = 0;
-    for (int t = 0; t < 6; t++) {
-      cend[t] = L.e(nb[t]);
-      lb[t] = first_at_or_after(ev, L.b(nb[t]), cend[t], tmin);
-      if (cend[t] - lb[t] > best) {
-        best = cend[t] - lb[t];
-        tl = t;
-      }
-    }
-    // this lane's range [tlo, thi): lane 0 from tmin, lane 63 to the end
-    const uint64_t tlo = lane == 0 ? tmin : ev_time(ev[lb[tl] + (best * (uint64_t)lane) / 64]);
-    const uint64_t thi = lane == 63 ? KT_INF : ev_time(ev[lb[tl] + (best * (uint64_t)(lane + 1)) / 64]);
-    for (int t = 0; t < 6; t++) {
-      cpos[t] = first_at_or_after(ev, lb[t], cend[t], tlo);
-      if (lane != 63) cend[t] = first_at_or_after(ev, cpos[t], cend[t], thi);
-    }
+    const uint64_t c0 = lbw[6 * (size_t)w + t] + (uint64_t)(r - cpre[7 * (size_t)w + t]) * KT_CHUNK;
+    const uint64_t le = L.e(nb[t]);
+    const uint64_t c1 = c0 + KT_CHUNK < le ? c0 + KT_CHUNK : le;
+    const uint64_t tfirst = ev_time(ev[c0]);
+    if (__hip_atomic_load(&ktmp[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tfirst) continue;
     const uint32_t y7[7] = {q, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5]};
     uint64_t vpos[7], vend[7], vbeg[7];
     for (int u = 0; u < 7; u++) {
       vbeg[u] = L.b(y7[u]);
       vend[u] = L.e(y7[u]);
-      vpos[u] = first_at_or_after(ev, vbeg[u], vend[u], tlo);
     }
-    uint64_t res = KT_INF;
-    bool more = true;
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (;;) {
-      if (more) res = kt_merge(ev, cpos, cend, vpos, vbeg, vend, 32, &more);
-      const uint64_t found = __ballot(res != KT_INF);
-      if (found & below) more = false;  // an earlier range holds the answer
-      if (__ballot(more) == 0) break;
-    }
-    const uint64_t found = __ballot(res != KT_INF);
-    const uint64_t kt = found ? __shfl(res, __ffsll((unsigned long long)found) - 1, 64) : KT_INF;
-    if (lane == 0) {
-      if (ktchg && kt != Kt[q]) ktchg[q] = 1;
-      Kt[q] = kt;
+    first_at_or_after_k<7>(ev, vbeg, vend, tfirst, vpos);
+    for (uint64_t i = c0; i < c1; i++) {
+      const uint64_t c = ev_time(ev[i]);
+      if (labels_agree(ev, c, vpos, vbeg, vend)) {
+        atomicMin((unsigned long long *)&ktmp[w], (unsigned long long)c);
+        break;
+      }
     }
   }
+}
+
+__global__ void k_kt_commit(uint32_t nl, const uint32_t *__restrict__ longq, const uint64_t *__restrict__ ktmp,
+                            uint64_t *__restrict__ Kt, uint8_t *__restrict__ ktchg) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nl) return;
+  const uint32_t q = longq[w];
+  const uint64_t kt = ktmp[w];
+  if (ktchg && kt != Kt[q]) ktchg[q] = 1;
+  Kt[q] = kt;
 }
 
 // ---------------------------------------------------------------- decisions
@@ -907,16 +998,28 @@ __global__ void __launch_bounds__(256) k_patch_keep(BParams P, uint32_t nd, cons
     const uint32_t lim = nn <= room ? nn : room;  // room >= nn by construction
     if (nn > room) atomicOr(err, 4);
     uint32_t kept = 0;
-    for (uint64_t x = ob; x < oe; x++) {
-      const uint64_t v = L.ev[x];
-      const uint32_t st = event_start(P, v);
-      if (!chg[st] || ev_pos(v) < from[st]) {
-        if (kept < lim)
-          L.ev[nb + kept] = v;
-        else
-          atomicOr(err, 4);
-        kept++;
+    // 8 events and their starts' flags are read before any of them is
+    // written: the writes (in place when the list does not move) never
+    // reach past the events already read
+    for (uint64_t x0 = ob; x0 < oe; x0 += 8) {
+      uint64_t v[8];
+      bool keep[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = x0 + u < oe ? L.ev[x0 + u] : 0;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const uint32_t st = event_start(P, v[u]);
+        keep[u] = x0 + u < oe && (!chg[st] || ev_pos(v[u]) < from[st]);
       }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (keep[u]) {
+          if (kept < lim)
+            L.ev[nb + kept] = v[u];
+          else
+            atomicOr(err, 4);
+          kept++;
+        }
     }
     cur[y] = kept;
     L.off[y] = nb;
@@ -1044,8 +1147,17 @@ __global__ void __launch_bounds__(256) k_mark_decide(BParams P, uint32_t nq, con
     sflag[x] = 1;
     marked = 1;
     if (!lng) {
+      // events loaded 8 at a time ahead of their flag stores (which the
+      // compiler must otherwise order after each load)
       const uint64_t b = L.b(x), e = L.e(x);
-      for (uint64_t q = b; q < e; q++) sflag[event_start(P, L.ev[q])] = 1;
+      for (uint64_t q0 = b; q0 < e; q0 += 8) {
+        uint64_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = q0 + u < e ? L.ev[q0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (q0 + u < e) sflag[event_start(P, v[u])] = 1;
+      }
       scanned = e - b;
     }
   }
@@ -1241,7 +1353,18 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   auto kentry = [&](uint32_t n, const uint32_t *list, uint8_t *flags) {
     C2_HIP(hipMemsetAsync(d_n + 6, 0, sizeof(uint32_t), st));
     C2_LAUNCH(ctx, k_kentry, dim3(grid_for(n, B)), dim3(B), 0, P, n, list, L, uni, Kt, flags, longq, d_n + 6);
-    C2_LAUNCH(ctx, k_kentry_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 6, L, Kt, flags);
+    const uint32_t nl = read_scalar(ctx, d_n + 6);
+    if (nl == 0) return;
+    uint64_t *lbw = buf<uint64_t>(ctx, "kt_lb", 6 * (size_t)nl);
+    uint32_t *cpre = buf<uint32_t>(ctx, "kt_cpre", 7 * (size_t)nl);
+    uint32_t *kcnt = buf<uint32_t>(ctx, "kt_cnt", (size_t)nl + 1);
+    uint64_t *koff = buf<uint64_t>(ctx, "kt_off", (size_t)nl + 1);
+    uint64_t *ktmp = buf<uint64_t>(ctx, "kt_tmp", nl);
+    C2_LAUNCH(ctx, k_kt_plan, dim3(grid_for((int64_t)nl + 1, B)), dim3(B), 0, P, nl, longq, L, lbw, cpre, kcnt,
+              ktmp);
+    exclusive_scan(ctx, "kt", kcnt, koff, (int64_t)nl + 1);
+    C2_LAUNCH(ctx, k_kt_items, dim3(2 * gLong), dim3(B), 0, P, nl, longq, L, lbw, cpre, koff, ktmp);
+    C2_LAUNCH(ctx, k_kt_commit, dim3(grid_for(nl, B)), dim3(B), 0, nl, longq, ktmp, Kt, flags);
   };
 
   // every list from the story: counts, reserved blocks, events, order,
