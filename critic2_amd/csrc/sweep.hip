@@ -52,7 +52,8 @@ namespace c2 {
 static constexpr uint64_t KT_INF = ~0ull;
 static constexpr int POS_BITS = 11;     // position on a path: < 2048
 static constexpr int LAB_BITS = 20;     // label: < 2^20
-static constexpr int SORT_INLINE = 48;  // longer event lists are sorted by workgroups
+static constexpr int SORT_INLINE = 8;   // longer event lists are sorted by wavefronts
+static constexpr int SORT_WAVE = 64;    // longer still: by workgroups
 static constexpr int SORT_LDS = 4096;   // longer still: segmented radix sort
 
 __device__ __forceinline__ uint64_t ev_time(uint64_t e) { return e >> LAB_BITS; }
@@ -73,7 +74,7 @@ __device__ __forceinline__ uint32_t event_start(const BParams &P, uint64_t e) {
 
 // The per-point event lists.
 // Lists longer than this are filtered / scanned by a whole workgroup.
-constexpr uint32_t LIST_LONG = 512;
+constexpr uint32_t LIST_LONG = 64;
 
 struct Lists {
   uint64_t *off;
@@ -487,12 +488,14 @@ __device__ __forceinline__ bool list_uniform(const uint64_t *__restrict__ ev, ui
   return true;
 }
 
-// Sorts the short lists of the points in `list` (every point when null) in
-// place and records whether each carries a single label; longer lists are
-// queued for the workgroup and radix sorts.
+// Sorts the short lists (up to SORT_INLINE events) of the points in `list`
+// (every point when null) in registers and records whether each carries a
+// single label; longer lists are queued for the wavefront sort (from the
+// back of longlist, n - 1 - index), the workgroup sort (from its front) and
+// the radix sort.  A list is in one queue only, so the two ends never meet.
 __global__ void k_ev_sort(uint32_t n, const uint32_t *__restrict__ list, Lists L, uint8_t *__restrict__ uni,
-                          uint32_t *__restrict__ longlist, uint32_t *nlong, uint32_t *__restrict__ hugelist,
-                          uint32_t *nhuge, unsigned long long *work) {
+                          uint32_t *__restrict__ longlist, uint32_t *nlong, uint32_t *nwave,
+                          uint32_t *__restrict__ hugelist, uint32_t *nhuge, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t sorted = 0;
   if (t < n) {
@@ -501,24 +504,59 @@ __global__ void k_ev_sort(uint32_t n, const uint32_t *__restrict__ list, Lists L
     const uint64_t m = e - b;
     if (m > SORT_LDS) {
       hugelist[atomicAdd(nhuge, 1u)] = y;
-    } else if (m > SORT_INLINE) {
+    } else if (m > SORT_WAVE) {
       longlist[atomicAdd(nlong, 1u)] = y;
+    } else if (m > SORT_INLINE) {
+      longlist[n - 1 - atomicAdd(nwave, 1u)] = y;
     } else {
       uint64_t *ev = L.ev;
-      for (uint64_t i = b + 1; i < e; i++) {
-        const uint64_t v = ev[i];
-        uint64_t j = i;
-        while (j > b && ev[j - 1] > v) {
-          ev[j] = ev[j - 1];
-          j--;
+      uint64_t r[SORT_INLINE];
+#pragma unroll
+      for (int u = 0; u < SORT_INLINE; u++) r[u] = u < (int)m ? ev[b + u] : ~0ull;
+#pragma unroll
+      for (int i = 1; i < SORT_INLINE; i++)
+#pragma unroll
+        for (int j = i; j > 0; j--)
+          if (r[j - 1] > r[j]) {
+            const uint64_t x = r[j - 1];
+            r[j - 1] = r[j];
+            r[j] = x;
+          }
+      bool same = true;
+#pragma unroll
+      for (int u = 0; u < SORT_INLINE; u++)
+        if (u < (int)m) {
+          ev[b + u] = r[u];
+          same = same && ev_lab(r[u]) == ev_lab(r[0]);
         }
-        ev[j] = v;
-      }
-      uni[y] = list_uniform(ev, b, e);
+      uni[y] = same;
       sorted = m;
     }
   }
   wave_add(work, sorted);
+}
+
+// One wavefront per list of SORT_INLINE+1 .. SORT_WAVE events: a bitonic
+// sort across the lanes (one event each, padding sorts last).
+__global__ void __launch_bounds__(256) k_ev_sort_wave(const uint32_t *__restrict__ wavelist, uint32_t nw, Lists L,
+                                                      uint8_t *__restrict__ uni) {
+  const uint32_t w = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (w >= nw) return;  // the whole wavefront
+  const int lane = threadIdx.x & 63;
+  const uint32_t y = wavelist[w];
+  const uint64_t b = L.b(y);
+  const int m = (int)L.cnt[y];
+  uint64_t v = lane < m ? L.ev[b + lane] : ~0ull;
+  for (int k = 2; k <= 64; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t o = __shfl_xor(v, j, 64);
+      const bool up = (lane & k) == 0, low = (lane & j) == 0;
+      v = (low == up) ? (o < v ? o : v) : (o > v ? o : v);
+    }
+  if (lane < m) L.ev[b + lane] = v;
+  const int32_t l0 = __shfl(ev_lab(v), 0, 64);
+  const uint64_t diff = __ballot(lane < m && ev_lab(v) != l0);
+  if (lane == 0) uni[y] = diff == 0;
 }
 
 // One workgroup per long list: bitonic sort in LDS.
@@ -1222,13 +1260,15 @@ static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const L
                        uint32_t *longlist, uint32_t *hugelist, unsigned long long *work) {
   hipStream_t st = ctx->stream;
   if (n == 0) return;
-  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_nlist", 2);
-  C2_HIP(hipMemsetAsync(d_n, 0, 2 * sizeof(uint32_t), st));
-  C2_LAUNCH(ctx, k_ev_sort, dim3(grid_for(n, 256)), dim3(256), 0, n, list, L, uni, longlist, d_n, hugelist,
-            d_n + 1, work);
-  uint32_t nlh[2];
-  copy_to_host(ctx, nlh, d_n, 2 * sizeof(uint32_t));
+  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_nlist", 3);
+  C2_HIP(hipMemsetAsync(d_n, 0, 3 * sizeof(uint32_t), st));
+  C2_LAUNCH(ctx, k_ev_sort, dim3(grid_for(n, 256)), dim3(256), 0, n, list, L, uni, longlist, d_n, d_n + 2,
+            hugelist, d_n + 1, work);
+  uint32_t nlh[3];
+  copy_to_host(ctx, nlh, d_n, 3 * sizeof(uint32_t));
   C2_HIP(hipStreamSynchronize(st));
+  if (nlh[2])
+    C2_LAUNCH(ctx, k_ev_sort_wave, dim3((nlh[2] + 3) / 4), dim3(256), 0, longlist + (n - nlh[2]), nlh[2], L, uni);
   if (nlh[0]) C2_LAUNCH(ctx, k_ev_sort_long, dim3(nlh[0]), dim3(256), 0, longlist, L, uni);
   if (nlh[1]) {
     // very long lists: gathered in batches of about HUGE_BATCH events (a
