@@ -270,15 +270,16 @@ __device__ __forceinline__ uint32_t traj_at(const BParams &P, uint32_t s, const 
 
 __device__ __forceinline__ int stop_position(const BParams &P, uint32_t s, const uint32_t *__restrict__ tp,
                                              uint64_t o, int len, const uint64_t *__restrict__ Kt, uint64_t ts) {
-  for (int q0 = 1; q0 < len; q0 += PRE_CHUNK) {
-    uint32_t x[PRE_CHUNK];
+  constexpr int SC = 8;  // path points whose indices and Kt are loaded together
+  for (int q0 = 1; q0 < len; q0 += SC) {
+    uint32_t x[SC];
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
-    uint64_t kt[PRE_CHUNK];
+    for (int u = 0; u < SC; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
+    uint64_t kt[SC];
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : KT_INF;
+    for (int u = 0; u < SC; u++) kt[u] = q0 + u < len ? Kt[x[u]] : KT_INF;
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++)
+    for (int u = 0; u < SC; u++)
       if (kt[u] < ts) return q0 + u + 1;
   }
   return len;
@@ -464,17 +465,27 @@ __global__ void __launch_bounds__(256) k_ev_fill(BParams P, uint32_t n, const ui
     const uint64_t t0 = sweep_index(P, s) << POS_BITS;
     const int32_t lab = lam[s];
     const uint64_t o = toff[s];
-    for (int l0 = f; l0 < e; l0 += 4) {  // four slot reservations in flight at once
-      const int m = (e - l0) < 4 ? e - l0 : 4;
-      uint32_t y[4], slot[4];
-      for (int u = 0; u < 4; u++) y[u] = u < m ? tp[o + l0 + u] : 0u;
-      for (int u = 0; u < 4; u++) slot[u] = u < m ? atomicAdd(&cur[y[u]], 1u) : 0u;
-      for (int u = 0; u < m; u++) {
-        if (slot[u] < L.cnt[y[u]])
-          L.ev[L.off[y[u]] + slot[u]] = make_event(t0, l0 + u, lab);
-        else
-          atomicOr(err, 4);  // count/fill mismatch (never expected): refuse to write
+    constexpr int FB = 16;  // slot reservations in flight at once
+    for (int l0 = f; l0 < e; l0 += FB) {
+      const int m = (e - l0) < FB ? e - l0 : FB;
+      uint32_t y[FB], slot[FB], cnt[FB];
+      uint64_t off[FB];
+#pragma unroll
+      for (int u = 0; u < FB; u++) y[u] = u < m ? tp[o + l0 + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < FB; u++) {
+        slot[u] = u < m ? atomicAdd(&cur[y[u]], 1u) : 0u;
+        cnt[u] = u < m ? L.cnt[y[u]] : 0u;
+        off[u] = u < m ? L.off[y[u]] : 0u;
       }
+#pragma unroll
+      for (int u = 0; u < FB; u++)
+        if (u < m) {
+          if (slot[u] < cnt[u])
+            L.ev[off[u] + slot[u]] = make_event(t0, l0 + u, lab);
+          else
+            atomicOr(err, 4);  // count/fill mismatch (never expected): refuse to write
+        }
     }
   }
   wave_add(work, (uint64_t)(e > f ? e - f : 0));
@@ -973,11 +984,19 @@ __global__ void __launch_bounds__(256) k_delta(uint32_t nc, const uint32_t *__re
     const bool same = lam[s] == lam2[s];
     const int f = same ? mn : 0;
     from[s] = (uint16_t)f;
-    for (int l = f; l < m; l++) {
-      const uint32_t y = tp[o + l];
-      const int d = (l < en ? 1 : 0) - (l < eo ? 1 : 0);
-      if (d) atomicAdd(&delta[y], d);
-      dflag[y] = 1;
+    for (int l0 = f; l0 < m; l0 += 8) {  // 8 path points read ahead of their updates
+      uint32_t y[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) y[u] = l0 + u < m ? tp[o + l0 + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int l = l0 + u;
+        if (l < m) {
+          const int d = (l < en ? 1 : 0) - (l < eo ? 1 : 0);
+          if (d) atomicAdd(&delta[y[u]], d);
+          dflag[y[u]] = 1;
+        }
+      }
     }
     st[0] = same && mn > 0;           // changes that keep the label
     st[1] = same ? (uint64_t)mn : 0;  // unchanged prefix events
