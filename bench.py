@@ -176,6 +176,65 @@ def host_threads():
     return max(1, n)
 
 
+def nci_cpu_time(cell, tables, species, c0, shape, threads):
+    """The oracle's promolecular NCI point loop (OpenMP, schedule(dynamic)
+    as nci@proc.f90:543) over a block of `shape` grid nodes from node c0 of
+    `cell`, with every atom image that reaches the block: (nodes, seconds)."""
+    from oracle import oracle
+    n = cell.n
+    I, J, K = np.meshgrid(*[np.arange(c0[d], c0[d] + shape[d]) for d in range(3)], indexing="ij")
+    frac = np.stack([I / n[0], J / n[1], K / n[2]], -1).reshape(-1, 3)
+    pts = frac @ np.asarray(cell.x2c).T
+    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
+    rc = max(min(g.cutrad, g.rmax) for g in tables)
+    centre = pts.mean(0)
+    rad = np.sqrt(((pts - centre) ** 2).sum(1)).max() + rc
+    pos, sp = [], []
+    for a, x in enumerate(cell.xat):
+        for t in np.ndindex(3, 3, 3):
+            p = np.asarray(cell.x2c) @ (x + np.array(t) - 1)
+            if np.linalg.norm(p - centre) <= rad:
+                pos.append(p)
+                sp.append(species[a])
+    t0 = time.perf_counter()
+    oracle.promol_nci_batch(pts, np.array(pos), np.array(sp, dtype=np.int32), otabs, threads=threads)
+    return len(pts), time.perf_counter() - t0
+
+
+def cpu_baseline_batch(ctx, cells, tables, species, bader_points=3_000_000, nci_cells=8):
+    """The oracle on a bounded sample of the batch: Bader serially (as
+    bader@proc.f90:151-224) on the first cells up to bader_points points,
+    and the promolecular NCI point loop (OpenMP dynamic, every host thread)
+    on a 32x32x16 node block at the centre of each of the first nci_cells
+    cells; the per-point rates are applied to every point of the batch."""
+    from oracle import oracle
+    t_b, n_b, used = 0.0, 0, 0
+    for c in cells:
+        if n_b >= bader_points:
+            break
+        f = gb.synth_slater(ctx, c)
+        t0 = time.perf_counter()
+        oracle.bader(f, c.n, c.x2c, c.xat)
+        t_b += time.perf_counter() - t0
+        n_b += f.size
+        used += 1
+    threads = host_threads()
+    t_n, n_n = 0.0, 0
+    for q, c in enumerate(cells[:nci_cells]):
+        shape = tuple(min(b, v) for b, v in zip((32, 32, 16), c.n))
+        npts, dt_ = nci_cpu_time(c, tables, species[q], [v // 2 - b // 2 for v, b in zip(c.n, shape)], shape,
+                                 threads)
+        t_n += dt_
+        n_n += npts
+    tb, tn = t_b / n_b, t_n / n_n
+    return {"value": 1e-9 / (tb + tn), "unit": "Gpoints/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "sample": f"oracle Bader (serial sweep + refinement, 1 thread, as the reference) on the first {used} "
+                      f"cells ({n_b} points): {tb * 1e6:.3f} us/pt; promolecular NCI (OpenMP dynamic over nodes, "
+                      f"{threads} threads) on {n_n} nodes from the first {min(nci_cells, len(cells))} cells: "
+                      f"{tn * 1e6:.3f} us/pt; both rates applied to every point of the batch",
+            "bader_us_per_point": round(tb * 1e6, 4), "nci_us_per_point": round(tn * 1e6, 4)}
+
+
 def cpu_baseline(ctx, cell, tables, species, n_bader=256):
     """The oracle (C restatement of critic2's algorithm) on a bounded sample
     of the same workload, with the reference's parallel structure: the
@@ -193,31 +252,15 @@ def cpu_baseline(ctx, cell, tables, species, n_bader=256):
     t_bader = (time.perf_counter() - t0) / f.size
     del f
     n = cell.n
-    c0 = [v // 2 - 64 for v in n[:2]] + [n[2] // 2 - 16]
-    I, J, K = np.meshgrid(np.arange(c0[0], c0[0] + 128), np.arange(c0[1], c0[1] + 128),
-                          np.arange(c0[2], c0[2] + 32), indexing="ij")
-    frac = np.stack([I / n[0], J / n[1], K / n[2]], -1).reshape(-1, 3)
-    pts = frac @ np.asarray(cell.x2c).T
-    otabs = [oracle.Table(g.r, g.f, g.fp, g.fpp, g.cutrad) for g in tables]
-    rc = max(min(g.cutrad, g.rmax) for g in tables)
-    centre = pts.mean(0)
-    rad = np.sqrt(((pts - centre) ** 2).sum(1)).max() + rc
-    pos, sp = [], []
-    for a, x in enumerate(cell.xat):
-        for t in np.ndindex(3, 3, 3):
-            p = np.asarray(cell.x2c) @ (x + np.array(t) - 1)
-            if np.linalg.norm(p - centre) <= rad:
-                pos.append(p)
-                sp.append(species[a])
     threads = host_threads()
-    t0 = time.perf_counter()
-    oracle.promol_nci_batch(pts, np.array(pos), np.array(sp, dtype=np.int32), otabs, threads=threads)
-    t_nci = (time.perf_counter() - t0) / len(pts)
+    pts_n, t_nci = nci_cpu_time(cell, tables, species, [v // 2 - 64 for v in n[:2]] + [n[2] // 2 - 16],
+                                (128, 128, 32), threads)
+    t_nci /= pts_n
     return {"value": 1e-9 / (t_bader + t_nci), "unit": "Gpoints/s", "cores": threads, "kind": "port",
             "cpu": cpu_model(),
             "sample": f"oracle Bader (serial sweep + refinement, 1 thread, as the reference) on the same "
                       f"30 A / 216-atom cell at {n_bader}^3: {t_bader * 1e6:.3f} us/pt; promolecular NCI "
-                      f"(OpenMP dynamic over nodes, {threads} threads) on {len(pts)} nodes of the 512^3 grid: "
+                      f"(OpenMP dynamic over nodes, {threads} threads) on {pts_n} nodes of the 512^3 grid: "
                       f"{t_nci * 1e6:.3f} us/pt; both rates applied to the whole grid",
             "bader_us_per_point": round(t_bader * 1e6, 4), "nci_us_per_point": round(t_nci * 1e6, 4)}
 
@@ -267,6 +310,16 @@ def run_batch(ctx, args):
         print(f"[bench] step {s_ + 1}/{args.steps} done", file=sys.stderr, flush=True)
     ctx.sync()
     dt = (time.perf_counter() - t0) / args.steps
+    # no single kernel dominates the batch (the sweep's list kernels of every
+    # cell): the step as a whole against HBM, with its compulsory bytes
+    ach = STEP_BYTES_PER_POINT * points / dt / 1e9
+    roof = {"bound": "hbm", "kernel": "whole step (every kernel of every cell)", "achieved": round(ach, 3),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None,
+            "compulsory_bytes_per_point": STEP_BYTES_PER_POINT, "points_per_step": points}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(ctx, cells, tables, species)
+        print("[bench] cpu baseline done", file=sys.stderr, flush=True)
     out = {
         "metric": f"Gpoints/s Bader+NCI, batch of {len(cells)} cells (labels exact)",
         "value": round(points / dt / 1e9, 6),
@@ -283,8 +336,9 @@ def run_batch(ctx, args):
         "config": {"workload": f"{len(cells)} seeded dimer cells (6-40 atoms, gap 2.5-4.0 A), edges from "
                                "{48,64,96,128,160}; Bader labels + Volume/Pop + FP32 promolecular NCI",
                    "cells": len(cells), "points": points, "parallelism": "single GPU"},
-        "roofline": None,
-        "cpu_baseline": None,
+        "hbm_frac_compulsory": round(ach / HBM_PEAK_GBS, 6),
+        "roofline": roof,
+        "cpu_baseline": cpu,
         "batch": {"nstreams": st["nstreams"], "queue_workgroups": st["queue_workgroups"],
                   "ms_nci_queue": round(st["ms_nci"], 3)},
     }
