@@ -456,11 +456,19 @@ __global__ void __launch_bounds__(256) k_ev_fill(BParams P, uint32_t n, const ui
                                                  const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
                                                  const int32_t *__restrict__ ell, const int32_t *__restrict__ lam,
                                                  const uint16_t *__restrict__ from, Lists L,
-                                                 uint32_t *__restrict__ cur, int *err, unsigned long long *work) {
+                                                 uint32_t *__restrict__ cur, int *err, unsigned long long *work,
+                                                 int32_t *__restrict__ ell_to, int32_t *__restrict__ lam_to,
+                                                 uint8_t *__restrict__ chg) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = t < n ? (list ? list[t] : t) : 0u;
   const int e = t < n ? ell[s] : 0;
   const int f = (t < n && from) ? (int)from[s] : 0;  // events before f are already in place
+  if (t < n && ell_to) {
+    // k_apply's commit of the new decision, for the starts of this launch
+    ell_to[s] = e;
+    lam_to[s] = lam[s];
+    chg[s] = 0;
+  }
   if (e > f) {
     const uint64_t t0 = sweep_index(P, s) << POS_BITS;
     const int32_t lab = lam[s];
@@ -809,13 +817,15 @@ constexpr uint64_t KT_CHUNK = 8;
 
 // Per deferred point w: its lists' first candidates (lb) and the prefix
 // of its item counts over the six lists (cpre, 7 entries); cnt[w] = items.
-__global__ void k_kt_plan(BParams P, uint32_t nl, const uint32_t *__restrict__ longq, Lists L,
-                          uint64_t *__restrict__ lbw, uint32_t *__restrict__ cpre, uint32_t *__restrict__ cnt,
-                          uint64_t *__restrict__ ktmp) {
+// nbound bounds the deferred count *nlp (read on the device); entries from
+// *nlp to nbound get no items.
+__global__ void k_kt_plan(BParams P, const uint32_t *nlp, uint32_t nbound, const uint32_t *__restrict__ longq,
+                          Lists L, uint64_t *__restrict__ lbw, uint32_t *__restrict__ cpre,
+                          uint32_t *__restrict__ cnt, uint64_t *__restrict__ ktmp) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w > nl) return;
-  if (w == nl) {  // scan sentinel
-    cnt[nl] = 0;
+  if (w > nbound) return;
+  if (w >= *nlp) {  // past the deferred points, and the scan sentinel
+    cnt[w] = 0;
     return;
   }
   const uint32_t q = longq[w];
@@ -849,6 +859,7 @@ __global__ void __launch_bounds__(256) k_kt_items(BParams P, uint32_t nl, const 
                                                   Lists L, const uint64_t *__restrict__ lbw,
                                                   const uint32_t *__restrict__ cpre,
                                                   const uint64_t *__restrict__ off, uint64_t *__restrict__ ktmp) {
+  // nl: the bound of k_kt_plan (points past the deferred ones own no items)
   const uint64_t total = off[nl];
   const uint64_t *ev = L.ev;
   for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total;
@@ -891,10 +902,10 @@ __global__ void __launch_bounds__(256) k_kt_items(BParams P, uint32_t nl, const 
   }
 }
 
-__global__ void k_kt_commit(uint32_t nl, const uint32_t *__restrict__ longq, const uint64_t *__restrict__ ktmp,
-                            uint64_t *__restrict__ Kt, uint8_t *__restrict__ ktchg) {
+__global__ void k_kt_commit(const uint32_t *nlp, uint32_t nb, const uint32_t *__restrict__ longq,
+                            const uint64_t *__restrict__ ktmp, uint64_t *__restrict__ Kt, uint8_t *__restrict__ ktchg) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nl) return;
+  if (w >= nb || w >= *nlp) return;
   const uint32_t q = longq[w];
   const uint64_t kt = ktmp[w];
   if (ktchg && kt != Kt[q]) ktchg[q] = 1;
@@ -1184,6 +1195,15 @@ __global__ void k_mark_kt(BParams P, uint32_t nd, const uint32_t *__restrict__ d
   qflag[lin3(P, i, j, wrap1(k - 1, P.n3))] = 1;
 }
 
+// The step's flags cleared through their lists: dflag over dl, ktchg over
+// ql.
+__global__ void k_clear_step(uint32_t nd, const uint32_t *__restrict__ dl, uint8_t *__restrict__ dflag, uint32_t nq,
+                             const uint32_t *__restrict__ ql, uint8_t *__restrict__ ktchg) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nd) dflag[dl[t]] = 0;
+  if (t < nq) ktchg[ql[t]] = 0;
+}
+
 // Starts to decide again: those with an event at a point whose list or
 // known==2 time changed, and those points themselves.  (A start's decision
 // reads volnum at the start and at its path's end and the known==2 marks
@@ -1275,12 +1295,11 @@ static unsigned long long sum_slots(c2gpu_ctx *ctx, const unsigned long long *d,
 constexpr uint64_t HUGE_BATCH = 1ull << 27;
 
 // Sorts the lists of the points in `list` (all points when null).
+// d_n: three counters (long, huge, wavefront lists), zero on entry.
 static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const Lists &L, uint8_t *uni,
-                       uint32_t *longlist, uint32_t *hugelist, unsigned long long *work) {
+                       uint32_t *longlist, uint32_t *hugelist, unsigned long long *work, uint32_t *d_n) {
   hipStream_t st = ctx->stream;
   if (n == 0) return;
-  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_nlist", 3);
-  C2_HIP(hipMemsetAsync(d_n, 0, 3 * sizeof(uint32_t), st));
   C2_LAUNCH(ctx, k_ev_sort, dim3(grid_for(n, 256)), dim3(256), 0, n, list, L, uni, longlist, d_n, d_n + 2,
             hugelist, d_n + 1, work);
   uint32_t nlh[3];
@@ -1378,13 +1397,17 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   uint32_t *ncnt = buf<uint32_t>(ctx, "sweep_ncnt", N);
   uint32_t *alloc = buf<uint32_t>(ctx, "sweep_alloc", (size_t)N + 1);
   uint64_t *aoff = buf<uint64_t>(ctx, "sweep_aoff", (size_t)N + 1);
-  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 7);  // nch, ndl, nql, nsl, long lists x3
+  // the step's counters: nch, ndl, nql, nsl, long patch / mark / known==2
+  // lists, sorted long / huge / wavefront lists (all zeroed by one memset
+  // per step), then the error bits, kept over the whole sweep
+  constexpr int NCOUNT = 10, CERR = 11;
+  uint32_t *d_n = buf<uint32_t>(ctx, "sweep_counts", 12);
   uint32_t *longq = buf<uint32_t>(ctx, "sweep_longq", N);
   const unsigned gLong = (unsigned)persistent_threads(ctx->device, 256) / 256 / 2;  // 4 workgroups per CU
-  int *d_err = buf<int>(ctx, "sweep_err", 1);
+  int *d_err = reinterpret_cast<int *>(d_n + CERR);
   unsigned long long *d_stats = buf<unsigned long long>(ctx, "sweep_stats", CSLOTS * CSTRIDE);
   unsigned long long *d_work = buf<unsigned long long>(ctx, "sweep_work", CSLOTS * CSTRIDE);  // algorithmic work counters
-  C2_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
+  C2_HIP(hipMemsetAsync(d_n, 0, 12 * sizeof(uint32_t), st));
   C2_HIP(hipMemsetAsync(d_work, 0, CSLOTS * CSTRIDE * sizeof(unsigned long long), st));
   // host-side totals for the algorithmic-bytes report
   double s_nc = 0, s_nd = 0, s_nq = 0, s_ns = 0, s_coll = 0, n_coll = 0, n_full = 0, n_decall = 0, s_nev = 0;
@@ -1398,8 +1421,8 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   L.ev = nullptr;
   uint64_t pool_cap = 0, pool_top = 0;
 
+  // d_cnt is zero on entry (the step's memset)
   auto collect = [&](uint8_t *flag, int clear, uint32_t *list, uint32_t *d_cnt) -> uint32_t {
-    C2_HIP(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st));
     C2_LAUNCH(ctx, k_collect, dim3(gCollect), dim3(B), 0, N, flag, clear, list, d_cnt);
     const uint32_t c = read_scalar(ctx, d_cnt);
     s_coll += c;
@@ -1409,21 +1432,26 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
 
   // known==2 times of the points in list (every point when null), the long
   // merges on wavefronts
+  // merges (work items).  Over every point the deferred count is read back
+  // to size the plan; within a step (list != null, counter zeroed by the
+  // step's memset) the plan is sized by n and reads the count on the device.
   auto kentry = [&](uint32_t n, const uint32_t *list, uint8_t *flags) {
-    C2_HIP(hipMemsetAsync(d_n + 6, 0, sizeof(uint32_t), st));
-    C2_LAUNCH(ctx, k_kentry, dim3(grid_for(n, B)), dim3(B), 0, P, n, list, L, uni, Kt, flags, longq, d_n + 6);
-    const uint32_t nl = read_scalar(ctx, d_n + 6);
-    if (nl == 0) return;
-    uint64_t *lbw = buf<uint64_t>(ctx, "kt_lb", 6 * (size_t)nl);
-    uint32_t *cpre = buf<uint32_t>(ctx, "kt_cpre", 7 * (size_t)nl);
-    uint32_t *kcnt = buf<uint32_t>(ctx, "kt_cnt", (size_t)nl + 1);
-    uint64_t *koff = buf<uint64_t>(ctx, "kt_off", (size_t)nl + 1);
-    uint64_t *ktmp = buf<uint64_t>(ctx, "kt_tmp", nl);
-    C2_LAUNCH(ctx, k_kt_plan, dim3(grid_for((int64_t)nl + 1, B)), dim3(B), 0, P, nl, longq, L, lbw, cpre, kcnt,
-              ktmp);
-    exclusive_scan(ctx, "kt", kcnt, koff, (int64_t)nl + 1);
-    C2_LAUNCH(ctx, k_kt_items, dim3(2 * gLong), dim3(B), 0, P, nl, longq, L, lbw, cpre, koff, ktmp);
-    C2_LAUNCH(ctx, k_kt_commit, dim3(grid_for(nl, B)), dim3(B), 0, nl, longq, ktmp, Kt, flags);
+    uint32_t *nlp = d_n + 6;
+    if (!list) C2_HIP(hipMemsetAsync(nlp, 0, sizeof(uint32_t), st));
+    C2_LAUNCH(ctx, k_kentry, dim3(grid_for(n, B)), dim3(B), 0, P, n, list, L, uni, Kt, flags, longq, nlp);
+    const uint32_t nb = list ? n : read_scalar(ctx, nlp);
+    if (nb == 0) return;
+    uint64_t *lbw = buf<uint64_t>(ctx, "kt_lb", 6 * (size_t)nb);
+    uint32_t *cpre = buf<uint32_t>(ctx, "kt_cpre", 7 * (size_t)nb);
+    uint32_t *kcnt = buf<uint32_t>(ctx, "kt_cnt", (size_t)nb + 1);
+    uint64_t *koff = buf<uint64_t>(ctx, "kt_off", (size_t)nb + 1);
+    uint64_t *ktmp = buf<uint64_t>(ctx, "kt_tmp", nb);
+    C2_LAUNCH(ctx, k_kt_plan, dim3(grid_for((int64_t)nb + 1, B)), dim3(B), 0, P, (const uint32_t *)nlp, nb, longq,
+              L, lbw, cpre, kcnt, ktmp);
+    exclusive_scan(ctx, "kt", kcnt, koff, (int64_t)nb + 1);
+    C2_LAUNCH(ctx, k_kt_items, dim3(2 * gLong), dim3(B), 0, P, nb, longq, L, lbw, cpre, koff, ktmp);
+    C2_LAUNCH(ctx, k_kt_commit, dim3(grid_for(nb, B)), dim3(B), 0, (const uint32_t *)nlp, nb, longq, ktmp, Kt,
+              flags);
   };
 
   // every list from the story: counts, reserved blocks, events, order,
@@ -1447,8 +1475,10 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     pool_top = reserved;
     C2_HIP(hipMemsetAsync(cur, 0, sizeof(uint32_t) * N, st));
     C2_LAUNCH(ctx, k_ev_fill, dim3(gN), dim3(B), 0, P, N, (const uint32_t *)nullptr, toff, tp, ell, lam,
-              (const uint16_t *)nullptr, L, cur, d_err, d_work + 4);
-    sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8);
+              (const uint16_t *)nullptr, L, cur, d_err, d_work + 4, (int32_t *)nullptr, (int32_t *)nullptr,
+              (uint8_t *)nullptr);
+    C2_HIP(hipMemsetAsync(d_n + 7, 0, 3 * sizeof(uint32_t), st));
+    sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8, d_n + 7);
     kentry(N, nullptr, nullptr);
     R.events_total += (int64_t)nev;
     n_full += 1;
@@ -1531,6 +1561,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       continue;
     }
     // 1. points whose lists change, and their new sizes
+    C2_HIP(hipMemsetAsync(d_n, 0, NCOUNT * sizeof(uint32_t), st));
     if (trace) C2_HIP(hipMemsetAsync(d_stats, 0, CSLOTS * CSTRIDE * sizeof(unsigned long long), st));
     C2_LAUNCH(ctx, k_delta, dim3(gC), dim3(B), 0, nc, chlist, toff, tp, ell, ell2, lam, lam2, delta, dflag, cfrom,
               trace ? d_stats : nullptr, d_work + 5);
@@ -1550,18 +1581,17 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       if (trace) std::fprintf(stderr, "[sweep] it %d full rebuild, changed %u\n", it, nc);
       continue;
     }
-    // 2. rewrite those lists; add the changed starts' events
-    C2_HIP(hipMemsetAsync(d_n + 4, 0, 2 * sizeof(uint32_t), st));
+    // 2. rewrite those lists; add the changed starts' events (and commit
+    // the new decisions, k_apply's work)
     C2_LAUNCH(ctx, k_patch_keep, dim3(gD), dim3(B), 0, P, nd, dl, ncnt, alloc, aoff, pool_top, L, chg, cfrom, cur,
               longq, d_n + 4, d_err, d_work + 0);
     C2_LAUNCH(ctx, k_patch_keep_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 4, dl, ncnt, alloc, aoff, pool_top,
               L, chg, cfrom, cur, d_err, d_work + 0);
     C2_LAUNCH(ctx, k_ev_fill, dim3(gC), dim3(B), 0, P, nc, chlist, toff, tp, ell2, lam2, cfrom, L, cur, d_err,
-              d_work + 4);
+              d_work + 4, ell, lam, chg);
     s_nd += nd;
-    C2_LAUNCH(ctx, k_apply, dim3(gC), dim3(B), 0, nc, chlist, ell, lam, ell2, lam2, chg);
     pool_top += added;
-    sort_lists(ctx, nd, dl, L, uni, longlist, hugelist, d_work + 8);
+    sort_lists(ctx, nd, dl, L, uni, longlist, hugelist, d_work + 8, d_n + 7);
     // 3. known==2 times around the rewritten lists
     C2_LAUNCH(ctx, k_mark_kt, dim3(gD), dim3(B), 0, P, nd, dl, qflag);
     const uint32_t nq = collect(qflag, 1, ql, d_n + 2);
@@ -1573,17 +1603,18 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
     C2_LAUNCH(ctx, k_mark_decide_long, dim3(gLong), dim3(B), 0, P, longq, d_n + 5, L, sflag, d_work + 2);
     s_nq += nq;
     const uint32_t ns = collect(sflag, 1, sl, d_n + 3);
-    C2_HIP(hipMemsetAsync(d_n, 0, sizeof(uint32_t), st));
     if (trace) C2_HIP(hipMemsetAsync(d_stats + 4, 0xff, sizeof(unsigned long long), st));
     if (ns)
       C2_LAUNCH(ctx, k_decide, dim3(grid_for(ns, B)), dim3(B), 0, P, ns, sl, toff, tp, L, Kt, maxlab, ell, lam,
                 ell2, lam2, chg, chlist, d_n, d_err, trace ? d_stats + 4 : nullptr, d_work + 7);
     s_ns += ns;
-    // 5. clear the step's flags through their lists
-    C2_LAUNCH(ctx, k_clear_list<uint8_t>, dim3(gD), dim3(B), 0, nd, dl, dflag);
-    C2_LAUNCH(ctx, k_clear_list<uint8_t>, dim3(gQ), dim3(B), 0, nq, ql, ktchg);
-    nc = read_scalar(ctx, d_n);
-    if (read_scalar(ctx, d_err) & 4) fail(C2GPU_ERR_ARG, "c2gpu_bader: internal inconsistency in the sweep event lists");
+    // 5. clear the step's flags through their lists; read the changes and
+    // the error bits together
+    C2_LAUNCH(ctx, k_clear_step, dim3(grid_for(nd > nq ? nd : nq, B)), dim3(B), 0, nd, dl, dflag, nq, ql, ktchg);
+    uint32_t hc[CERR + 1];
+    copy_to_host(ctx, hc, d_n, sizeof(hc));
+    nc = hc[0];
+    if ((int)hc[CERR] & 4) fail(C2GPU_ERR_ARG, "c2gpu_bader: internal inconsistency in the sweep event lists");
     R.events_total += (int64_t)added;
     if (trace) {
       unsigned long long h[5];
