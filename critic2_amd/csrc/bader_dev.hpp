@@ -206,16 +206,24 @@ __device__ inline bool step_near_candidate(const BParams &P, const int p[3], dou
 // trajectories).  Returns the path length, or -1 if it exceeds maxlen.
 // 256-bit membership filter of the points on the current path: a clear bit
 // proves absence, so the exact scan of the path runs only on filter hits.
+// The four words are separate members picked by selects: an array indexed
+// by the hash would live in scratch memory, one more memory access in the
+// dependent chain of every step.
 struct PathFilter {
-  uint64_t w[4] = {0, 0, 0, 0};
+  uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
   __device__ __forceinline__ static uint32_t bit(uint32_t l) { return (l * 0x9E3779B1u) >> 24; }
   __device__ __forceinline__ void add(uint32_t l) {
-    const uint32_t b = bit(l);
-    w[b >> 6] |= 1ull << (b & 63);
+    const uint32_t b = bit(l), k = b >> 6;
+    const uint64_t m = 1ull << (b & 63);
+    w0 |= k == 0 ? m : 0ull;
+    w1 |= k == 1 ? m : 0ull;
+    w2 |= k == 2 ? m : 0ull;
+    w3 |= k == 3 ? m : 0ull;
   }
   __device__ __forceinline__ bool maybe(uint32_t l) const {
-    const uint32_t b = bit(l);
-    return (w[b >> 6] >> (b & 63)) & 1ull;
+    const uint32_t b = bit(l), k = b >> 6;
+    const uint64_t x = k == 0 ? w0 : k == 1 ? w1 : k == 2 ? w2 : w3;
+    return (x >> (b & 63)) & 1ull;
   }
 };
 

@@ -881,7 +881,10 @@ __global__ void __launch_bounds__(256) k_kt_items(BParams P, uint32_t nl, const 
     uint32_t nb[6];
     face_neighbours(P, q, nb);
     const uint64_t c0 = lbw[6 * (size_t)w + t] + (uint64_t)(r - cpre[7 * (size_t)w + t]) * KT_CHUNK;
-    const uint64_t le = L.e(nb[t]);
+    uint32_t nbt = nb[0];  // nb[t] by selects (a dynamic index would put nb in scratch)
+#pragma unroll
+    for (int u = 1; u < 6; u++) nbt = u == t ? nb[u] : nbt;
+    const uint64_t le = L.e(nbt);
     const uint64_t c1 = c0 + KT_CHUNK < le ? c0 + KT_CHUNK : le;
     const uint64_t tfirst = ev_time(ev[c0]);
     if (__hip_atomic_load(&ktmp[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tfirst) continue;
