@@ -381,16 +381,21 @@ __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ 
   // PRE_CHUNK path points at a time: their indices and Kt are loaded
   // together (independent loads), then the stop is found among them
   for (int q0 = 1; q0 < len; q0 += PRE_CHUNK) {
-    uint32_t x[PRE_CHUNK], kt[PRE_CHUNK];
+    uint32_t x[PRE_CHUNK], kt[PRE_CHUNK], cv[PRE_CHUNK];
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
+    // Kt and the current covering times of the chunk loaded together; a
+    // stale (larger) covering time only costs an atomic that changes nothing
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : 0u;
+    for (int u = 0; u < PRE_CHUNK; u++) {
+      kt[u] = q0 + u < len ? Kt[x[u]] : 0u;
+      cv[u] = q0 + u < len ? __hip_atomic_load(&Cnew[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    }
     bool stop = false;
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) {
       if (stop || q0 + u >= len) break;
-      if (ts < __hip_atomic_load(&Cnew[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&Cnew[x[u]], ts);
+      if (ts < cv[u]) atomicMin(&Cnew[x[u]], ts);
       stop = kt[u] < ts;
     }
     if (stop) break;
@@ -1435,14 +1440,15 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
 
   // known==2 times of the points in list (every point when null), the long
   // merges on wavefronts
-  // merges (work items).  Over every point the deferred count is read back
-  // to size the plan; within a step (list != null, counter zeroed by the
-  // step's memset) the plan is sized by n and reads the count on the device.
+  // merges (work items).  The plan's buffers (96 bytes a point) are sized
+  // by the deferred count read back, or, within a step (list != null,
+  // counter zeroed by the step's memset) of at most 1 M points, by n with
+  // no read-back (the plan reads the count on the device).
   auto kentry = [&](uint32_t n, const uint32_t *list, uint8_t *flags) {
     uint32_t *nlp = d_n + 6;
     if (!list) C2_HIP(hipMemsetAsync(nlp, 0, sizeof(uint32_t), st));
     C2_LAUNCH(ctx, k_kentry, dim3(grid_for(n, B)), dim3(B), 0, P, n, list, L, uni, Kt, flags, longq, nlp);
-    const uint32_t nb = list ? n : read_scalar(ctx, nlp);
+    const uint32_t nb = list && n <= (1u << 20) ? n : read_scalar(ctx, nlp);
     if (nb == 0) return;
     uint64_t *lbw = buf<uint64_t>(ctx, "kt_lb", 6 * (size_t)nb);
     uint32_t *cpre = buf<uint32_t>(ctx, "kt_cpre", 7 * (size_t)nb);
