@@ -134,7 +134,7 @@ __global__ void k_step_dirs(BParams P, uint32_t N, double4 *__restrict__ G) {
   int p[3];
   coords(P, s, p[0], p[1], p[2]);
   double g[3];
-  G[s] = step_dir(P, p, g) ? make_double4(g[0], g[1], g[2], 0.0) : make_double4(0.0, 0.0, 0.0, 1.0);
+  G[s] = step_dir(P, p, g) ? make_double4(g[0], g[1], g[2], P.f[s]) : make_double4(0.0, 0.0, 0.0, P.f[s]);
 }
 
 // Pool -> CSR in start order (toff = exclusive scan of len): REORDER_LANES
@@ -201,7 +201,8 @@ __global__ void k_step_pointers(BParams P, uint32_t N, int64_t *ongrid, int64_t 
     double dr[3] = {0.0, 0.0, 0.0};
     int pm[3];
     int64_t tgt = s;
-    if (step_near_candidate(P, p, dr, pm)) {
+    const double4 t = P.G ? P.G[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if (step_near_candidate(P, p, t, dr, pm)) {
       if (pm[0] == p[0] && pm[1] == p[1] && pm[2] == p[2]) {  // known(pm)==1 (pm == p)
         step_ongrid(P, pm);
         dr[0] = dr[1] = dr[2] = 0.0;

@@ -15,8 +15,9 @@ struct BParams {
   double inv1, inv2, inv3;  // 1/n1, 1/n2, 1/n3 (index arithmetic)
   double c[9];    // car2lat, column-major
   double lid[27]; // lat_i_dist
-  // when set: each point's normalised step direction (x, y, z; w = 1 where
-  // max|g| < 1e-30), so a step reads one record instead of seven values
+  // when set: each point's normalised step direction (x, y, z; all 0 where
+  // max|g| < 1e-30) and value (w), so a step reads one record instead of
+  // seven values
   const double4 *__restrict__ G;
   // when set: positions 1..TPK of every free trajectory, position-major
   // (tpk[(q-1)*npts + s]), so the lanes of a wavefront (neighbouring starts)
@@ -162,16 +163,18 @@ __device__ __forceinline__ bool step_dir(const BParams &P, const int p[3], doubl
   return true;
 }
 
-__device__ inline bool step_near_candidate(const BParams &P, const int p[3], double dr[3],
+// t: p's record of the step direction table when the table is set (a
+// point with no step direction has g = 0; a moving point has a component
+// of magnitude ~1 after the normalisation).
+__device__ inline bool step_near_candidate(const BParams &P, const int p[3], const double4 &t, double dr[3],
                                            int pm[3]) {
   double g[3];
   bool moving;
   if (P.G) {
-    const double4 t = P.G[lin3(P, p[0], p[1], p[2])];
     g[0] = t.x;
     g[1] = t.y;
     g[2] = t.z;
-    moving = t.w == 0.0;
+    moving = g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0;
   } else {
     moving = step_dir(P, p, g);
   }
@@ -261,6 +264,11 @@ struct RingPath {
 };
 
 template <class Stop, class Visit, class Path = FlatPath>
+// A candidate whose value exceeds every value on the path so far cannot be
+// on it: the membership filter and the exact scan run only otherwise (a
+// path mostly climbs, and a long path saturates the filter).  The table
+// record of the candidate (its step direction and value) is loaded once
+// and carried to the next step.
 __device__ inline int walk_path(const BParams &P, uint32_t start, Path path, int maxlen, Stop stop,
                                 Visit visit) {
   int p[3];
@@ -271,12 +279,16 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, Path path, int
   PathFilter filt;
   filt.add(start);
   visit(start);
+  double4 rec = P.G ? P.G[start] : make_double4(0.0, 0.0, 0.0, 0.0);
+  double fmax = P.G ? rec.w : P.f[start];
   for (;;) {
     int pm[3];
-    if (!step_near_candidate(P, p, dr, pm)) break;  // maximum: no move
+    if (!step_near_candidate(P, p, rec, dr, pm)) break;  // maximum: no move
     uint32_t lm = lin3(P, pm[0], pm[1], pm[2]);
+    double4 recm = P.G ? P.G[lm] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double fm = P.G ? recm.w : P.f[lm];
     bool seen = false;
-    if (filt.maybe(lm))
+    if (!(fm > fmax) && filt.maybe(lm))
       for (int q = len - 1; q >= 0; q--)
         if (path.get(q, len) == lm) {
           seen = true;
@@ -289,6 +301,8 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, Path path, int
       step_ongrid(P, pm);
       dr[0] = dr[1] = dr[2] = 0.0;
       lm = lin3(P, pm[0], pm[1], pm[2]);
+      if (P.G) recm = P.G[lm];
+      fm = P.G ? recm.w : P.f[lm];
     }
     if (pm[0] == p[0] && pm[1] == p[1] && pm[2] == p[2]) break;  // no move
     if (len >= maxlen) return -1;
@@ -298,6 +312,8 @@ __device__ inline int walk_path(const BParams &P, uint32_t start, Path path, int
     p[0] = pm[0];
     p[1] = pm[1];
     p[2] = pm[2];
+    rec = recm;
+    fmax = fm > fmax ? fm : fmax;
     if (stop(lm)) break;
   }
   return len;
