@@ -522,17 +522,26 @@ __global__ void k_ev_sort(uint32_t n, const uint32_t *__restrict__ list, Lists L
                           uint32_t *__restrict__ hugelist, uint32_t *nhuge, unsigned long long *work) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t sorted = 0;
-  if (t < n) {
-    const uint32_t y = list ? list[t] : t;
-    const uint64_t b = L.b(y), e = L.e(y);
-    const uint64_t m = e - b;
-    if (m > SORT_LDS) {
-      hugelist[atomicAdd(nhuge, 1u)] = y;
-    } else if (m > SORT_WAVE) {
-      longlist[atomicAdd(nlong, 1u)] = y;
-    } else if (m > SORT_INLINE) {
-      longlist[n - 1 - atomicAdd(nwave, 1u)] = y;
-    } else {
+  const uint32_t y = t < n ? (list ? list[t] : t) : 0u;
+  const uint64_t m = t < n ? (uint64_t)L.cnt[y] : 0;
+  // the queues take one atomic per wavefront each
+  wave_append(m > SORT_LDS, y, hugelist, nhuge);
+  wave_append(m > SORT_WAVE && m <= SORT_LDS, y, longlist, nlong);
+  {
+    const bool w = m > SORT_INLINE && m <= SORT_WAVE;
+    const uint64_t mask = __ballot(w);
+    if (mask) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(nwave, (uint32_t)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      if (w) longlist[n - 1 - (base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)))] = y;
+    }
+  }
+  if (t < n && m <= SORT_INLINE) {
+    const uint64_t b = L.b(y), e = b + m;
+    {
       uint64_t *ev = L.ev;
       uint64_t r[SORT_INLINE];
 #pragma unroll
@@ -1141,6 +1150,7 @@ __global__ void __launch_bounds__(256) k_patch_keep_long(BParams P, const uint32
                                                          unsigned long long *work) {
   __shared__ uint32_t lds4[4];
   const uint32_t nl = *nlong;
+  uint64_t wold = 0, wkept = 0;  // this workgroup's work, added once at the end
   for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
     const uint32_t t = longq[q];
     const uint32_t y = dl[t];
@@ -1177,12 +1187,16 @@ __global__ void __launch_bounds__(256) k_patch_keep_long(BParams P, const uint32
       L.off[y] = nb;
       L.cnt[y] = lim;
       if (move) L.cap[y] = room;
-      if (work) {
-        atomicAdd(work, (unsigned long long)(oe - ob));
-        atomicAdd(work + 1, (unsigned long long)kept);
-      }
+      wold += oe - ob;
+      wkept += kept;
     }
     __syncthreads();
+  }
+  // one slotted add per workgroup (a shared address per list serialises)
+  if (work && threadIdx.x == 0 && wold) {
+    unsigned long long *w = work + CSTRIDE * (blockIdx.x & (CSLOTS - 1));
+    atomicAdd(w, (unsigned long long)wold);
+    atomicAdd(w + 1, (unsigned long long)wkept);
   }
 }
 
