@@ -1318,8 +1318,11 @@ constexpr uint64_t HUGE_BATCH = 1ull << 27;
 
 // Sorts the lists of the points in `list` (all points when null).
 // d_n: three counters (long, huge, wavefront lists), zero on entry.
+// end_bit: the events' time bits end there (the radix sort of the very long
+// lists orders the time bits only: a list's times are distinct).
 static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const Lists &L, uint8_t *uni,
-                       uint32_t *longlist, uint32_t *hugelist, unsigned long long *work, uint32_t *d_n) {
+                       uint32_t *longlist, uint32_t *hugelist, unsigned long long *work, uint32_t *d_n,
+                       int end_bit) {
   hipStream_t st = ctx->stream;
   if (n == 0) return;
   C2_LAUNCH(ctx, k_ev_sort, dim3(grid_for(n, 256)), dim3(256), 0, n, list, L, uni, longlist, d_n, d_n + 2,
@@ -1357,10 +1360,10 @@ static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const L
       C2_LAUNCH(ctx, k_huge_gather, dim3(nb), dim3(256), 0, hugelist + q0, L, boff, keys);
       size_t tmp = 0;
       C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, keys, alt, (int)tot, (int)nb, boff, boff + 1,
-                                                        0, 64, st));
+                                                        LAB_BITS, end_bit, st));
       void *t = buffer(ctx, "sweep_segsort_tmp", tmp);
-      C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(t, tmp, keys, alt, (int)tot, (int)nb, boff, boff + 1, 0, 64,
-                                                        st));
+      C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(t, tmp, keys, alt, (int)tot, (int)nb, boff, boff + 1,
+                                                        LAB_BITS, end_bit, st));
       C2_LAUNCH(ctx, k_huge_copyback, dim3(nb), dim3(256), 0, hugelist + q0, L, boff, alt, uni);
       // the host offsets are rewritten by the next batch: wait for this one's copy
       C2_HIP(hipStreamSynchronize(st));
@@ -1436,6 +1439,9 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   C2_HIP(hipMemsetAsync(delta, 0, sizeof(int32_t) * N, st));
   for (uint8_t *f : {chg, ktchg, dflag, qflag, sflag}) C2_HIP(hipMemsetAsync(f, 0, NF, st));
 
+  // events carry their time (sweep index, then position) above the label
+  int sort_end_bit = LAB_BITS + POS_BITS;
+  while (sort_end_bit < 64 && (1ull << (sort_end_bit - LAB_BITS - POS_BITS)) < (uint64_t)N) sort_end_bit++;
   Lists L;
   L.off = offN;
   L.cnt = cntN;
@@ -1501,7 +1507,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
               (const uint16_t *)nullptr, L, cur, d_err, d_work + 4, (int32_t *)nullptr, (int32_t *)nullptr,
               (uint8_t *)nullptr);
     C2_HIP(hipMemsetAsync(d_n + 7, 0, 3 * sizeof(uint32_t), st));
-    sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8, d_n + 7);
+    sort_lists(ctx, N, nullptr, L, uni, longlist, hugelist, d_work + 8, d_n + 7, sort_end_bit);
     kentry(N, nullptr, nullptr);
     R.events_total += (int64_t)nev;
     n_full += 1;
@@ -1614,7 +1620,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
               d_work + 4, ell, lam, chg);
     s_nd += nd;
     pool_top += added;
-    sort_lists(ctx, nd, dl, L, uni, longlist, hugelist, d_work + 8, d_n + 7);
+    sort_lists(ctx, nd, dl, L, uni, longlist, hugelist, d_work + 8, d_n + 7, sort_end_bit);
     // 3. known==2 times around the rewritten lists
     C2_LAUNCH(ctx, k_mark_kt, dim3(gD), dim3(B), 0, P, nd, dl, qflag);
     const uint32_t nq = collect(qflag, 1, ql, d_n + 2);
