@@ -592,38 +592,101 @@ __global__ void __launch_bounds__(256) k_ev_sort_wave(const uint32_t *__restrict
   if (lane == 0) uni[y] = diff == 0;
 }
 
-// One workgroup per long list: bitonic sort in LDS.
+// One workgroup per long list, in LDS.  A rewritten list is its kept
+// events (still in order) followed by the new ones: up to its first descent
+// k it is sorted.  When at most 256 events follow, they alone are sorted
+// (bitonic) and merged with the first k (each event's place is its index
+// plus the count of the other run's smaller events, a binary search);
+// otherwise the whole list is sorted (bitonic).
 __global__ void __launch_bounds__(256) k_ev_sort_long(const uint32_t *__restrict__ longlist, Lists L,
                                                       uint8_t *__restrict__ uni) {
   __shared__ uint64_t sh[SORT_LDS];
+  __shared__ uint64_t tl[256];
   __shared__ int notuni;
+  __shared__ uint32_t kd;
   const uint32_t y = longlist[blockIdx.x];
   const uint64_t b = L.b(y), n = L.cnt[y];
   uint32_t m = 1;
   while (m < n) m <<= 1;
   for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) sh[i] = (i < n) ? L.ev[b + i] : ~0ull;
-  if (threadIdx.x == 0) notuni = 0;
-  __syncthreads();
-  for (uint32_t k = 2; k <= m; k <<= 1)
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const bool up = ((i & k) == 0);
-          const uint64_t a = sh[i], c = sh[ixj];
-          if ((a > c) == up) {
-            sh[i] = c;
-            sh[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    L.ev[b + i] = sh[i];
-    if (ev_lab(sh[i]) != ev_lab(sh[0])) notuni = 1;
+  if (threadIdx.x == 0) {
+    notuni = 0;
+    kd = (uint32_t)n;
   }
   __syncthreads();
+  for (uint32_t i = threadIdx.x + 1; i < n; i += blockDim.x)
+    if (sh[i] < sh[i - 1]) atomicMin(&kd, i);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    if (ev_lab(sh[i]) != ev_lab(sh[0])) notuni = 1;
+  __syncthreads();
+  const uint32_t k = kd, t = (uint32_t)n - k;
+  if (t == 0) {
+    // already in order
+  } else if (t <= 256) {
+    uint32_t tp = 1;
+    while (tp < t) tp <<= 1;
+    if (threadIdx.x < tp) tl[threadIdx.x] = threadIdx.x < t ? sh[k + threadIdx.x] : ~0ull;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= tp; kk <<= 1)
+      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+        const uint32_t i = threadIdx.x;
+        if (i < tp) {
+          const uint32_t ixj = i ^ j;
+          if (ixj > i) {
+            const bool up = ((i & kk) == 0);
+            const uint64_t a = tl[i], c = tl[ixj];
+            if ((a > c) == up) {
+              tl[i] = c;
+              tl[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    // the kept run: index + tail events below it
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+      const uint64_t v = sh[i];
+      uint32_t lo = 0, hi = t;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tl[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      L.ev[b + i + lo] = v;
+    }
+    // the tail: index + kept events below it
+    if (threadIdx.x < t) {
+      const uint64_t v = tl[threadIdx.x];
+      uint32_t lo = 0, hi = k;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sh[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      L.ev[b + threadIdx.x + lo] = v;
+    }
+  } else {
+    for (uint32_t kk = 2; kk <= m; kk <<= 1)
+      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+          const uint32_t ixj = i ^ j;
+          if (ixj > i) {
+            const bool up = ((i & kk) == 0);
+            const uint64_t a = sh[i], c = sh[ixj];
+            if ((a > c) == up) {
+              sh[i] = c;
+              sh[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) L.ev[b + i] = sh[i];
+  }
   if (threadIdx.x == 0) uni[y] = notuni ? 0 : 1;
 }
 
