@@ -1587,9 +1587,10 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   // rounds of the approximate first story: the bulk of what the early
   // fixpoint steps would correct, at a fraction of their cost (512^3: the
   // fixpoint from 2.64 s to 1.03 s with 32 rounds, 73 steps instead of 104;
-  // 768^3: 10.4 s to 3.7 s); the best count grows with the grid edge
-  const int nmax = std::max(P.n1, std::max(P.n2, P.n3));
-  int pre_rounds = std::min(32, std::max(8, nmax / 16));
+  // 768^3: 10.4 s to 3.7 s).  32 is best at 512^3 (20: +11%, 40: +3%) and
+  // on cells of 48-160 points an edge too, where a round costs far less
+  // than a fixpoint step (8-10 rounds: +27%, 24: +2%, 48: +3%)
+  int pre_rounds = 32;
   if (const char *pr = std::getenv("C2GPU_SWEEP_PRE")) pre_rounds = std::atoi(pr);
   // the rounds follow each start's path for at most 32 points: longer ones
   // (rare, and mostly in the early rounds when the known == 2 times are
