@@ -721,6 +721,116 @@ __global__ void k_huge_copyback(const uint32_t *__restrict__ hugelist, Lists L, 
   if (threadIdx.x == 0) uni[y] = notuni ? 0 : 1;
 }
 
+// A rewritten very long list is its kept events (still in order) followed
+// by the new ones, few next to millions.  Such a list is merged instead of
+// radix-sorted: its first descent k ends the sorted run, the tail after it
+// (at most HUGE_TAIL events, else the batch takes the radix sort) is sorted
+// in LDS, and every event goes to its index in its run plus the count of
+// the other run's smaller events (distinct times: a permutation).
+constexpr uint32_t HUGE_TAIL = 4096;
+
+__global__ void __launch_bounds__(256) k_huge_desc(const uint64_t *__restrict__ boff,
+                                                   const uint64_t *__restrict__ keys, uint32_t *__restrict__ kd,
+                                                   uint32_t *bigtail) {
+  __shared__ uint32_t m;
+  const uint64_t o = boff[blockIdx.x], n = boff[blockIdx.x + 1] - o;
+  if (threadIdx.x == 0) m = (uint32_t)n;
+  __syncthreads();
+  for (uint64_t i = threadIdx.x + 1; i < n; i += blockDim.x)
+    if (keys[o + i] < keys[o + i - 1]) atomicMin(&m, (uint32_t)i);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    kd[blockIdx.x] = m;
+    if (n - m > HUGE_TAIL) atomicOr(bigtail, 1u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_huge_tail_sort(const uint64_t *__restrict__ boff,
+                                                        const uint64_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ kd, uint64_t *__restrict__ tails) {
+  __shared__ uint64_t sh[HUGE_TAIL];
+  const uint64_t o = boff[blockIdx.x], n = boff[blockIdx.x + 1] - o;
+  const uint32_t k = kd[blockIdx.x], t = (uint32_t)(n - k);
+  if (t == 0) return;  // the whole workgroup
+  uint32_t m = 1;
+  while (m < t) m <<= 1;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) sh[i] = i < t ? keys[o + k + i] : ~0ull;
+  __syncthreads();
+  for (uint32_t kk = 2; kk <= m; kk <<= 1)
+    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = ((i & kk) == 0);
+          const uint64_t a = sh[i], c = sh[ixj];
+          if ((a > c) == up) {
+            sh[i] = c;
+            sh[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (uint32_t i = threadIdx.x; i < t; i += blockDim.x) tails[o + k + i] = sh[i];
+}
+
+// Every event of the batch to its merged place in its list (L.ev; the runs
+// are read from the gathered copies); notuni[q] is set when list q holds
+// more than one label.
+__global__ void k_huge_merge(uint32_t nb, const uint32_t *__restrict__ hugelist, Lists L,
+                             const uint64_t *__restrict__ boff, const uint64_t *__restrict__ keys,
+                             const uint64_t *__restrict__ tails, const uint32_t *__restrict__ kd,
+                             uint32_t *__restrict__ notuni) {
+  const uint64_t total = boff[nb];
+  for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total;
+       x += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nb;  // the list: the last q with boff[q] <= x
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (boff[mid] <= x)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t q = lo;
+    const uint64_t o = boff[q], r = x - o, n = boff[q + 1] - o;
+    const uint32_t k = kd[q];
+    const uint64_t t = n - k;
+    uint64_t v, pos;
+    if (r < k) {
+      v = keys[x];
+      uint64_t a = 0, b = t;  // tail events below v
+      while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if (tails[o + k + mid] < v)
+          a = mid + 1;
+        else
+          b = mid;
+      }
+      pos = r + a;
+    } else {
+      v = tails[x];
+      uint64_t a = 0, b = k;  // kept events below v
+      while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if (keys[o + mid] < v)
+          a = mid + 1;
+        else
+          b = mid;
+      }
+      pos = (r - k) + a;
+    }
+    if (t) L.ev[L.b(hugelist[q]) + pos] = v;  // (no tail: the list is in order already)
+    if (ev_lab(v) != ev_lab(keys[o])) notuni[q] = 1;
+  }
+}
+
+__global__ void k_huge_uni(uint32_t nb, const uint32_t *__restrict__ hugelist, const uint32_t *__restrict__ notuni,
+                           uint8_t *__restrict__ uni) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nb) uni[hugelist[q]] = notuni[q] ? 0 : 1;
+}
+
 // ---------------------------------------------------------------- known == 2
 
 // The six face neighbours of q (periodic).
@@ -1421,6 +1531,19 @@ static void sort_lists(c2gpu_ctx *ctx, uint32_t n, const uint32_t *list, const L
       C2_HIP(hipMemcpyAsync(boff, bo.data(), 8 * ((size_t)nb + 1), hipMemcpyHostToDevice, st));
       uint64_t *keys = buf<uint64_t>(ctx, "sweep_hugekeys", tot), *alt = buf<uint64_t>(ctx, "sweep_hugealt", tot);
       C2_LAUNCH(ctx, k_huge_gather, dim3(nb), dim3(256), 0, hugelist + q0, L, boff, keys);
+      // a sorted run and a short tail each: merge (see k_huge_desc)
+      uint32_t *kd = buf<uint32_t>(ctx, "sweep_hugekd", nb), *hf = buf<uint32_t>(ctx, "sweep_hugeflag", nb + 1);
+      C2_HIP(hipMemsetAsync(hf, 0, 4 * ((size_t)nb + 1), st));
+      C2_LAUNCH(ctx, k_huge_desc, dim3(nb), dim3(256), 0, boff, keys, kd, hf);
+      if (read_scalar(ctx, hf) == 0) {
+        C2_LAUNCH(ctx, k_huge_tail_sort, dim3(nb), dim3(256), 0, boff, keys, kd, alt);
+        C2_LAUNCH(ctx, k_huge_merge, dim3(std::min<uint64_t>((tot + 255) / 256, 8192)), dim3(256), 0, nb,
+                  hugelist + q0, L, boff, keys, alt, kd, hf + 1);
+        C2_LAUNCH(ctx, k_huge_uni, dim3(grid_for(nb, 256)), dim3(256), 0, nb, hugelist + q0, hf + 1, uni);
+        C2_HIP(hipStreamSynchronize(st));
+        q0 = q1;
+        continue;
+      }
       size_t tmp = 0;
       C2_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, keys, alt, (int)tot, (int)nb, boff, boff + 1,
                                                         LAB_BITS, end_bit, st));
