@@ -74,7 +74,7 @@ __device__ __forceinline__ uint32_t event_start(const BParams &P, uint64_t e) {
 
 // The per-point event lists.
 // Lists longer than this are filtered / scanned by a whole workgroup.
-constexpr uint32_t LIST_LONG = 64;
+constexpr uint32_t LIST_LONG = 32;
 
 struct Lists {
   uint64_t *off;
