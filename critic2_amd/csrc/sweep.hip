@@ -1000,7 +1000,7 @@ __global__ void k_kentry(BParams P, uint32_t n, const uint32_t *__restrict__ lis
 // the point's time to its first agreement.  The minimum over the items is
 // the first agreement in the merged order, the serial merge's answer.
 
-constexpr uint64_t KT_CHUNK = 8;
+constexpr uint64_t KT_CHUNK = 4;
 
 // Per deferred point w: its lists' first candidates (lb) and the prefix
 // of its item counts over the six lists (cpre, 7 entries); cnt[w] = items.
