@@ -847,7 +847,7 @@ __device__ __forceinline__ void face_neighbours(const BParams &P, uint32_t q, ui
 
 // Candidate events above which a point's merge is cut into work items
 // (k_kt_items) instead of running on one thread.
-constexpr uint64_t KT_LONG = 8;
+constexpr uint64_t KT_LONG = 4;
 
 // First index at or after pos (in [pos, end)) whose event time exceeds c:
 // a gallop from pos, then a binary search (cursors that jump far over a
