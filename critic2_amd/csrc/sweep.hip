@@ -303,24 +303,34 @@ __device__ __forceinline__ uint32_t sweep_index32(const BParams &P, uint32_t l) 
   return (uint32_t)sweep_index(P, l);
 }
 
+// The rounds keep each point's known == 2 time and covering time in one
+// 64-bit word (time low, covering time high), so a path point costs one
+// gather; the covering time is lowered with a 32-bit atomic on its half.
+__device__ __forceinline__ uint64_t pre_word(uint32_t kt, uint32_t c) { return ((uint64_t)c << 32) | kt; }
+__device__ __forceinline__ uint32_t pre_kt(uint64_t w) { return (uint32_t)w; }
+__device__ __forceinline__ uint32_t pre_c(uint64_t w) { return (uint32_t)(w >> 32); }
+__device__ __forceinline__ uint32_t *pre_c_ptr(uint64_t *W, uint32_t x) {
+  return reinterpret_cast<uint32_t *>(W + x) + 1;
+}
+
 __global__ void k_pre_flab(uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
                            const int32_t *__restrict__ maxlab, int32_t *__restrict__ flab,
-                           uint32_t *__restrict__ C, BParams P) {
+                           uint64_t *__restrict__ W, BParams P) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   flab[s] = maxlab[tp[toff[s + 1] - 1]];
-  C[s] = sweep_index32(P, s);
+  W[s] = pre_word(PRE_INF, sweep_index32(P, s));
 }
 
 // Covering by every point's first step (the plain first story's paths of
 // two points), a middle ground to start the rounds from.
 __global__ void k_pre_seed(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                           uint32_t *__restrict__ C) {
+                           uint64_t *__restrict__ W) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const uint64_t o = toff[s];
   if (toff[s + 1] - o < 2) return;
-  atomicMin(&C[traj_at(P, s, tp, o, 1)], sweep_index32(P, s));
+  atomicMin(pre_c_ptr(W, traj_at(P, s, tp, o, 1)), sweep_index32(P, s));
 }
 
 // Whether a point and its six face neighbours carry one positive free
@@ -338,40 +348,38 @@ __global__ void k_pre_agree(BParams P, uint32_t N, const int32_t *__restrict__ f
   agree[s] = same;
 }
 
-__global__ void k_pre_kt(BParams P, uint32_t N, const uint32_t *__restrict__ C, const uint8_t *__restrict__ agree,
-                         uint32_t *__restrict__ Kt, uint32_t *__restrict__ Cnext, const uint32_t *__restrict__ Cold,
-                         unsigned long long *nchanged) {
+// Win: this round's covering times (high halves).  Wout[s] = the next
+// round's known == 2 time from them, and its covering time starting at the
+// point's own time (a point no path covers is a start and covers itself).
+// nchanged (diagnostics) compares with Wout's previous covering times.
+__global__ void k_pre_kt(BParams P, uint32_t N, const uint64_t *__restrict__ Win, const uint8_t *__restrict__ agree,
+                         uint64_t *__restrict__ Wout, unsigned long long *nchanged) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nchanged) wave_add(nchanged, s < N && C[s] != Cold[s] ? 1 : 0);
+  if (nchanged) wave_add(nchanged, s < N && pre_c(Win[s]) != pre_c(Wout[s]) ? 1 : 0);
   if (s >= N) return;
-  // the next round's covering times start at each point's own time (a point
-  // no path covers is a start and covers itself)
-  Cnext[s] = sweep_index32(P, s);
-  if (!agree[s]) {
-    Kt[s] = PRE_INF;
-    return;
-  }
-  int i, j, k;
-  coords(P, s, i, j, k);
-  uint32_t m = C[s];
-  const uint32_t z[6] = {lin3(P, wrap1(i + 1, P.n1), j, k), lin3(P, wrap1(i - 1, P.n1), j, k),
-                         lin3(P, i, wrap1(j + 1, P.n2), k), lin3(P, i, wrap1(j - 1, P.n2), k),
-                         lin3(P, i, j, wrap1(k + 1, P.n3)), lin3(P, i, j, wrap1(k - 1, P.n3))};
+  uint32_t m = PRE_INF;
+  if (agree[s]) {
+    int i, j, k;
+    coords(P, s, i, j, k);
+    m = pre_c(Win[s]);
+    const uint32_t z[6] = {lin3(P, wrap1(i + 1, P.n1), j, k), lin3(P, wrap1(i - 1, P.n1), j, k),
+                           lin3(P, i, wrap1(j + 1, P.n2), k), lin3(P, i, wrap1(j - 1, P.n2), k),
+                           lin3(P, i, j, wrap1(k + 1, P.n3)), lin3(P, i, j, wrap1(k - 1, P.n3))};
 #pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const uint32_t c = C[z[q]];
-    m = c > m ? c : m;
+    for (int q = 0; q < 6; q++) {
+      const uint32_t c = pre_c(Win[z[q]]);
+      m = c > m ? c : m;
+    }
   }
-  Kt[s] = m;
+  Wout[s] = pre_word(m, sweep_index32(P, s));
 }
 
 __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                            const uint32_t *__restrict__ Cold, const uint32_t *__restrict__ Kt,
-                            uint32_t *__restrict__ Cnew, int cap) {
+                            const uint64_t *__restrict__ Wold, uint64_t *__restrict__ W, int cap) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const uint32_t ts = sweep_index32(P, s);
-  if (Cold[s] < ts) return;  // covered before its turn: not a start
+  if (pre_c(Wold[s]) < ts) return;  // covered before its turn: not a start
   // Cnew starts at each point's own time, so the start itself needs no
   // update; a covering time only decreases, so a value already at or below
   // this path's needs no atomic either (most points lie on many paths)
@@ -381,34 +389,35 @@ __global__ void k_pre_cover(BParams P, uint32_t N, const uint64_t *__restrict__ 
   // PRE_CHUNK path points at a time: their indices and Kt are loaded
   // together (independent loads), then the stop is found among them
   for (int q0 = 1; q0 < len; q0 += PRE_CHUNK) {
-    uint32_t x[PRE_CHUNK], kt[PRE_CHUNK], cv[PRE_CHUNK];
+    uint32_t x[PRE_CHUNK];
+    uint64_t w[PRE_CHUNK];
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
-    // Kt and the current covering times of the chunk loaded together; a
-    // stale (larger) covering time only costs an atomic that changes nothing
+    // each path point's Kt and current covering time in one load; a stale
+    // (larger) covering time only costs an atomic that changes nothing
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++) {
-      kt[u] = q0 + u < len ? Kt[x[u]] : 0u;
-      cv[u] = q0 + u < len ? __hip_atomic_load(&Cnew[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-    }
+    for (int u = 0; u < PRE_CHUNK; u++)
+      w[u] = q0 + u < len ? __hip_atomic_load(&W[x[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
     bool stop = false;
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) {
       if (stop || q0 + u >= len) break;
-      if (ts < cv[u]) atomicMin(&Cnew[x[u]], ts);
-      stop = kt[u] < ts;
+      if (ts < pre_c(w[u])) atomicMin(pre_c_ptr(W, x[u]), ts);
+      stop = pre_kt(w[u]) < ts;
     }
     if (stop) break;
   }
 }
 
+// WC: the final covering times (high halves); WK: the known == 2 times
+// from them (low halves).
 __global__ void k_pre_story(BParams P, uint32_t N, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tp,
-                            const uint32_t *__restrict__ C, const uint32_t *__restrict__ Kt,
+                            const uint64_t *__restrict__ WC, const uint64_t *__restrict__ WK,
                             const int32_t *__restrict__ flab, int32_t *__restrict__ ell, int32_t *__restrict__ lam) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const uint32_t ts = sweep_index32(P, s);
-  if (C[s] < ts) {
+  if (pre_c(WC[s]) < ts) {
     ell[s] = 0;
     lam[s] = 0;
     return;
@@ -421,7 +430,7 @@ __global__ void k_pre_story(BParams P, uint32_t N, const uint64_t *__restrict__ 
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++) x[u] = q0 + u < len ? traj_at(P, s, tp, o, q0 + u) : 0u;
 #pragma unroll
-    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? Kt[x[u]] : PRE_INF;
+    for (int u = 0; u < PRE_CHUNK; u++) kt[u] = q0 + u < len ? pre_kt(WK[x[u]]) : PRE_INF;
 #pragma unroll
     for (int u = 0; u < PRE_CHUNK; u++)
       if (e == len && kt[u] < ts) e = q0 + u + 1;
@@ -1721,17 +1730,19 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   // uncapped; 8 points: 1289 ms)
   const int pre_cap = 32;
   if (pre_rounds > 0) {
-    // buffers the first full build overwrites: flab in cur, C in offN / aoff
+    // buffers the first full build overwrites: flab in cur, the rounds'
+    // (known == 2 time, covering time) words in offN / aoff
     int32_t *flab = reinterpret_cast<int32_t *>(cur);
-    uint32_t *Ca = reinterpret_cast<uint32_t *>(offN), *Cb = reinterpret_cast<uint32_t *>(aoff);
-    uint32_t *Kp = alloc;  // the rounds' known == 2 times (sweep indices)
-    C2_LAUNCH(ctx, k_pre_flab, dim3(gN), dim3(B), 0, N, toff, tp, maxlab, flab, Ca, P);
+    uint64_t *Wa = offN, *Wb = aoff;
+    C2_LAUNCH(ctx, k_pre_flab, dim3(gN), dim3(B), 0, N, toff, tp, maxlab, flab, Wb, P);
     // start from every point's first step covered (512^3: 900 ms of
     // fixpoint and rounds against 923 ms from own times only)
-    C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca);
+    C2_LAUNCH(ctx, k_pre_seed, dim3(gN), dim3(B), 0, P, N, toff, tp, Wb);
     uint8_t *agree = uni;  // rewritten by the first full build
     C2_LAUNCH(ctx, k_pre_agree, dim3(gN), dim3(B), 0, P, N, flab, agree);
-    C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Ca, agree, Kp, Cb, (const uint32_t *)nullptr,
+    // Wb: the covering times a round starts from; Wa: Kt and the covering
+    // times it builds
+    C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, (const uint64_t *)Wb, agree, Wa,
               (unsigned long long *)nullptr);
     unsigned long long *d_pc = buf<unsigned long long>(ctx, "sweep_precount", CSLOTS * CSTRIDE);
     const bool count = trace;  // covering times changed per round (diagnostics)
@@ -1740,16 +1751,18 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
       // the first rounds, whose known == 2 times are still late, follow 8
       // points only (512^3: 773 ms of rounds and fixpoint against 813 ms)
       const int cap_r = r < 8 ? 8 : pre_cap;
-      C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kp, Cb, cap_r);
-      // Cb now holds this round's covering times; Ca is reset for the next
-      C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, Cb, agree, Kp, Ca, (const uint32_t *)Ca,
+      C2_LAUNCH(ctx, k_pre_cover, dim3(gN), dim3(B), 0, P, N, toff, tp, (const uint64_t *)Wb, Wa, cap_r);
+      // Wa now holds this round's covering times; Wb gets the next round's
+      // Kt and starting covering times
+      C2_LAUNCH(ctx, k_pre_kt, dim3(gN), dim3(B), 0, P, N, (const uint64_t *)Wa, agree, Wb,
                 count ? d_pc : (unsigned long long *)nullptr);
-      std::swap(Ca, Cb);
+      std::swap(Wa, Wb);
       if (count)
         std::fprintf(stderr, "[sweep] pre round %d: %llu covering times changed\n", r,
                      (unsigned long long)sum_slots(ctx, d_pc, 0));
     }
-    C2_LAUNCH(ctx, k_pre_story, dim3(gN), dim3(B), 0, P, N, toff, tp, Ca, Kp, flab, ell, lam);
+    C2_LAUNCH(ctx, k_pre_story, dim3(gN), dim3(B), 0, P, N, toff, tp, (const uint64_t *)Wb, (const uint64_t *)Wa,
+              flab, ell, lam);
   } else {
     C2_LAUNCH(ctx, k_story_init, dim3(gN), dim3(B), 0, N, toff, tp, maxlab, ell, lam);
   }
