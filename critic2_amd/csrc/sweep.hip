@@ -1654,8 +1654,7 @@ SweepResult sweep_run(c2gpu_ctx *ctx, const BParams &P, uint32_t N, const uint64
   };
 
   // known==2 times of the points in list (every point when null), the long
-  // merges on wavefronts
-  // merges (work items).  The plan's buffers (96 bytes a point) are sized
+  // merges as work items.  The plan's buffers (96 bytes a point) are sized
   // by the deferred count read back, or, within a step (list != null,
   // counter zeroed by the step's memset) of at most 1 M points, by n with
   // no read-back (the plan reads the count on the device).
